@@ -1,0 +1,413 @@
+"""curfil_amd -- MI355X-native hot path of CURFIL (random-forest training and
+dense prediction on RGB-D images), exposed from libcurfil_hip.so through its
+C ABI (include/curfil_hip.h).
+
+The Python layer mirrors the reference's library surface for this path:
+TrainingConfiguration (random_tree.h:187-493), RandomForestImage train /
+predict / normalizeHistograms (random_forest_image.h:44-168), and the plugin
+seams ImageFeatureEvaluation::generateRandomFeatures / calculateScores and the
+counters of calculateFeatureResponsesAndHistograms.  All compute runs in the
+HIP library; importing this package fails if the library is not built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcurfil_hip.so")
+
+CF_OK = 0
+CF_ERR_ARG = -1
+CF_ERR_HIP = -2
+CF_ERR_RUNTIME = -3
+CF_ERR_UNSUPPORTED = -4
+CF_ERR_NOMEM = -5
+
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+
+SAMPLE_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("image", "<u4"), ("depth_mm", "<i4"),
+                         ("label", "u1"), ("flip", "u1"), ("pad0", "u1"), ("pad1", "u1")])
+NODE_DTYPE = np.dtype([("id", "<i4"), ("level", "<i4"), ("parent", "<i4"), ("left", "<i4"), ("right", "<i4"),
+                       ("num_samples", "<i4"), ("feature_id", "<i4"), ("threshold", "<f4"), ("score", "<f8"),
+                       ("feature", "i1", (11,)), ("is_leaf", "u1"), ("pad", "<i4")])
+assert SAMPLE_DTYPE.itemsize == 16 and NODE_DTYPE.itemsize == 56
+
+# feature parameter rows of the [11][F] SoA (random_tree_image.h:702-773)
+FEATURE_ROWS = ("type", "offset1X", "offset1Y", "offset2X", "offset2Y", "region1X", "region1Y",
+                "region2X", "region2Y", "channel1", "channel2")
+DEPTH, COLOR = 0, 1
+
+
+class CurfilError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class _TrainConfig(ctypes.Structure):
+    _fields_ = [("random_seed", ctypes.c_int32), ("samples_per_image", ctypes.c_uint32),
+                ("feature_count", ctypes.c_uint32), ("min_sample_count", ctypes.c_uint32),
+                ("max_depth", ctypes.c_int32), ("box_radius", ctypes.c_uint16), ("region_size", ctypes.c_uint16),
+                ("thresholds", ctypes.c_uint16), ("pad0", ctypes.c_uint16), ("num_threads", ctypes.c_int32),
+                ("max_images", ctypes.c_int32), ("use_depth_images", ctypes.c_int32),
+                ("horizontal_flipping", ctypes.c_int32), ("subsampling_pixel_uniform", ctypes.c_int32),
+                ("num_ignored", ctypes.c_int32), ("ignored_labels", ctypes.c_int32 * 8)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m curfil_amd.build` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, D, U32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32
+    sig = {
+        "cf_ctx_create": [I, ctypes.POINTER(P)], "cf_ctx_destroy": [P], "cf_ctx_synchronize": [P],
+        "cf_images_create": [P, I, I, I, ctypes.POINTER(P)], "cf_images_destroy": [P],
+        "cf_images_set_rgbd_u8": [P, I, P, P], "cf_images_set_rgbd_f32": [P, I, P, P],
+        "cf_images_set_labels": [P, I, P], "cf_images_get_integrals": [P, I, P, P, P],
+        "cf_generate_features": [P, P, P, I, I, I, I, I, I, I, P, P],
+        "cf_feature_responses": [P, P, P, I, P, I, P],
+        "cf_node_counters": [P, P, P, I, P, P, I, I, I, P],
+        "cf_scores": [P, P, P, I, I, I, P],
+        "cf_node_best_split": [P, P, P, I, P, P, I, I, I, P, P, P, P],
+        "cf_forest_train": [P, P, P, I, I, I, ctypes.POINTER(P)],
+        "cf_tree_train_from_samples": [P, P, P, I, I, I, P, I, I, ctypes.POINTER(P)],
+        "cf_forest_create": [P, I, P, P, P, I, P, ctypes.POINTER(P)],
+        "cf_forest_destroy": [P], "cf_forest_num_trees": [P], "cf_forest_num_classes": [P],
+        "cf_forest_tree_size": [P, I], "cf_forest_get_tree": [P, I, P, P],
+        "cf_forest_get_root_samples": [P, I, P, I], "cf_forest_get_prior": [P, P],
+        "cf_forest_train_stats": [P, P, P], "cf_forest_normalize": [P, D, I],
+        "cf_forest_get_posteriors": [P, I, P],
+        "cf_predict": [P, P, P, I, P, P], "cf_predict_batch": [P, P, P, P, I, P, P],
+        "cf_synth_rgbd": [U32, U32, I, I, I, I, D, P, P, P],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.cf_last_error.argtypes = [P]
+    lib.cf_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One HIP device + stream (cf_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        st = lib.cf_ctx_create(device, ctypes.byref(h))
+        if st != CF_OK:
+            raise CurfilError(st, f"cf_ctx_create(device={device}) failed")
+        self.h = h
+
+    def check(self, st):
+        if st != CF_OK:
+            raise CurfilError(st, lib.cf_last_error(self.h).decode())
+
+    def synchronize(self):
+        self.check(lib.cf_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib.cf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Images:
+    """A set of equally sized RGB-D images with device-resident integrals."""
+
+    def __init__(self, ctx, count, width, height):
+        self.ctx, self.count, self.width, self.height = ctx, count, width, height
+        h = ctypes.c_void_p()
+        ctx.check(lib.cf_images_create(ctx.h, count, width, height, ctypes.byref(h)))
+        self.h = h
+
+    def set_rgbd_u8(self, index, rgb, depth):
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        depth = np.ascontiguousarray(depth, dtype=np.uint16)
+        assert rgb.shape == (self.height, self.width, 3) and depth.shape == (self.height, self.width)
+        self.ctx.check(lib.cf_images_set_rgbd_u8(self.h, index, _ptr(rgb), _ptr(depth)))
+
+    def set_rgbd_f32(self, index, color, depth):
+        color = np.ascontiguousarray(color, dtype=np.float32)
+        depth = np.ascontiguousarray(depth, dtype=np.int32)
+        assert color.shape == (3, self.height, self.width) and depth.shape == (self.height, self.width)
+        self.ctx.check(lib.cf_images_set_rgbd_f32(self.h, index, _ptr(color), _ptr(depth)))
+
+    def set_labels(self, index, labels):
+        labels = np.ascontiguousarray(labels, dtype=np.uint8)
+        assert labels.shape == (self.height, self.width)
+        self.ctx.check(lib.cf_images_set_labels(self.h, index, _ptr(labels)))
+
+    def integrals(self, index):
+        col = np.empty((3, self.height, self.width), np.float32)
+        dep = np.empty((self.height, self.width), np.int32)
+        val = np.empty((self.height, self.width), np.int32)
+        self.ctx.check(lib.cf_images_get_integrals(self.h, index, _ptr(col), _ptr(dep), _ptr(val)))
+        return col, dep, val
+
+    def close(self):
+        if self.h:
+            lib.cf_images_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_samples(x, y, image, depth_mm, label, flip=0):
+    n = len(x)
+    s = np.zeros(n, SAMPLE_DTYPE)
+    s["x"], s["y"], s["image"], s["depth_mm"], s["label"], s["flip"] = x, y, image, depth_mm, label, flip
+    return s
+
+
+def generate_features(ctx, images, samples, seed, num_features, num_thresholds, box_radius, region_size,
+                      use_depth_images=True):
+    """ImageFeatureEvaluation::generateRandomFeatures (CPU generator semantics), on the device.
+    Returns (features int8 [11][F], thresholds float32 [T][F]), sorted by feature key."""
+    samples = np.ascontiguousarray(samples, SAMPLE_DTYPE)
+    feats = np.zeros((11, num_features), np.int8)
+    thr = np.zeros((num_thresholds, num_features), np.float32)
+    ctx.check(lib.cf_generate_features(ctx.h, images.h, _ptr(samples), len(samples), seed, num_features,
+                                       num_thresholds, box_radius, region_size, int(use_depth_images),
+                                       _ptr(feats), _ptr(thr)))
+    return feats, thr
+
+
+def feature_responses(ctx, images, samples, features):
+    samples = np.ascontiguousarray(samples, SAMPLE_DTYPE)
+    features = np.ascontiguousarray(features, np.int8)
+    F = features.shape[1]
+    out = np.zeros((F, len(samples)), np.float64)
+    ctx.check(lib.cf_feature_responses(ctx.h, images.h, _ptr(samples), len(samples), _ptr(features), F, _ptr(out)))
+    return out
+
+
+def node_counters(ctx, images, samples, features, thresholds, num_classes):
+    """counters [F][T][C][2] (left, right) of one node."""
+    samples = np.ascontiguousarray(samples, SAMPLE_DTYPE)
+    features = np.ascontiguousarray(features, np.int8)
+    thresholds = np.ascontiguousarray(thresholds, np.float32)
+    T, F = thresholds.shape
+    out = np.zeros((F, T, num_classes, 2), np.uint32)
+    ctx.check(lib.cf_node_counters(ctx.h, images.h, _ptr(samples), len(samples), _ptr(features), _ptr(thresholds),
+                                   F, T, num_classes, _ptr(out)))
+    return out
+
+
+def scores(ctx, counters, histogram):
+    """NormalizedInformationGainScore of every (threshold, feature): [T][F] float64."""
+    counters = np.ascontiguousarray(counters, np.uint32)
+    histogram = np.ascontiguousarray(histogram, np.uint32)
+    F, T, C, _ = counters.shape
+    out = np.zeros((T, F), np.float64)
+    ctx.check(lib.cf_scores(ctx.h, _ptr(counters), _ptr(histogram), F, T, C, _ptr(out)))
+    return out
+
+
+def node_best_split(ctx, images, samples, features, thresholds, num_classes, histogram):
+    samples = np.ascontiguousarray(samples, SAMPLE_DTYPE)
+    features = np.ascontiguousarray(features, np.int8)
+    thresholds = np.ascontiguousarray(thresholds, np.float32)
+    histogram = np.ascontiguousarray(histogram, np.uint32)
+    T, F = thresholds.shape
+    bf, bt, bs = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    ctx.check(lib.cf_node_best_split(ctx.h, images.h, _ptr(samples), len(samples), _ptr(features), _ptr(thresholds),
+                                     F, T, num_classes, _ptr(histogram), ctypes.byref(bf), ctypes.byref(bt),
+                                     ctypes.byref(bs)))
+    return bf.value, bt.value, bs.value
+
+
+class TrainingConfiguration:
+    """TrainingConfiguration(randomSeed, samplesPerImage, featureCount, minSampleCount, maxDepth, boxRadius,
+    regionSize, thresholds, numThreads, maxImages, imageCacheSize, maxSamplesPerBatch, accelerationMode,
+    useCIELab=True, useDepthFilling=False, deviceIds=[0], subsamplingType="classUniform", ignoredColors=[],
+    useDepthImages=True, horizontalFlipping=False) -- random_tree.h:222-273.
+    `ignored_labels` takes label ids (the reference maps ignored colours to labels)."""
+
+    def __init__(self, randomSeed, samplesPerImage, featureCount, minSampleCount, maxDepth, boxRadius, regionSize,
+                 thresholds, numThreads=1, maxImages=0, imageCacheSize=0, maxSamplesPerBatch=0,
+                 accelerationMode="gpu", useCIELab=False, useDepthFilling=False, deviceIds=(0,),
+                 subsamplingType="classUniform", ignored_labels=(), useDepthImages=True, horizontalFlipping=False):
+        if maxImages > 0 and maxImages < imageCacheSize:
+            raise ValueError(f"illegal configuration: maxImages ({maxImages}) must not be lower than "
+                             f"imageCacheSize ({imageCacheSize})")
+        if accelerationMode not in ("cpu", "gpu", "compare"):
+            raise ValueError(f"illegal acceleration mode: {accelerationMode}")
+        if subsamplingType not in ("classUniform", "pixelUniform"):
+            raise ValueError(f"unknown subsamplingType: {subsamplingType}")
+        if len(ignored_labels) > 8:
+            raise ValueError("at most 8 ignored labels")
+        self.randomSeed, self.samplesPerImage, self.featureCount = randomSeed, samplesPerImage, featureCount
+        self.minSampleCount, self.maxDepth, self.boxRadius = minSampleCount, maxDepth, boxRadius
+        self.regionSize, self.thresholds, self.numThreads = regionSize, thresholds, numThreads
+        self.maxImages, self.imageCacheSize, self.maxSamplesPerBatch = maxImages, imageCacheSize, maxSamplesPerBatch
+        self.accelerationMode, self.useCIELab, self.useDepthFilling = accelerationMode, useCIELab, useDepthFilling
+        self.deviceIds, self.subsamplingType = list(deviceIds), subsamplingType
+        self.ignored_labels, self.useDepthImages = list(ignored_labels), useDepthImages
+        self.horizontalFlipping = horizontalFlipping
+
+    def to_c(self):
+        c = _TrainConfig()
+        c.random_seed, c.samples_per_image, c.feature_count = self.randomSeed, self.samplesPerImage, self.featureCount
+        c.min_sample_count, c.max_depth = self.minSampleCount, self.maxDepth
+        c.box_radius, c.region_size, c.thresholds = self.boxRadius, self.regionSize, self.thresholds
+        c.num_threads, c.max_images = self.numThreads, self.maxImages
+        c.use_depth_images, c.horizontal_flipping = int(self.useDepthImages), int(self.horizontalFlipping)
+        c.subsampling_pixel_uniform = int(self.subsamplingType == "pixelUniform")
+        c.num_ignored = len(self.ignored_labels)
+        for i, l in enumerate(self.ignored_labels):
+            c.ignored_labels[i] = l
+        return c
+
+
+class RandomForestImage:
+    """RandomForestImage (random_forest_image.h:44-168) on the HIP path."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    @classmethod
+    def train(cls, ctx, images, configuration, tree_count, num_labels, mode=MODE_BATCHED):
+        h = ctypes.c_void_p()
+        c = configuration.to_c()
+        ctx.check(lib.cf_forest_train(ctx.h, images.h, ctypes.byref(c), tree_count, num_labels, mode,
+                                      ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def train_tree_from_samples(cls, ctx, images, configuration, tree_id, num_classes, rs_seed, roots,
+                                mode=MODE_BATCHED):
+        roots = np.ascontiguousarray(roots, SAMPLE_DTYPE)
+        h = ctypes.c_void_p()
+        c = configuration.to_c()
+        ctx.check(lib.cf_tree_train_from_samples(ctx.h, images.h, ctypes.byref(c), tree_id, num_classes, rs_seed,
+                                                 _ptr(roots), len(roots), mode, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_trees(cls, ctx, trees, num_classes, prior=None):
+        """trees: list of (nodes NODE_DTYPE array, histograms uint32 [nodes][C])"""
+        counts = np.array([len(n) for n, _ in trees], np.int32)
+        nodes = np.ascontiguousarray(np.concatenate([n for n, _ in trees]), NODE_DTYPE)
+        hist = np.ascontiguousarray(np.concatenate([h for _, h in trees]), np.uint32)
+        pr = None if prior is None else np.ascontiguousarray(prior, np.uint32)
+        h = ctypes.c_void_p()
+        ctx.check(lib.cf_forest_create(ctx.h, len(trees), _ptr(counts), _ptr(nodes), _ptr(hist), num_classes,
+                                       _ptr(pr), ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @property
+    def num_trees(self):
+        return lib.cf_forest_num_trees(self.h)
+
+    @property
+    def num_classes(self):
+        return lib.cf_forest_num_classes(self.h)
+
+    def tree(self, t):
+        n = lib.cf_forest_tree_size(self.h, t)
+        nodes = np.zeros(n, NODE_DTYPE)
+        hist = np.zeros((n, self.num_classes), np.uint32)
+        self.ctx.check(lib.cf_forest_get_tree(self.h, t, _ptr(nodes), _ptr(hist)))
+        return nodes, hist
+
+    def root_samples(self, t):
+        n = lib.cf_forest_get_root_samples(self.h, t, None, 0)
+        out = np.zeros(n, SAMPLE_DTYPE)
+        lib.cf_forest_get_root_samples(self.h, t, _ptr(out), n)
+        return out
+
+    def prior(self):
+        out = np.zeros(self.num_classes, np.uint32)
+        lib.cf_forest_get_prior(self.h, _ptr(out))
+        return out
+
+    def train_stats(self):
+        ev, sec = ctypes.c_double(), ctypes.c_double()
+        replays = lib.cf_forest_train_stats(self.h, ctypes.byref(ev), ctypes.byref(sec))
+        return {"evaluations": ev.value, "seconds": sec.value, "replays": replays}
+
+    def normalizeHistograms(self, histogramBias=0.0, useLabelsPrior=True):
+        self.ctx.check(lib.cf_forest_normalize(self.h, histogramBias, int(useLabelsPrior)))
+
+    def posteriors(self, t):
+        n = lib.cf_forest_tree_size(self.h, t)
+        out = np.zeros((n, self.num_classes), np.float64)
+        self.ctx.check(lib.cf_forest_get_posteriors(self.h, t, _ptr(out)))
+        return out
+
+    def predict(self, images, index, probabilities=True):
+        C, H, W = self.num_classes, images.height, images.width
+        probs = np.zeros((C, H, W), np.float32) if probabilities else None
+        labels = np.zeros((H, W), np.uint8)
+        self.ctx.check(lib.cf_predict(self.ctx.h, self.h, images.h, index, _ptr(probs), _ptr(labels)))
+        return labels, probs
+
+    def predict_batch(self, images, indices, probabilities=True):
+        idx = np.ascontiguousarray(indices, np.int32)
+        C, H, W = self.num_classes, images.height, images.width
+        probs = np.zeros((len(idx), C, H, W), np.float32) if probabilities else None
+        labels = np.zeros((len(idx), H, W), np.uint8)
+        self.ctx.check(lib.cf_predict_batch(self.ctx.h, self.h, images.h, _ptr(idx), len(idx), _ptr(probs),
+                                            _ptr(labels)))
+        return labels, probs
+
+    def close(self):
+        if self.h:
+            lib.cf_forest_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_rgbd(seed, width, height, sites, classes, zero_frac, class_seed=4711):
+    """One synthetic labelled RGB-D frame (rgb [H][W][3] u8, depth [H][W] u16 mm, labels [H][W] u8)."""
+    rgb = np.zeros((height, width, 3), np.uint8)
+    depth = np.zeros((height, width), np.uint16)
+    labels = np.zeros((height, width), np.uint8)
+    st = lib.cf_synth_rgbd(seed, class_seed, width, height, sites, classes, zero_frac, _ptr(rgb), _ptr(depth),
+                           _ptr(labels))
+    if st != CF_OK:
+        raise CurfilError(st, "cf_synth_rgbd failed")
+    return rgb, depth, labels
+
+
+def synth_dataset(n, width, height, sites, classes, zero_frac, seed0=4711, threads=8):
+    """n frames with seeds seed0 + i (ctypes releases the GIL, so frames build in parallel)."""
+    import concurrent.futures
+    rgb = np.zeros((n, height, width, 3), np.uint8)
+    depth = np.zeros((n, height, width), np.uint16)
+    labels = np.zeros((n, height, width), np.uint8)
+
+    def one(i):
+        st = lib.cf_synth_rgbd(seed0 + i, seed0, width, height, sites, classes, zero_frac,
+                               _ptr(rgb[i]), _ptr(depth[i]), _ptr(labels[i]))
+        if st != CF_OK:
+            raise CurfilError(st, "cf_synth_rgbd failed")
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, range(n)))
+    return rgb, depth, labels

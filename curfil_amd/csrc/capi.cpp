@@ -1,0 +1,586 @@
+// C ABI of the library (include/curfil_hip.h): handles, argument checks,
+// host<->device copies, and the forest wrappers around the training driver.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cf_host.h"
+
+using namespace cf;
+
+namespace {
+
+template<class Fn>
+int guarded(cf_ctx* ctx, Fn&& fn) {
+    if (!ctx) return CF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    try {
+        fn();
+        ctx->error.clear();
+        return CF_OK;
+    } catch (const Error& e) {
+        ctx->error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->error = "host allocation failed";
+        return CF_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        ctx->error = e.what();
+        return CF_ERR_RUNTIME;
+    }
+}
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw Error(CF_ERR_ARG, msg);
+}
+
+void checkSamples(const cf_images* im, const cf_sample* s, int n) {
+    for (int i = 0; i < n; i++) {
+        if (s[i].image >= static_cast<uint32_t>(im->count)) throw Error(CF_ERR_ARG, "sample image index out of range");
+        if (s[i].x >= im->W || s[i].y >= im->H) throw Error(CF_ERR_ARG, "sample position outside the image");
+        if (s[i].flip > 2) throw Error(CF_ERR_ARG, "invalid flip setting");
+    }
+}
+
+void uploadFeaturesSoA(cf_ctx* ctx, FeatTables& tabs, const int8_t* soa, const float* thrTF, int F, int T) {
+    tabs.soa.resize(static_cast<size_t>(F) * kFeatParams);
+    tabs.thrTF.resize(static_cast<size_t>(F) * T);
+    CF_HIP(hipMemcpyAsync(tabs.soa.get(), soa, static_cast<size_t>(F) * kFeatParams, hipMemcpyHostToDevice, ctx->stream));
+    if (thrTF) {
+        CF_HIP(hipMemcpyAsync(tabs.thrTF.get(), thrTF, static_cast<size_t>(F) * T * sizeof(float), hipMemcpyHostToDevice,
+                ctx->stream));
+    } else {
+        CF_HIP(hipMemsetAsync(tabs.thrTF.get(), 0, static_cast<size_t>(F) * T * sizeof(float), ctx->stream));
+    }
+    launchPrepareTables(ctx->stream, 1, F, T, tabs);
+}
+
+// Interval histograms of one node (slot 0) into dG.
+void singleNodeBins(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n, const FeatTables& tabs, int C,
+        DevBuf<cf_sample>& dS, DevBuf<Chunk>& dC, DevBuf<uint32_t>& dG) {
+    dS.resize(std::max(n, 1));
+    CF_HIP(hipMemcpyAsync(dS.get(), samples, static_cast<size_t>(n) * sizeof(cf_sample), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<Chunk> chunks;
+    for (int b = 0; b < n; b += 4096) chunks.push_back(Chunk{0, 0, b, std::min(4096, n - b)});
+    dC.resize(std::max<size_t>(chunks.size(), 1));
+    if (!chunks.empty()) {
+        CF_HIP(hipMemcpyAsync(dC.get(), chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, ctx->stream));
+    }
+    const size_t perSlot = static_cast<size_t>(tabs.F) * (tabs.T + 1) * C;
+    dG.resize(perSlot);
+    CF_HIP(hipMemsetAsync(dG.get(), 0, perSlot * sizeof(uint32_t), ctx->stream));
+    launchEvalHistograms(ctx->stream, im->desc.get(), dS.get(), dC.get(), static_cast<int>(chunks.size()), tabs, C,
+            dG.get());
+}
+
+void uploadForest(cf_forest* fo) {
+    if (fo->uploaded) return;
+    if (fo->post.size() != fo->trees.size()) throw Error(CF_ERR_RUNTIME, "histograms not normalized");
+    std::vector<SplitRec> nodes;
+    std::vector<int32_t> child, roots;
+    std::vector<double> post;
+    for (size_t t = 0; t < fo->trees.size(); t++) {
+        const TreeBuild& tb = fo->trees[t];
+        const int base = static_cast<int>(nodes.size());
+        roots.push_back(base);
+        for (size_t i = 0; i < tb.nodes.size(); i++) {
+            const cf_tree_node& n = tb.nodes[i];
+            SplitRec sr{};
+            for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = n.feature[p];
+            sr.threshold = n.threshold;
+            nodes.push_back(sr);
+            if (n.is_leaf) {
+                const int leaf = static_cast<int>(post.size() / fo->C);
+                child.push_back(-leaf - 1);
+                post.insert(post.end(), fo->post[t].begin() + i * fo->C, fo->post[t].begin() + (i + 1) * fo->C);
+            } else {
+                if (n.right != n.left + 1) throw Error(CF_ERR_ARG, "children must be stored as adjacent pairs");
+                child.push_back(base + n.left);
+            }
+        }
+    }
+    fo->dNodes.resize(nodes.size());
+    fo->dChild.resize(child.size());
+    fo->dPost.resize(std::max<size_t>(post.size(), 1));
+    fo->dRoots.resize(roots.size());
+    hipStream_t s = fo->ctx->stream;
+    CF_HIP(hipMemcpyAsync(fo->dNodes.get(), nodes.data(), nodes.size() * sizeof(SplitRec), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemcpyAsync(fo->dChild.get(), child.data(), child.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemcpyAsync(fo->dPost.get(), post.data(), post.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemcpyAsync(fo->dRoots.get(), roots.data(), roots.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CF_HIP(hipStreamSynchronize(s));
+    fo->uploaded = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cf_ctx_create(int device, cf_ctx** out) {
+    if (!out) return CF_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return CF_ERR_HIP;
+    if (device < 0 || device >= n) return CF_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CF_ERR_HIP;
+    auto* c = new cf_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CF_ERR_HIP;
+    }
+    *out = c;
+    return CF_OK;
+}
+
+int cf_ctx_destroy(cf_ctx* ctx) {
+    if (!ctx) return CF_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch0.release();
+    ctx->scratch1.release();
+    ctx->scratch2.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CF_OK;
+}
+
+const char* cf_last_error(const cf_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int cf_ctx_synchronize(cf_ctx* ctx) {
+    return guarded(ctx, [&] { CF_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int cf_images_create(cf_ctx* ctx, int count, int width, int height, cf_images** out) {
+    return guarded(ctx, [&] {
+        require(out != nullptr, "null output");
+        require(count > 0 && width > 0 && height > 0, "image set dimensions must be positive");
+        require(width <= 32767 && height <= 32767, "image too large");
+        CF_HIP(hipSetDevice(ctx->device));
+        auto im = std::make_unique<cf_images>();
+        im->ctx = ctx;
+        im->count = count;
+        im->W = width;
+        im->H = height;
+        const size_t px = static_cast<size_t>(width) * height;
+        im->col.resize(static_cast<size_t>(count) * 3 * px);
+        im->dep.resize(static_cast<size_t>(count) * px);
+        im->desc.resize(count);
+        std::vector<DevImage> d(count);
+        for (int i = 0; i < count; i++) {
+            d[i].col = im->col.get() + static_cast<size_t>(i) * 3 * px;
+            d[i].dep = im->dep.get() + static_cast<size_t>(i) * px;
+            d[i].W = width;
+            d[i].H = height;
+        }
+        CF_HIP(hipMemcpyAsync(im->desc.get(), d.data(), count * sizeof(DevImage), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemsetAsync(im->col.get(), 0, im->col.bytes(), ctx->stream));
+        CF_HIP(hipMemsetAsync(im->dep.get(), 0, im->dep.bytes(), ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        im->labels.assign(static_cast<size_t>(count) * px, 0);
+        im->labelsSet.assign(count, 0);
+        im->depthHost.assign(static_cast<size_t>(count) * px, 0);
+        im->depthSet.assign(count, 0);
+        *out = im.release();
+    });
+}
+
+int cf_images_destroy(cf_images* images) {
+    if (!images) return CF_ERR_ARG;
+    cf_ctx* ctx = images->ctx;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete images;
+    return CF_OK;
+}
+
+int cf_images_set_rgbd_u8(cf_images* im, int index, const uint8_t* rgb, const uint16_t* depth) {
+    if (!im) return CF_ERR_ARG;
+    return guarded(im->ctx, [&] {
+        require(index >= 0 && index < im->count, "image index out of range");
+        require(rgb && depth, "null input");
+        cf_ctx* ctx = im->ctx;
+        const size_t px = static_cast<size_t>(im->W) * im->H;
+        ctx->scratch0.resize(px * 3 + px * 2 + 64);
+        ctx->scratch1.resize(px * 5 * sizeof(uint32_t));
+        uint8_t* dRgb = ctx->scratch0.get();
+        uint16_t* dDepth = reinterpret_cast<uint16_t*>(ctx->scratch0.get() + ((px * 3 + 15) / 16) * 16);
+        CF_HIP(hipMemcpyAsync(dRgb, rgb, px * 3, hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dDepth, depth, px * 2, hipMemcpyHostToDevice, ctx->stream));
+        launchIntegralU8(ctx->stream, dRgb, dDepth, im->W, im->H, reinterpret_cast<uint32_t*>(ctx->scratch1.get()),
+                im->col.get() + static_cast<size_t>(index) * 3 * px, im->dep.get() + static_cast<size_t>(index) * px);
+        int32_t* hd = im->depthHost.data() + static_cast<size_t>(index) * px;
+        for (size_t p = 0; p < px; p++) hd[p] = depth[p];
+        im->depthSet[index] = 1;
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_images_set_rgbd_f32(cf_images* im, int index, const float* color, const int32_t* depth) {
+    if (!im) return CF_ERR_ARG;
+    return guarded(im->ctx, [&] {
+        require(index >= 0 && index < im->count, "image index out of range");
+        require(color && depth, "null input");
+        cf_ctx* ctx = im->ctx;
+        const size_t px = static_cast<size_t>(im->W) * im->H;
+        for (size_t p = 0; p < px; p++) {
+            if (depth[p] > 50000) throw Error(CF_ERR_ARG, "illegal depth value (> 50000 mm)");
+        }
+        ctx->scratch0.resize(px * 3 * sizeof(float) + px * sizeof(int32_t));
+        ctx->scratch1.resize(px * 5 * sizeof(uint32_t));
+        ctx->scratch2.resize(px * 3 * sizeof(double));
+        float* dCol = reinterpret_cast<float*>(ctx->scratch0.get());
+        int32_t* dDep = reinterpret_cast<int32_t*>(ctx->scratch0.get() + px * 3 * sizeof(float));
+        CF_HIP(hipMemcpyAsync(dCol, color, px * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dDep, depth, px * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        launchIntegralF32(ctx->stream, dCol, dDep, im->W, im->H, reinterpret_cast<uint32_t*>(ctx->scratch1.get()),
+                reinterpret_cast<double*>(ctx->scratch2.get()), im->col.get() + static_cast<size_t>(index) * 3 * px,
+                im->dep.get() + static_cast<size_t>(index) * px);
+        int32_t* hd = im->depthHost.data() + static_cast<size_t>(index) * px;
+        for (size_t p = 0; p < px; p++) hd[p] = depth[p] > 0 ? depth[p] : 0;
+        im->depthSet[index] = 1;
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_images_set_labels(cf_images* im, int index, const uint8_t* labels) {
+    if (!im) return CF_ERR_ARG;
+    return guarded(im->ctx, [&] {
+        require(index >= 0 && index < im->count, "image index out of range");
+        require(labels != nullptr, "null input");
+        const size_t px = static_cast<size_t>(im->W) * im->H;
+        std::memcpy(im->labels.data() + static_cast<size_t>(index) * px, labels, px);
+        im->labelsSet[index] = 1;
+    });
+}
+
+int cf_images_get_integrals(cf_images* im, int index, float* color, int32_t* depth, int32_t* valid) {
+    if (!im) return CF_ERR_ARG;
+    return guarded(im->ctx, [&] {
+        require(index >= 0 && index < im->count, "image index out of range");
+        cf_ctx* ctx = im->ctx;
+        const size_t px = static_cast<size_t>(im->W) * im->H;
+        if (color) {
+            CF_HIP(hipMemcpyAsync(color, im->col.get() + static_cast<size_t>(index) * 3 * px, px * 3 * sizeof(float),
+                    hipMemcpyDeviceToHost, ctx->stream));
+        }
+        std::vector<int2> d(px);
+        CF_HIP(hipMemcpyAsync(d.data(), im->dep.get() + static_cast<size_t>(index) * px, px * sizeof(int2),
+                hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t p = 0; p < px; p++) {
+            if (depth) depth[p] = d[p].x;
+            if (valid) valid[p] = d[p].y;
+        }
+    });
+}
+
+int cf_generate_features(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n, int seed, int F, int T,
+        int boxRadius, int regionSize, int useDepth, int8_t* featuresOut, float* thresholdsOut) {
+    return guarded(ctx, [&] {
+        require(im && samples && featuresOut && thresholdsOut, "null argument");
+        require(n > 0 && F > 0 && T > 0, "sizes must be positive");
+        require(boxRadius >= 0 && regionSize >= 1, "invalid boxRadius / regionSize");
+        checkSamples(im, samples, n);
+        DevBuf<cf_sample> dS(n);
+        CF_HIP(hipMemcpyAsync(dS.get(), samples, static_cast<size_t>(n) * sizeof(cf_sample), hipMemcpyHostToDevice,
+                ctx->stream));
+        FeatGenJob j{dS.get(), n, seed, 0, 0};
+        DevBuf<FeatGenJob> dJ(1);
+        CF_HIP(hipMemcpyAsync(dJ.get(), &j, sizeof(j), hipMemcpyHostToDevice, ctx->stream));
+        FeatTables tabs;
+        launchFeatureGeneration(ctx->stream, im->desc.get(), dJ.get(), 1, F, T, boxRadius, regionSize, useDepth, &seed, tabs);
+        CF_HIP(hipMemcpyAsync(featuresOut, tabs.soa.get(), static_cast<size_t>(F) * kFeatParams, hipMemcpyDeviceToHost,
+                ctx->stream));
+        CF_HIP(hipMemcpyAsync(thresholdsOut, tabs.thrTF.get(), static_cast<size_t>(F) * T * sizeof(float),
+                hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_feature_responses(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n, const int8_t* features, int F,
+        double* out) {
+    return guarded(ctx, [&] {
+        require(im && samples && features && out, "null argument");
+        require(n > 0 && F > 0, "sizes must be positive");
+        checkSamples(im, samples, n);
+        std::vector<FeatRec> fr(F);
+        for (int f = 0; f < F; f++) {
+            std::memset(&fr[f], 0, sizeof(FeatRec));
+            for (int p = 0; p < kFeatParams; p++) fr[f].p[p] = features[p * F + f];
+        }
+        DevBuf<cf_sample> dS(n);
+        DevBuf<FeatRec> dF(F);
+        DevBuf<double> dO(static_cast<size_t>(n) * F);
+        CF_HIP(hipMemcpyAsync(dS.get(), samples, static_cast<size_t>(n) * sizeof(cf_sample), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dF.get(), fr.data(), F * sizeof(FeatRec), hipMemcpyHostToDevice, ctx->stream));
+        launchResponses(ctx->stream, im->desc.get(), dS.get(), n, dF.get(), F, dO.get());
+        CF_HIP(hipMemcpyAsync(out, dO.get(), static_cast<size_t>(n) * F * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_node_counters(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n, const int8_t* features,
+        const float* thresholds, int F, int T, int C, uint32_t* counters) {
+    return guarded(ctx, [&] {
+        require(im && samples && features && thresholds && counters, "null argument");
+        require(n > 0 && F > 0 && T > 0 && C > 0 && C <= kMaxClasses, "invalid sizes");
+        checkSamples(im, samples, n);
+        for (int i = 0; i < n; i++) require(samples[i].label < C, "sample label >= number of classes");
+        FeatTables tabs;
+        uploadFeaturesSoA(ctx, tabs, features, thresholds, F, T);
+        DevBuf<cf_sample> dS;
+        DevBuf<Chunk> dC;
+        DevBuf<uint32_t> dG;
+        singleNodeBins(ctx, im, samples, n, tabs, C, dS, dC, dG);
+        DevBuf<uint32_t> dCnt(static_cast<size_t>(F) * T * C * 2);
+        launchCountersFromBins(ctx->stream, dG.get(), tabs, C, dCnt.get());
+        CF_HIP(hipMemcpyAsync(counters, dCnt.get(), dCnt.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_node_best_split(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n, const int8_t* features,
+        const float* thresholds, int F, int T, int C, const uint32_t* histogram, int* bestFeature, int* bestThreshold,
+        double* bestScore) {
+    return guarded(ctx, [&] {
+        require(im && samples && features && thresholds && histogram && bestFeature && bestThreshold && bestScore,
+                "null argument");
+        require(n > 0 && F > 0 && T > 0 && C > 0 && C <= kMaxClasses, "invalid sizes");
+        checkSamples(im, samples, n);
+        for (int i = 0; i < n; i++) require(samples[i].label < C, "sample label >= number of classes");
+        FeatTables tabs;
+        uploadFeaturesSoA(ctx, tabs, features, thresholds, F, T);
+        DevBuf<cf_sample> dS;
+        DevBuf<Chunk> dC;
+        DevBuf<uint32_t> dG;
+        singleNodeBins(ctx, im, samples, n, tabs, C, dS, dC, dG);
+        DevBuf<uint32_t> dH(C);
+        DevBuf<int> dT(1);
+        DevBuf<FeatSummary> dSumm(F);
+        DevBuf<NodeBest> dBest(1);
+        const int zero = 0;
+        CF_HIP(hipMemcpyAsync(dH.get(), histogram, C * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dT.get(), &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        launchScores(ctx->stream, dG.get(), dH.get(), dT.get(), 1, tabs, C, dSumm.get(), nullptr, -1);
+        launchReduce(ctx->stream, dSumm.get(), 1, F, dBest.get());
+        NodeBest b;
+        CF_HIP(hipMemcpyAsync(&b, dBest.get(), sizeof(b), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        if (b.flag) {
+            DevBuf<double> dAll(static_cast<size_t>(F) * T);
+            launchScores(ctx->stream, dG.get(), dH.get(), dT.get(), 1, tabs, C, dSumm.get(), dAll.get(), 0);
+            launchSequentialArgmax(ctx->stream, dAll.get(), T, F, dBest.get());
+            CF_HIP(hipMemcpyAsync(&b, dBest.get(), sizeof(b), hipMemcpyDeviceToHost, ctx->stream));
+            CF_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        *bestFeature = b.feat;
+        *bestThreshold = b.thresh;
+        *bestScore = b.score;
+    });
+}
+
+int cf_scores(cf_ctx* ctx, const uint32_t* counters, const uint32_t* histogram, int F, int T, int C, double* scores) {
+    return guarded(ctx, [&] {
+        require(counters && histogram && scores, "null argument");
+        require(F > 0 && T > 0 && C > 0 && C <= kMaxClasses, "invalid sizes");
+        DevBuf<uint32_t> dCnt(static_cast<size_t>(F) * T * C * 2), dH(C);
+        DevBuf<double> dAll(static_cast<size_t>(F) * T);
+        CF_HIP(hipMemcpyAsync(dCnt.get(), counters, dCnt.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dH.get(), histogram, C * 4, hipMemcpyHostToDevice, ctx->stream));
+        launchScoresFromCounters(ctx->stream, dCnt.get(), dH.get(), F, T, C, dAll.get());
+        CF_HIP(hipMemcpyAsync(scores, dAll.get(), dAll.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_forest_train(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int numTrees, int numLabels, int mode,
+        cf_forest** out) {
+    return guarded(ctx, [&] {
+        require(im && cfg && out, "null argument");
+        require(numTrees > 0, "cannot construct empty forest");
+        if (cfg->subsampling_pixel_uniform) throw Error(CF_ERR_UNSUPPORTED, "pixelUniform subsampling is not implemented");
+        if (cfg->max_images > 0 && cfg->max_images < im->count) {
+            throw Error(CF_ERR_UNSUPPORTED, "maxImages image subsampling is not implemented");
+        }
+        auto fo = std::make_unique<cf_forest>();
+        fo->ctx = ctx;
+        fo->prior = labelPrior(im, numLabels);
+        fo->C = static_cast<int>(fo->prior.size());
+        // RandomForestImage::train: SEED from the configured seed, tree t uses
+        // RandomSource(SEED + t) (random_forest_image.cpp:150-173).
+        HostRandomSource master(cfg->random_seed);
+        const int SEED = master.uniformSampler(0xFFFF).next();
+        std::vector<TreeJob> jobs;
+        for (int t = 0; t < numTrees; t++) {
+            TreeJob j{t, HostRandomSource(SEED + t), {}};
+            j.roots = sampleClassUniform(im, fo->prior, *cfg, j.rs, cfg->samples_per_image / numTrees);
+            fo->roots.push_back(j.roots);
+            jobs.push_back(std::move(j));
+        }
+        trainTrees(ctx, im, *cfg, fo->C, jobs, mode, fo->trees, fo->stats);
+        *out = fo.release();
+    });
+}
+
+int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int treeId, int C, int rsSeed,
+        const cf_sample* roots, int n, int mode, cf_forest** out) {
+    return guarded(ctx, [&] {
+        require(im && cfg && roots && out, "null argument");
+        require(n > 0, "no samples");
+        checkSamples(im, roots, n);
+        auto fo = std::make_unique<cf_forest>();
+        fo->ctx = ctx;
+        fo->C = C;
+        fo->prior.assign(C, 1u);
+        std::vector<TreeJob> jobs;
+        TreeJob j{treeId, HostRandomSource(rsSeed), std::vector<cf_sample>(roots, roots + n)};
+        fo->roots.push_back(j.roots);
+        jobs.push_back(std::move(j));
+        trainTrees(ctx, im, *cfg, C, jobs, mode, fo->trees, fo->stats);
+        *out = fo.release();
+    });
+}
+
+int cf_forest_create(cf_ctx* ctx, int numTrees, const int* numNodes, const cf_tree_node* nodes,
+        const uint32_t* histograms, int C, const uint32_t* prior, cf_forest** out) {
+    return guarded(ctx, [&] {
+        require(numNodes && nodes && histograms && out, "null argument");
+        require(numTrees > 0 && C > 0 && C <= kMaxClasses, "invalid sizes");
+        auto fo = std::make_unique<cf_forest>();
+        fo->ctx = ctx;
+        fo->C = C;
+        fo->prior.assign(C, 1u);
+        if (prior) fo->prior.assign(prior, prior + C);
+        size_t off = 0;
+        for (int t = 0; t < numTrees; t++) {
+            require(numNodes[t] > 0, "empty tree");
+            TreeBuild tb;
+            tb.nodes.assign(nodes + off, nodes + off + numNodes[t]);
+            tb.hist.assign(histograms + off * C, histograms + (off + numNodes[t]) * C);
+            for (int i = 0; i < numNodes[t]; i++) {
+                const auto& n = tb.nodes[i];
+                if (!n.is_leaf) {
+                    // children after their parent (guarantees traversal terminates), stored as a pair
+                    require(n.left > i && n.left + 1 < numNodes[t] && n.right == n.left + 1, "invalid child indices");
+                }
+            }
+            fo->trees.push_back(std::move(tb));
+            off += numNodes[t];
+        }
+        *out = fo.release();
+    });
+}
+
+int cf_forest_destroy(cf_forest* fo) {
+    if (!fo) return CF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lock(fo->ctx->mu);
+    delete fo;
+    return CF_OK;
+}
+
+int cf_forest_num_trees(const cf_forest* fo) { return fo ? static_cast<int>(fo->trees.size()) : CF_ERR_ARG; }
+int cf_forest_num_classes(const cf_forest* fo) { return fo ? fo->C : CF_ERR_ARG; }
+int cf_forest_tree_size(const cf_forest* fo, int t) {
+    if (!fo || t < 0 || t >= static_cast<int>(fo->trees.size())) return CF_ERR_ARG;
+    return static_cast<int>(fo->trees[t].nodes.size());
+}
+
+int cf_forest_get_tree(const cf_forest* fo, int t, cf_tree_node* nodes, uint32_t* hist) {
+    if (!fo || t < 0 || t >= static_cast<int>(fo->trees.size())) return CF_ERR_ARG;
+    const auto& tb = fo->trees[t];
+    if (nodes) std::memcpy(nodes, tb.nodes.data(), tb.nodes.size() * sizeof(cf_tree_node));
+    if (hist) std::memcpy(hist, tb.hist.data(), tb.hist.size() * sizeof(uint32_t));
+    return CF_OK;
+}
+
+int cf_forest_get_root_samples(const cf_forest* fo, int t, cf_sample* out, int capacity) {
+    if (!fo || t < 0 || t >= static_cast<int>(fo->roots.size())) return CF_ERR_ARG;
+    const auto& r = fo->roots[t];
+    if (out) {
+        if (capacity < static_cast<int>(r.size())) return CF_ERR_ARG;
+        std::memcpy(out, r.data(), r.size() * sizeof(cf_sample));
+    }
+    return static_cast<int>(r.size());
+}
+
+int cf_forest_get_prior(const cf_forest* fo, uint32_t* prior) {
+    if (!fo || !prior) return CF_ERR_ARG;
+    std::memcpy(prior, fo->prior.data(), fo->prior.size() * sizeof(uint32_t));
+    return static_cast<int>(fo->prior.size());
+}
+
+int cf_forest_train_stats(const cf_forest* fo, double* evaluations, double* seconds) {
+    if (!fo) return CF_ERR_ARG;
+    if (evaluations) *evaluations = fo->stats.evaluations;
+    if (seconds) *seconds = fo->stats.seconds;
+    return fo->stats.replays;
+}
+
+// normalizeHistogram (random_tree.cpp:54-104) for every node.
+int cf_forest_normalize(cf_forest* fo, double bias, int usePrior) {
+    if (!fo) return CF_ERR_ARG;
+    return guarded(fo->ctx, [&] {
+        const int C = fo->C;
+        std::vector<double> w(C);
+        double psum = 0;
+        for (int i = 0; i < C; i++) psum += fo->prior[i];
+        for (int i = 0; i < C; i++) w[i] = fo->prior[i] / psum;
+        fo->post.assign(fo->trees.size(), {});
+        for (size_t t = 0; t < fo->trees.size(); t++) {
+            const TreeBuild& tb = fo->trees[t];
+            std::vector<double>& out = fo->post[t];
+            out.assign(tb.nodes.size() * C, 0.0);
+            for (size_t n = 0; n < tb.nodes.size(); n++) {
+                const uint32_t* h = &tb.hist[n * C];
+                double* o = &out[n * C];
+                double sum = 0;
+                for (int i = 0; i < C; i++) sum += h[i];
+                for (int i = 0; i < C; i++) {
+                    o[i] = h[i] - bias * sum;
+                    if (o[i] < 0.0) o[i] = 0.0;
+                }
+                sum = 0;
+                for (int i = 0; i < C; i++) sum += o[i];
+                if (sum > 0) {
+                    for (int i = 0; i < C; i++) o[i] = usePrior ? o[i] / sum * w[i] : o[i] / sum;
+                }
+            }
+        }
+        fo->uploaded = false;
+    });
+}
+
+int cf_forest_get_posteriors(const cf_forest* fo, int t, double* out) {
+    if (!fo || !out || t < 0 || t >= static_cast<int>(fo->post.size())) return CF_ERR_ARG;
+    std::memcpy(out, fo->post[t].data(), fo->post[t].size() * sizeof(double));
+    return CF_OK;
+}
+
+int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indices, int count, float* probs,
+        uint8_t* labels) {
+    return guarded(ctx, [&] {
+        require(fo && im && indices, "null argument");
+        require(count > 0, "no frames");
+        for (int i = 0; i < count; i++) require(indices[i] >= 0 && indices[i] < im->count, "frame index out of range");
+        uploadForest(fo);
+        const size_t plane = static_cast<size_t>(im->W) * im->H;
+        DevBuf<int> dIdx(count);
+        DevBuf<float> dP(probs ? static_cast<size_t>(count) * fo->C * plane : 1);
+        DevBuf<uint8_t> dL(labels ? static_cast<size_t>(count) * plane : 1);
+        CF_HIP(hipMemcpyAsync(dIdx.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        DevForest df{fo->dNodes.get(), fo->dChild.get(), fo->dPost.get(), fo->dRoots.get(),
+                static_cast<int>(fo->trees.size()), fo->C};
+        launchClassify(ctx->stream, im->desc.get(), dIdx.get(), count, im->W, im->H, df, probs ? dP.get() : nullptr,
+                labels ? dL.get() : nullptr);
+        if (probs) CF_HIP(hipMemcpyAsync(probs, dP.get(), dP.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        if (labels) CF_HIP(hipMemcpyAsync(labels, dL.get(), dL.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_predict(cf_ctx* ctx, cf_forest* fo, cf_images* im, int index, float* probs, uint8_t* labels) {
+    return cf_predict_batch(ctx, fo, im, &index, 1, probs, labels);
+}
+
+}  // extern "C"

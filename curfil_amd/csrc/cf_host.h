@@ -1,0 +1,60 @@
+// Host-side training driver and forest handle.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "cf_internal.h"
+#include "cf_rng.h"
+
+namespace cf {
+
+struct TreeBuild {
+    std::vector<cf_tree_node> nodes;  // creation order
+    std::vector<uint32_t> hist;       // [nodes][C]
+};
+
+struct TrainStats {
+    double evaluations = 0;  // sample x feature x threshold evaluations
+    double seconds = 0;      // wall time of the level loop (device work included)
+    int levels = 0;
+    int replays = 0;         // nodes resolved by the sequential argmax replay
+};
+
+// One tree to grow: its id, random source state and root samples.
+struct TreeJob {
+    int treeId;
+    HostRandomSource rs;
+    std::vector<cf_sample> roots;
+};
+
+// Level-wise training of several trees on one device (RandomTreeTrain::train,
+// random_tree.h:1249-1415, for every tree at once).
+void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int numClasses,
+        std::vector<TreeJob>& jobs, int mode, std::vector<TreeBuild>& out, TrainStats& stats);
+
+// RandomTreeImage::calculateLabelPriorDistribution (random_tree_image.cpp:885-909)
+std::vector<uint32_t> labelPrior(const cf_images* images, int numLabels);
+
+// RandomTreeImage::subsampleTrainingDataClassUniform with one sampling chunk
+// (random_tree_image.cpp:911-1059) followed by the root sort
+// (random_tree_image.cpp:776-802, image index in place of the image address).
+std::vector<cf_sample> sampleClassUniform(const cf_images* images, const std::vector<uint32_t>& prior,
+        const cf_train_config& cfg, HostRandomSource& rs, size_t subsampleCount);
+
+}  // namespace cf
+
+struct cf_forest {
+    cf_ctx* ctx = nullptr;
+    int C = 0;
+    std::vector<cf::TreeBuild> trees;
+    std::vector<std::vector<cf_sample>> roots;
+    std::vector<uint32_t> prior;
+    std::vector<std::vector<double>> post;  // normalized histograms [nodes][C] per tree
+    bool uploaded = false;
+    cf::DevBuf<cf::SplitRec> dNodes;
+    cf::DevBuf<int32_t> dChild;
+    cf::DevBuf<double> dPost;
+    cf::DevBuf<int32_t> dRoots;
+    cf::TrainStats stats;
+};

@@ -1,0 +1,194 @@
+// Internal host-side declarations: handles, device buffers, kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cf_common.h"
+
+namespace cf {
+
+// Thrown inside the library, turned into a status at the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CF_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::cf::Error(CF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+// Owning device buffer.
+template<class T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { resize(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_) { o.p_ = nullptr; o.n_ = o.cap_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_; n_ = o.n_; cap_ = o.cap_;
+            o.p_ = nullptr; o.n_ = o.cap_ = 0;
+        }
+        return *this;
+    }
+    // grows (never shrinks) the allocation; contents are not preserved
+    void resize(size_t n) {
+        if (n > cap_) {
+            release();
+            if (n > 0) {
+                hipError_t e = hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T));
+                if (e != hipSuccess) {
+                    p_ = nullptr;
+                    throw Error(e == hipErrorOutOfMemory ? CF_ERR_NOMEM : CF_ERR_HIP,
+                            std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes: " +
+                            hipGetErrorString(e));
+                }
+            }
+            cap_ = n;
+        }
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = cap_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0, cap_ = 0;
+};
+
+}  // namespace cf
+
+struct cf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    std::recursive_mutex mu;
+    // scratch buffers reused across calls
+    cf::DevBuf<uint8_t> scratch0, scratch1, scratch2;
+};
+
+struct cf_images {
+    cf_ctx* ctx = nullptr;
+    int count = 0, W = 0, H = 0;
+    cf::DevBuf<float> col;         // [count][3][H][W]
+    cf::DevBuf<int2> dep;          // [count][H][W]
+    cf::DevBuf<cf::DevImage> desc; // [count]
+    std::vector<uint8_t> labels;   // host [count][H][W]
+    std::vector<uint8_t> labelsSet;
+    std::vector<int32_t> depthHost; // host raw depth mm [count][H][W] (pixel validity for sampling)
+    std::vector<uint8_t> depthSet;
+};
+
+namespace cf {
+
+// ---- launchers (implemented in the .hip files) ----
+void launchIntegralU8(hipStream_t s, const uint8_t* rgb, const uint16_t* depth, int W, int H, uint32_t* tmp,
+        float* col, int2* dep);
+void launchIntegralF32(hipStream_t s, const float* color, const int32_t* depth, int W, int H, uint32_t* tmp,
+        double* work, float* col, int2* dep);
+
+// Feature generation job of one (tree, level).
+struct FeatGenJob {
+    const cf_sample* samples; // the level's samples of this tree, frontier order
+    int n;                    // number of samples
+    int seed;                 // seed of the level's RandomSource
+    int outOffset;            // feature slot offset (tree * F)
+    int pad;
+};
+
+// Per-(tree, level) feature tables on the device, slot = tree * F + f.
+struct FeatTables {
+    DevBuf<FeatRec> feat;      // sorted features [trees*F]
+    DevBuf<float> thrFT;       // thresholds feature-major [trees*F][T]
+    DevBuf<float> thrTF;       // thresholds threshold-major [trees][T][F] (reference layout)
+    DevBuf<uint16_t> rank;     // rank[tf*T + t] = #{k : thr[k] < thr[t]} within the feature
+    DevBuf<uint16_t> order;    // thresholds of each feature in non-decreasing rank order
+    DevBuf<int8_t> soa;        // [trees][11][F] (reference SoA layout)
+    // candidate / unsorted staging
+    DevBuf<FeatRec> cand;
+    DevBuf<FeatRec> unsortedFeat;
+    DevBuf<float> unsortedThr;
+    DevBuf<int> unsortedKey;
+    DevBuf<int> err;
+    int trees = 0, F = 0, T = 0;
+};
+
+void launchFeatureGeneration(hipStream_t s, const DevImage* imgs, const FeatGenJob* jobsDev, int numJobs,
+        int F, int T, int boxRadius, int regionSize, int useDepth, const int* seedsHost, FeatTables& tabs);
+// thresholds given in reference layout [trees][T][F] + features SoA [trees][11][F] -> tables
+void launchPrepareTables(hipStream_t s, int trees, int F, int T, FeatTables& tabs);
+
+// Evaluation work item: a chunk of one node's samples.
+struct Chunk {
+    int slot;   // node slot within the launch batch
+    int tree;   // tree index (feature table)
+    int begin;  // first sample (index into the level sample array)
+    int count;  // samples in this chunk
+};
+
+struct NodeBest {
+    double score;
+    int feat;     // sorted feature index
+    int thresh;   // threshold index
+    int flag;     // 1: near-tie band not clean -> sequential replay needed
+    int pad;
+};
+
+struct FeatSummary {
+    double best;   // max score over thresholds
+    double second; // largest score strictly below best (-inf if none)
+    int thresh;    // smallest threshold index reaching best
+    int pad;
+};
+
+// histogram bins G[slot][F][T+1][C] (interval histograms)
+void launchEvalHistograms(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
+        int numChunks, const FeatTables& tabs, int C, uint32_t* G);
+void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
+        const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot);
+void launchScoresFromCounters(hipStream_t s, const uint32_t* counters, const uint32_t* hist, int F, int T, int C,
+        double* scores);
+void launchReduce(hipStream_t s, const FeatSummary* summ, int numSlots, int F, NodeBest* best);
+void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, NodeBest* best);
+void launchCountersFromBins(hipStream_t s, const uint32_t* G, const FeatTables& tabs, int C, uint32_t* counters);
+void launchResponses(hipStream_t s, const DevImage* imgs, const cf_sample* samples, int n, const FeatRec* feats,
+        int F, double* out);
+
+// Partition
+void launchPartitionDirections(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
+        int numChunks, const SplitRec* splits, int C, uint8_t* dir, int* chunkLeft, uint32_t* leftHist);
+void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk* chunks, int numChunks,
+        const uint8_t* dir, const int2* dst, cf_sample* out);
+
+// Prediction
+struct DevForest {
+    const SplitRec* nodes;      // all trees concatenated
+    const int32_t* child;       // left child (global index) or -(leaf index)-1
+    const double* post;         // [leaves][C]
+    const int32_t* treeRoot;    // root node index of each tree
+    int numTrees;
+    int C;
+};
+void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
+        const DevForest& forest, float* probs, uint8_t* labels);
+
+}  // namespace cf

@@ -1,0 +1,374 @@
+// Split evaluation of a frontier of nodes:
+//
+//   k_eval_hist   one workgroup per (node chunk, feature tile): the double
+//                 response of every (sample, feature), mapped to its interval
+//                 among the feature's thresholds, counted per class in LDS
+//                 ("interval histogram", T+1 bins x C classes per feature),
+//                 then added to the node's global bins.
+//                 Replaces featureResponseKernel + aggregateHistogramsKernel
+//                 (random_tree_image_gpu.cu:1455-1720) and the CPU counter
+//                 loop (random_tree_image.cpp:97-153): the response is never
+//                 stored and never re-read per threshold.
+//   k_scores      one thread per (node, feature): prefix sums over the bins
+//                 give every threshold's left/right class counts; FP64
+//                 NormalizedInformationGainScore (score.h:45-216) in the
+//                 reference's operation order; per-feature best / second best.
+//   k_reduce      per node: the reference's threshold-major sequential argmax
+//                 with its 1e-13 tie rule (random_tree_image.cpp:445-457,
+//                 random_tree.cpp:36-52) as max + first occurrence, valid
+//                 whenever no score lies within 1e-13 below the maximum; such
+//                 nodes are flagged and replayed sequentially (k_seq_argmax).
+//
+// Left at threshold t  <=>  r <= t  <=>  q := roundUpToFloat(r) <= t
+//                      <=>  bin(q) := #{k : thr[k] < q} <= rank[t]
+// with NaN responses in bin T (always right), so left(t, c) is the prefix sum
+// of the bins up to rank[t].
+#include <cfloat>
+
+#include "cf_internal.h"
+
+namespace cf {
+
+constexpr int kEvalThreads = 256;
+constexpr int kFeatTile = 16;
+
+__global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const DevImage* __restrict__ imgs,
+        const cf_sample* __restrict__ samples, const Chunk* __restrict__ chunks, const FeatRec* __restrict__ feat,
+        const float* __restrict__ thrFT, int F, int T, int C, uint32_t* __restrict__ G) {
+    extern __shared__ uint32_t h[];
+    const Chunk ch = chunks[blockIdx.x];
+    const int f0 = blockIdx.y * kFeatTile;
+    const int nf = min(kFeatTile, F - f0);
+    const int binsPerF = (T + 1) * C;
+    const int nb = nf * binsPerF;
+    for (int k = threadIdx.x; k < nb; k += kEvalThreads) h[k] = 0;
+    __syncthreads();
+    const FeatRec* fp = feat + static_cast<size_t>(ch.tree) * F + f0;
+    const float* tp = thrFT + (static_cast<size_t>(ch.tree) * F + f0) * T;
+    for (int i = threadIdx.x; i < ch.count; i += kEvalThreads) {
+        const cf_sample s = samples[ch.begin + i];
+        const DevImage im = imgs[s.image];
+        for (int fi = 0; fi < nf; fi++) {
+            const FeatRec fr = fp[fi];
+            const float* th = tp + static_cast<size_t>(fi) * T;
+            for (int pass = 0; pass < (s.flip == 2 ? 2 : 1); pass++) {
+                const bool flip = (s.flip == 1) || (pass == 1);
+                const double r = featureResponse(im, fr.p, s.x, s.y, s.depth_mm, flip);
+                const float q = roundUpToFloat(r);
+                int b = T;
+                if (!__builtin_isnan(q)) {
+                    b = 0;
+                    for (int t = 0; t < T; t++) b += (th[t] < q) ? 1 : 0;
+                }
+                atomicAdd(&h[fi * binsPerF + b * C + s.label], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t* g = G + (static_cast<size_t>(ch.slot) * F + f0) * binsPerF;
+    for (int k = threadIdx.x; k < nb; k += kEvalThreads) {
+        const uint32_t v = h[k];
+        if (v) atomicAdd(&g[k], v);
+    }
+}
+
+// NormalizedInformationGainScore::calculateScore (score.h:191-215) over
+// InformationGainScore::calculateScore (score.h:87-139); the operation order
+// follows the reference so host and device round identically apart from log2.
+__device__ __forceinline__ double entropyOf(double p) {
+    if (p == 0.0) return 0.0;
+    return -p * log2(p);
+}
+__device__ __forceinline__ double clamp01(double s) {
+    if (s > 1.0) return 1.0;
+    if (s < 0.0) return 0.0;
+    return s;
+}
+
+template<int CM>
+__device__ double nigScore(int C, const uint32_t* L, const uint32_t* R, const uint32_t* A) {
+    double totalLeft = 0, totalRight = 0;
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        if (c < C) {
+            totalLeft += static_cast<double>(L[c]);
+            totalRight += static_cast<double>(R[c]);
+        }
+    }
+    const double total = totalLeft + totalRight;
+    const double leftProb = totalLeft / total;
+    const double rightProb = totalRight / total;
+    double score = 0;
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        if (c < C) {
+            const uint32_t l = L[c], r = R[c];
+            const double classProb = static_cast<double>(A[c]) / total;
+            if (l > 0) {
+                const double p = static_cast<double>(l) / total;
+                score = __dadd_rn(score, __dmul_rn(p, log2(p / (leftProb * classProb))));
+            }
+            if (r > 0) {
+                const double p = static_cast<double>(r) / total;
+                score = __dadd_rn(score, __dmul_rn(p, log2(p / (rightProb * classProb))));
+            }
+        }
+    }
+    const double ig = clamp01(score);
+    if (ig == 0) return 0;
+    const double Hs = (entropyOf(totalLeft) + entropyOf(totalRight) - entropyOf(total)) / total;
+    double Hc = 0;
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        if (c < C && A[c] > 0) Hc += entropyOf(static_cast<double>(A[c]));
+    }
+    Hc -= entropyOf(total);
+    Hc /= total;
+    return clamp01((2 * ig) / (Hs + Hc));
+}
+
+template<int CM>
+__global__ void __launch_bounds__(256) k_scores(const uint32_t* __restrict__ G, const uint32_t* __restrict__ nodeHist,
+        const int* __restrict__ slotTree, int numSlots, const uint16_t* __restrict__ rank,
+        const uint16_t* __restrict__ order, int F, int T, int C, FeatSummary* __restrict__ summ,
+        double* __restrict__ allScores, int onlySlot) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int slot, f;
+    if (onlySlot >= 0) {
+        slot = onlySlot;
+        f = gid;
+        if (f >= F) return;
+    } else {
+        if (gid >= numSlots * F) return;
+        slot = gid / F;
+        f = gid % F;
+    }
+    const int tree = slotTree[slot];
+    uint32_t A[CM], run[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        A[c] = (c < C) ? nodeHist[static_cast<size_t>(slot) * C + c] : 0u;
+        run[c] = 0;
+    }
+    const size_t tf = static_cast<size_t>(tree) * F + f;
+    const uint32_t* g = G + (static_cast<size_t>(slot) * F + f) * (T + 1) * C;
+    const uint16_t* rk = rank + tf * T;
+    const uint16_t* od = order + tf * T;
+    double best = -DBL_MAX, second = -DBL_MAX;
+    int bestT = 0x7fffffff;
+    int b = 0;
+    for (int k = 0; k < T; k++) {
+        const int t = od[k];
+        const int r = rk[t];
+        while (b <= r) {
+#pragma unroll
+            for (int c = 0; c < CM; c++) {
+                if (c < C) run[c] += g[b * C + c];
+            }
+            b++;
+        }
+        uint32_t right[CM];
+#pragma unroll
+        for (int c = 0; c < CM; c++) right[c] = A[c] - run[c];
+        const double sc = nigScore<CM>(C, run, right, A);
+        if (allScores) allScores[static_cast<size_t>(t) * F + f] = sc;
+        if (sc > best) {
+            second = best;
+            best = sc;
+            bestT = t;
+        } else if (sc == best) {
+            bestT = min(bestT, t);
+        } else if (sc > second) {
+            second = sc;
+        }
+    }
+    FeatSummary s;
+    s.best = best;
+    s.second = second;
+    s.thresh = bestT;
+    s.pad = 0;
+    summ[static_cast<size_t>(slot) * F + f] = s;
+}
+
+// scores[T][F] of one node straight from counters [F][T][C][2]
+// (ImageFeatureEvaluation::calculateScores, random_tree_image.cpp:188-237).
+template<int CM>
+__global__ void __launch_bounds__(256) k_scores_counters(const uint32_t* __restrict__ counters,
+        const uint32_t* __restrict__ hist, int F, int T, int C, double* __restrict__ scores) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F * T) return;
+    const int f = i / T, t = i % T;
+    uint32_t L[CM], R[CM], A[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) {
+        const size_t k = ((static_cast<size_t>(f) * T + t) * C + c) * 2;
+        L[c] = c < C ? counters[k] : 0u;
+        R[c] = c < C ? counters[k + 1] : 0u;
+        A[c] = c < C ? hist[c] : 0u;
+    }
+    scores[static_cast<size_t>(t) * F + f] = nigScore<CM>(C, L, R, A);
+}
+
+void launchScoresFromCounters(hipStream_t s, const uint32_t* counters, const uint32_t* hist, int F, int T, int C,
+        double* scores) {
+    const dim3 grid((F * T + 255) / 256);
+    if (C <= 8) {
+        hipLaunchKernelGGL(k_scores_counters<8>, grid, dim3(256), 0, s, counters, hist, F, T, C, scores);
+    } else {
+        hipLaunchKernelGGL(k_scores_counters<kMaxClasses>, grid, dim3(256), 0, s, counters, hist, F, T, C, scores);
+    }
+}
+
+__device__ __forceinline__ double waveMax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ long long waveMinLL(long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const long long u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+
+// One wave per node.
+__global__ void __launch_bounds__(64) k_reduce(const FeatSummary* __restrict__ summ, int numSlots, int F,
+        NodeBest* __restrict__ out) {
+    const int slot = blockIdx.x;
+    const int lane = threadIdx.x;
+    const FeatSummary* s = summ + static_cast<size_t>(slot) * F;
+    double m = -DBL_MAX;
+    for (int f = lane; f < F; f += 64) m = fmax(m, s[f].best);
+    m = waveMax(m);
+    long long key = 0x7fffffffffffffffll;
+    int band = 0;
+    for (int f = lane; f < F; f += 64) {
+        const FeatSummary x = s[f];
+        if (x.best == m) {
+            const long long k = static_cast<long long>(x.thresh) * F + f;
+            key = k < key ? k : key;
+            if (x.second > -DBL_MAX && (m - x.second) < 1e-13) band = 1;
+        } else if ((m - x.best) < 1e-13) {
+            band = 1;
+        }
+    }
+    key = waveMinLL(key);
+    band = __any(band) ? 1 : 0;
+    if (lane == 0) {
+        NodeBest nb;
+        nb.score = m;
+        nb.thresh = static_cast<int>(key / F);
+        nb.feat = static_cast<int>(key % F);
+        nb.flag = band;
+        nb.pad = 0;
+        out[slot] = nb;
+    }
+}
+
+// The literal scan of random_tree_image.cpp:445-457 for one node.
+__global__ void k_seq_argmax(const double* __restrict__ scores, int T, int F, NodeBest* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double bestScore = -__builtin_inf();
+    int bestT = 0, bestF = 0;
+    for (int t = 0; t < T; t++) {
+        for (int f = 0; f < F; f++) {
+            const double sc = scores[static_cast<size_t>(t) * F + f];
+            bool better;
+            if (__builtin_isnan(bestScore)) {
+                better = true;
+            } else {
+                const double diff = fabs(sc - bestScore);
+                better = !(diff < 1e-13) && (sc > bestScore);
+            }
+            if (better) {
+                bestF = f;
+                bestT = t;
+                bestScore = sc;
+            }
+        }
+    }
+    NodeBest nb;
+    nb.score = bestScore;
+    nb.feat = bestF;
+    nb.thresh = bestT;
+    nb.flag = 0;
+    nb.pad = 0;
+    *out = nb;
+}
+
+// counters[F][T][C][2] of one node (slot 0) from its bins.
+__global__ void __launch_bounds__(256) k_counters_from_bins(const uint32_t* __restrict__ G,
+        const uint16_t* __restrict__ rank, int F, int T, int C, uint32_t* __restrict__ counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F * T) return;
+    const int f = i / T, t = i % T;
+    const uint32_t* g = G + static_cast<size_t>(f) * (T + 1) * C;
+    const int r = rank[static_cast<size_t>(f) * T + t];
+    for (int c = 0; c < C; c++) {
+        uint32_t left = 0, total = 0;
+        for (int b = 0; b <= T; b++) {
+            const uint32_t v = g[b * C + c];
+            total += v;
+            if (b <= r) left += v;
+        }
+        counters[((static_cast<size_t>(f) * T + t) * C + c) * 2 + 0] = left;
+        counters[((static_cast<size_t>(f) * T + t) * C + c) * 2 + 1] = total - left;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_responses(const DevImage* __restrict__ imgs,
+        const cf_sample* __restrict__ samples, int n, const FeatRec* __restrict__ feats, int F,
+        double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * F) return;
+    const int f = i / n, k = i % n;
+    const cf_sample s = samples[k];
+    out[i] = featureResponse(imgs[s.image], feats[f].p, s.x, s.y, s.depth_mm, s.flip == 1);
+}
+
+void launchEvalHistograms(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
+        int numChunks, const FeatTables& tabs, int C, uint32_t* G) {
+    if (numChunks == 0) return;
+    const int tiles = (tabs.F + kFeatTile - 1) / kFeatTile;
+    const size_t lds = static_cast<size_t>(kFeatTile) * (tabs.T + 1) * C * sizeof(uint32_t);
+    if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
+    hipLaunchKernelGGL(k_eval_hist, dim3(numChunks, tiles), dim3(kEvalThreads), lds, s, imgs, samples, chunks,
+            tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+}
+
+void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
+        const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot) {
+    const int threads = onlySlot >= 0 ? tabs.F : numSlots * tabs.F;
+    const dim3 grid((threads + 255) / 256);
+    if (C <= 8) {
+        hipLaunchKernelGGL(k_scores<8>, grid, dim3(256), 0, s, G, nodeHist, slotTree, numSlots, tabs.rank.get(),
+                tabs.order.get(), tabs.F, tabs.T, C, summ, allScores, onlySlot);
+    } else {
+        hipLaunchKernelGGL(k_scores<kMaxClasses>, grid, dim3(256), 0, s, G, nodeHist, slotTree, numSlots,
+                tabs.rank.get(), tabs.order.get(), tabs.F, tabs.T, C, summ, allScores, onlySlot);
+    }
+}
+
+void launchReduce(hipStream_t s, const FeatSummary* summ, int numSlots, int F, NodeBest* best) {
+    if (numSlots == 0) return;
+    hipLaunchKernelGGL(k_reduce, dim3(numSlots), dim3(64), 0, s, summ, numSlots, F, best);
+}
+
+void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, NodeBest* best) {
+    hipLaunchKernelGGL(k_seq_argmax, dim3(1), dim3(64), 0, s, scores, T, F, best);
+}
+
+void launchCountersFromBins(hipStream_t s, const uint32_t* G, const FeatTables& tabs, int C, uint32_t* counters) {
+    const int n = tabs.F * tabs.T;
+    hipLaunchKernelGGL(k_counters_from_bins, dim3((n + 255) / 256), dim3(256), 0, s, G, tabs.rank.get(), tabs.F,
+            tabs.T, C, counters);
+}
+
+void launchResponses(hipStream_t s, const DevImage* imgs, const cf_sample* samples, int n, const FeatRec* feats,
+        int F, double* out) {
+    const int total = n * F;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_responses, dim3((total + 255) / 256), dim3(256), 0, s, imgs, samples, n, feats, F, out);
+}
+
+}  // namespace cf

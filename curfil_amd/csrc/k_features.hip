@@ -1,0 +1,361 @@
+// Per-(tree, level) random feature generation on the device, with the exact
+// semantics of the reference CPU generator:
+//   ImageFeatureEvaluation::generateRandomFeatures  random_tree_image.cpp:577-646
+//   ImageFeatureEvaluation::sampleFeature           random_tree_image.cpp:648-700
+//   sortFeatures (stable sort by getSortKey)        random_tree_image.cpp:504-566,
+//                                                   random_tree_image.h:444-452
+//
+// RandomSource(seed) hands seed to the sample-index sampler and seeds
+// seed+1+4k .. seed+4+4k to candidate k's four samplers (type, offset,
+// region, channel).  Candidates are therefore independent and are drawn in
+// parallel (k_fg_candidates).  Thresholds consume one shared draw stream in
+// order, and how many draws a candidate consumes depends on which responses
+// are NaN; k_fg_walk keeps that order exactly: one workgroup per (tree, level)
+// owns the sample-index generator (MT19937 state twisted in parallel in LDS),
+// evaluates the next 256 queued draws for the current candidate in parallel
+// and lets one lane walk them with the reference's retry rules.
+#include "cf_internal.h"
+#include "cf_rng.h"
+
+namespace cf {
+
+constexpr int kPrefixDraws = 24;   // engine outputs available to one feature sampler
+constexpr int kDrawQueue = 2048;   // ring of pending sample-index draws
+
+// g++ on x86-64 evaluates the two getNext() calls of `Offset(rgen.getNext(),
+// rgen.getNext())` right to left, so the second coordinate is drawn first
+// (pinned by the reference's golden feature hash).
+__global__ void __launch_bounds__(256) k_fg_candidates(const int* __restrict__ seeds, int numCand, int boxRadius,
+        int regionSize, int useDepth, FeatRec* __restrict__ cand, int* __restrict__ err) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int job = blockIdx.y;
+    if (k >= numCand) return;
+    const int base = seeds[job] + 1 + 4 * k;
+    FeatRec f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) f.p[i] = 0;
+    bool ok = true;
+    int a, b;
+    {
+        DevPrefixSampler<kPrefixDraws> xy;
+        xy.init(base + 1, -boxRadius, boxRadius);
+        int o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+        ok &= xy.next(b);
+        ok &= xy.next(a);
+        o1x = a;
+        o1y = b;
+        do {
+            ok &= xy.next(b);
+            ok &= xy.next(a);
+            o2x = a;
+            o2y = b;
+        } while (ok && o1x == o2x && o1y == o2y);
+        f.p[P_O1X] = static_cast<int8_t>(o1x);
+        f.p[P_O1Y] = static_cast<int8_t>(o1y);
+        f.p[P_O2X] = static_cast<int8_t>(o2x);
+        f.p[P_O2Y] = static_cast<int8_t>(o2y);
+    }
+    {
+        DevPrefixSampler<kPrefixDraws> rg;
+        rg.init(base + 2, 1, regionSize);
+        ok &= rg.next(b);
+        ok &= rg.next(a);
+        f.p[P_R1X] = static_cast<int8_t>(a);
+        f.p[P_R1Y] = static_cast<int8_t>(b);
+        ok &= rg.next(b);
+        ok &= rg.next(a);
+        f.p[P_R2X] = static_cast<int8_t>(a);
+        f.p[P_R2Y] = static_cast<int8_t>(b);
+    }
+    int type = T_COLOR;
+    if (useDepth) {
+        DevPrefixSampler<kPrefixDraws> ft;
+        ft.init(base, 0, 1);
+        int v = 0;
+        ok &= ft.next(v);
+        type = (v == 0) ? T_DEPTH : T_COLOR;
+    }
+    f.p[P_TYPE] = static_cast<int8_t>(type);
+    if (type == T_COLOR) {
+        DevPrefixSampler<kPrefixDraws> cc;
+        cc.init(base + 3, 0, 2);
+        ok &= cc.next(b);
+        ok &= cc.next(a);
+        f.p[P_C1] = static_cast<int8_t>(a);
+        f.p[P_C2] = static_cast<int8_t>(b);
+    }
+    if (!ok) atomicExch(err, 2);
+    cand[static_cast<size_t>(job) * numCand + k] = f;
+}
+
+// getSortKey, random_tree_image.h:444-452
+__device__ __forceinline__ int featureSortKey(const FeatRec& f) {
+    int k = 0;
+    k |= static_cast<int>(static_cast<uint8_t>(f.p[P_TYPE] & 0x03)) << 30;
+    k |= static_cast<int>(static_cast<uint8_t>(f.p[P_C1] & 0x0F)) << 26;
+    k |= static_cast<int>(static_cast<uint8_t>(f.p[P_C2] & 0x0F)) << 22;
+    k |= static_cast<int>(static_cast<uint8_t>((f.p[P_O1Y] + 127) & 0xFF)) << 14;
+    k |= static_cast<int>(static_cast<uint8_t>((f.p[P_O1X] + 127) & 0xFF)) << 6;
+    return k;
+}
+
+struct WalkShared {
+    uint32_t mt[624];
+    int dq[kDrawQueue];
+    float resp[256];
+    float thr[kMaxThresholds];
+    int waveCounts[4];
+    int status, consumed;
+};
+
+// One MT19937 twist of the LDS state, in three dependency-free phases.
+__device__ void mtTwistShared(uint32_t* mt) {
+    const int tid = threadIdx.x;
+    uint32_t v = 0;
+    if (tid < 227) v = mtMix(mt[tid], mt[tid + 1], mt[tid + 397]);
+    __syncthreads();
+    if (tid < 227) mt[tid] = v;
+    __syncthreads();
+    if (tid < 227) v = mtMix(mt[227 + tid], mt[228 + tid], mt[tid]);
+    __syncthreads();
+    if (tid < 227) mt[227 + tid] = v;
+    __syncthreads();
+    if (tid < 169) v = mtMix(mt[454 + tid], mt[455 + tid], mt[227 + tid]);
+    __syncthreads();
+    if (tid < 169) mt[454 + tid] = v;
+    __syncthreads();
+    if (tid == 0) mt[623] = mtMix(mt[623], mt[0], mt[396]);
+    __syncthreads();
+}
+
+// Appends the accepted uniform_int draws of one engine block to the queue.
+__device__ int refillDraws(WalkShared& sh, int tail, const UniformIntParams& up) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    if (up.range == 0) {
+        for (int i = tid; i < 624; i += 256) sh.dq[(tail + i) % kDrawQueue] = up.lo;
+        __syncthreads();
+        return tail + 624;
+    }
+    mtTwistShared(sh.mt);
+    for (int r = 0; r < 3; r++) {
+        const int i = r * 256 + tid;
+        bool acc = false;
+        uint32_t v = 0;
+        if (i < 624) {
+            v = mtTemper(sh.mt[i]) / up.bucket;
+            acc = v <= up.range;
+        }
+        const unsigned long long mask = __ballot(acc);
+        const int lanePrefix = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) sh.waveCounts[wave] = __popcll(mask);
+        __syncthreads();
+        int wavePrefix = 0, total = 0;
+        for (int w = 0; w < 4; w++) {
+            if (w < wave) wavePrefix += sh.waveCounts[w];
+            total += sh.waveCounts[w];
+        }
+        if (acc) sh.dq[(tail + wavePrefix + lanePrefix) % kDrawQueue] = static_cast<int>(static_cast<uint32_t>(up.lo) + v);
+        tail += total;
+        __syncthreads();
+    }
+    return tail;
+}
+
+__global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ jobs, const DevImage* __restrict__ imgs,
+        const FeatRec* __restrict__ cand, int numCand, int F, int T, FeatRec* __restrict__ outFeat,
+        float* __restrict__ outThr, int* __restrict__ outKey, int* __restrict__ err) {
+    __shared__ WalkShared sh;
+    const FeatGenJob job = jobs[blockIdx.x];
+    const FeatRec* myCand = cand + static_cast<size_t>(blockIdx.x) * numCand;
+    const int tid = threadIdx.x;
+    const UniformIntParams up = uniformIntParams(0, job.n - 1);
+    if (tid == 0) {
+        uint32_t v = static_cast<uint32_t>(job.seed);
+        sh.mt[0] = v;
+        for (uint32_t i = 1; i < 624; i++) {
+            v = mtInitStep(v, i);
+            sh.mt[i] = v;
+        }
+    }
+    __syncthreads();
+    int tail = 0, head = 0, generated = 0;
+    for (int k = 0; generated < F; k++) {
+        // `--maxLoops <= 0` with maxLoops = 3F, checked right after sampling candidate k
+        if (k + 1 >= 3 * F) {
+            if (tid == 0) atomicExch(err, 1);
+            break;
+        }
+        const FeatRec fr = myCand[k];
+        int pos = head;
+        int t = 0, tries = 10;  // meaningful in lane 0 only
+        int status = 0;
+        for (;;) {
+            while (tail - pos < 256) tail = refillDraws(sh, tail, up);
+            {
+                const int d = sh.dq[(pos + tid) % kDrawQueue];
+                const cf_sample s = job.samples[d];
+                const double r = featureResponse(imgs[s.image], fr.p, s.x, s.y, s.depth_mm, false);
+                sh.resp[tid] = static_cast<float>(r);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int consumed = 256, st = 0;
+                for (int i = 0; i < 256; i++) {
+                    const float v = sh.resp[i];
+                    if (!__builtin_isnan(v)) {
+                        sh.thr[t++] = v;
+                        tries = 10;
+                        if (t == T) { st = 1; consumed = i + 1; break; }
+                    } else if (--tries == 0) {
+                        st = 2;
+                        consumed = i + 1;
+                        break;
+                    }
+                }
+                sh.status = st;
+                sh.consumed = consumed;
+            }
+            __syncthreads();
+            status = sh.status;
+            pos += sh.consumed;
+            __syncthreads();
+            if (status != 0) break;
+        }
+        head = pos;
+        if (status == 1) {
+            const int slot = job.outOffset + generated;
+            for (int i = tid; i < T; i += 256) outThr[static_cast<size_t>(slot) * T + i] = sh.thr[i];
+            if (tid == 0) {
+                outFeat[slot] = fr;
+                outKey[slot] = featureSortKey(fr);
+            }
+            generated++;
+        }
+        __syncthreads();
+    }
+}
+
+// Stable sort of each job's F features by key (rank = #smaller keys + #equal
+// keys earlier), scattering features and thresholds into both layouts.
+__global__ void __launch_bounds__(256) k_fg_sort(const int* __restrict__ keys, const FeatRec* __restrict__ inFeat,
+        const float* __restrict__ inThr, int F, int T, FeatRec* __restrict__ feat, float* __restrict__ thrFT,
+        float* __restrict__ thrTF, int8_t* __restrict__ soa) {
+    extern __shared__ int sk[];
+    const int job = blockIdx.x;
+    const int* jk = keys + static_cast<size_t>(job) * F;
+    for (int i = threadIdx.x; i < F; i += blockDim.x) sk[i] = jk[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < F; i += blockDim.x) {
+        const int ki = sk[i];
+        int rank = 0;
+        for (int j = 0; j < F; j++) {
+            const int kj = sk[j];
+            rank += (kj < ki || (kj == ki && j < i)) ? 1 : 0;
+        }
+        const size_t src = static_cast<size_t>(job) * F + i;
+        const size_t dst = static_cast<size_t>(job) * F + rank;
+        const FeatRec fr = inFeat[src];
+        feat[dst] = fr;
+        for (int p = 0; p < kFeatParams; p++) soa[(static_cast<size_t>(job) * kFeatParams + p) * F + rank] = fr.p[p];
+        for (int t = 0; t < T; t++) {
+            const float v = inThr[src * T + t];
+            thrFT[dst * T + t] = v;
+            thrTF[(static_cast<size_t>(job) * T + t) * F + rank] = v;
+        }
+    }
+}
+
+// Reference layout in (soa [trees][11][F], thrTF [trees][T][F]) -> feat, thrFT.
+__global__ void __launch_bounds__(256) k_tables_from_reference(const int8_t* __restrict__ soa,
+        const float* __restrict__ thrTF, int trees, int F, int T, FeatRec* __restrict__ feat,
+        float* __restrict__ thrFT) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= trees * F) return;
+    const int job = i / F, f = i % F;
+    FeatRec fr;
+#pragma unroll
+    for (int p = 0; p < 16; p++) fr.p[p] = 0;
+    for (int p = 0; p < kFeatParams; p++) fr.p[p] = soa[(static_cast<size_t>(job) * kFeatParams + p) * F + f];
+    feat[i] = fr;
+    for (int t = 0; t < T; t++) thrFT[static_cast<size_t>(i) * T + t] = thrTF[(static_cast<size_t>(job) * T + t) * F + f];
+}
+
+// Per feature: rank[t] = #{k : thr[k] < thr[t]} and the thresholds ordered by
+// (value, index).  A sample with float key q goes left at threshold t iff
+// q <= thr[t] iff #{k : thr[k] < q} <= rank[t].
+__global__ void __launch_bounds__(256) k_threshold_ranks(const float* __restrict__ thrFT, int total, int T,
+        uint16_t* __restrict__ rank, uint16_t* __restrict__ order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float* th = thrFT + static_cast<size_t>(i) * T;
+    for (int t = 0; t < T; t++) {
+        const float v = th[t];
+        int r = 0, pos = 0;
+        for (int k = 0; k < T; k++) {
+            const float u = th[k];
+            r += (u < v) ? 1 : 0;
+            pos += (u < v || (u == v && k < t)) ? 1 : 0;
+        }
+        rank[static_cast<size_t>(i) * T + t] = static_cast<uint16_t>(r);
+        order[static_cast<size_t>(i) * T + pos] = static_cast<uint16_t>(t);
+    }
+}
+
+void launchFeatureGeneration(hipStream_t s, const DevImage* imgs, const FeatGenJob* jobsDev, int numJobs, int F,
+        int T, int boxRadius, int regionSize, int useDepth, const int* seedsHost, FeatTables& tabs) {
+    if (T > kMaxThresholds) throw Error(CF_ERR_ARG, "too many thresholds (max 256)");
+    if (F > 8192) throw Error(CF_ERR_ARG, "too many features (max 8192)");
+    if (boxRadius > 127 || regionSize > 127) throw Error(CF_ERR_ARG, "boxRadius/regionSize must fit int8");
+    const int numCand = 3 * F;
+    tabs.trees = numJobs;
+    tabs.F = F;
+    tabs.T = T;
+    tabs.cand.resize(static_cast<size_t>(numJobs) * numCand);
+    tabs.unsortedFeat.resize(static_cast<size_t>(numJobs) * F);
+    tabs.unsortedThr.resize(static_cast<size_t>(numJobs) * F * T);
+    tabs.unsortedKey.resize(static_cast<size_t>(numJobs) * F);
+    tabs.feat.resize(static_cast<size_t>(numJobs) * F);
+    tabs.thrFT.resize(static_cast<size_t>(numJobs) * F * T);
+    tabs.thrTF.resize(static_cast<size_t>(numJobs) * F * T);
+    tabs.soa.resize(static_cast<size_t>(numJobs) * F * kFeatParams);
+    tabs.rank.resize(static_cast<size_t>(numJobs) * F * T);
+    tabs.order.resize(static_cast<size_t>(numJobs) * F * T);
+    tabs.err.resize(1);
+    CF_HIP(hipMemsetAsync(tabs.err.get(), 0, sizeof(int), s));
+    DevBuf<int> seeds(numJobs);
+    CF_HIP(hipMemcpyAsync(seeds.get(), seedsHost, numJobs * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fg_candidates, dim3((numCand + 255) / 256, numJobs), dim3(256), 0, s, seeds.get(), numCand,
+            boxRadius, regionSize, useDepth, tabs.cand.get(), tabs.err.get());
+    hipLaunchKernelGGL(k_fg_walk, dim3(numJobs), dim3(256), 0, s, jobsDev, imgs, tabs.cand.get(), numCand, F, T,
+            tabs.unsortedFeat.get(), tabs.unsortedThr.get(), tabs.unsortedKey.get(), tabs.err.get());
+    hipLaunchKernelGGL(k_fg_sort, dim3(numJobs), dim3(256), F * sizeof(int), s, tabs.unsortedKey.get(),
+            tabs.unsortedFeat.get(), tabs.unsortedThr.get(), F, T, tabs.feat.get(), tabs.thrFT.get(),
+            tabs.thrTF.get(), tabs.soa.get());
+    const int total = numJobs * F;
+    hipLaunchKernelGGL(k_threshold_ranks, dim3((total + 255) / 256), dim3(256), 0, s, tabs.thrFT.get(), total, T,
+            tabs.rank.get(), tabs.order.get());
+    int errHost = 0;
+    CF_HIP(hipMemcpyAsync(&errHost, tabs.err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    CF_HIP(hipStreamSynchronize(s));
+    if (errHost == 1) throw Error(CF_ERR_RUNTIME, "failed to generate random features. max loops exceeded");
+    if (errHost == 2) throw Error(CF_ERR_RUNTIME, "feature sampler needed more engine outputs than provisioned");
+}
+
+void launchPrepareTables(hipStream_t s, int trees, int F, int T, FeatTables& tabs) {
+    if (T > kMaxThresholds) throw Error(CF_ERR_ARG, "too many thresholds (max 256)");
+    tabs.trees = trees;
+    tabs.F = F;
+    tabs.T = T;
+    tabs.feat.resize(static_cast<size_t>(trees) * F);
+    tabs.thrFT.resize(static_cast<size_t>(trees) * F * T);
+    tabs.rank.resize(static_cast<size_t>(trees) * F * T);
+    tabs.order.resize(static_cast<size_t>(trees) * F * T);
+    const int total = trees * F;
+    hipLaunchKernelGGL(k_tables_from_reference, dim3((total + 255) / 256), dim3(256), 0, s, tabs.soa.get(),
+            tabs.thrTF.get(), trees, F, T, tabs.feat.get(), tabs.thrFT.get());
+    hipLaunchKernelGGL(k_threshold_ranks, dim3((total + 255) / 256), dim3(256), 0, s, tabs.thrFT.get(), total, T,
+            tabs.rank.get(), tabs.order.get());
+}
+
+}  // namespace cf

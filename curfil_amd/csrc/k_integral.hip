@@ -1,0 +1,144 @@
+// Integral images on the device (RGBDImage::calculateIntegral,
+// image.cpp:236-277 and :531-556).
+//
+// 8-bit colour and integer depth: the reference integrates in double with a
+// Kahan term; for integer input every partial sum is an exact integer below
+// 2^53, so its result is the exact sum rounded once to float (colour) or the
+// exact integer (depth, valid).  Here the sums are exact 32-bit integer prefix
+// sums (wrapping for depth, see boxDepth), built in two passes:
+//   pass A: one thread per column walks down the rows (coalesced along x);
+//   pass B: one wave per row scans 64-pixel segments with DPP/shuffle scans.
+// Float colour (non-integer values): the reference's sequential recurrence is
+// replayed exactly, one thread per channel (used for small test images).
+#include "cf_internal.h"
+
+namespace cf {
+
+// pass A: column prefix of 5 channels (r, g, b, depth mm, valid) into tmp[5][H][W]
+__global__ void __launch_bounds__(256) k_integral_cols_u8(const uint8_t* __restrict__ rgb,
+        const uint16_t* __restrict__ depth, int W, int H, uint32_t* __restrict__ tmp) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= W) return;
+    uint32_t r = 0, g = 0, b = 0, d = 0, v = 0;
+    const size_t plane = static_cast<size_t>(W) * H;
+    for (int y = 0; y < H; y++) {
+        const size_t p = static_cast<size_t>(y) * W + x;
+        r += rgb[3 * p + 0];
+        g += rgb[3 * p + 1];
+        b += rgb[3 * p + 2];
+        const uint32_t mm = depth[p];
+        d += mm;
+        v += (mm > 0) ? 1u : 0u;
+        tmp[p] = r;
+        tmp[plane + p] = g;
+        tmp[2 * plane + p] = b;
+        tmp[3 * plane + p] = d;
+        tmp[4 * plane + p] = v;
+    }
+}
+
+// same for int32 depth (float-colour path: colour handled by the Kahan kernel)
+__global__ void __launch_bounds__(256) k_integral_cols_depth_i32(const int32_t* __restrict__ depth, int W, int H,
+        uint32_t* __restrict__ tmp) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= W) return;
+    uint32_t d = 0, v = 0;
+    const size_t plane = static_cast<size_t>(W) * H;
+    for (int y = 0; y < H; y++) {
+        const size_t p = static_cast<size_t>(y) * W + x;
+        const int32_t mm = depth[p];
+        d += static_cast<uint32_t>(mm > 0 ? mm : 0);
+        v += (mm > 0) ? 1u : 0u;
+        tmp[3 * plane + p] = d;
+        tmp[4 * plane + p] = v;
+    }
+}
+
+__device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+
+// pass B: row prefix.  blockDim = 256 (4 waves), one wave per (row, channel)
+// job; channels 0..2 -> float colour, channel 3+4 -> int2 depth.
+__global__ void __launch_bounds__(256) k_integral_rows(const uint32_t* __restrict__ tmp, int W, int H,
+        int firstChannel, float* __restrict__ col, int2* __restrict__ dep) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int numCh = 4 - firstChannel;  // jobs: colour channels [firstChannel,3) + one depth job
+    if (wave >= H * numCh) return;
+    const int y = wave / numCh;
+    const int ch = firstChannel + wave % numCh;
+    const size_t plane = static_cast<size_t>(W) * H;
+    const size_t row = static_cast<size_t>(y) * W;
+    if (ch < 3) {
+        uint32_t carry = 0;
+        for (int x0 = 0; x0 < W; x0 += 64) {
+            const int x = x0 + lane;
+            const uint32_t v = (x < W) ? tmp[ch * plane + row + x] : 0u;
+            const uint32_t s = waveInclusiveScan(v) + carry;
+            if (x < W) col[ch * plane + row + x] = __uint2float_rn(s);
+            carry = __shfl(s, 63, 64);
+        }
+    } else {
+        uint32_t cd = 0, cv = 0;
+        for (int x0 = 0; x0 < W; x0 += 64) {
+            const int x = x0 + lane;
+            const uint32_t d = (x < W) ? tmp[3 * plane + row + x] : 0u;
+            const uint32_t v = (x < W) ? tmp[4 * plane + row + x] : 0u;
+            const uint32_t sd = waveInclusiveScan(d) + cd;
+            const uint32_t sv = waveInclusiveScan(v) + cv;
+            if (x < W) dep[row + x] = make_int2(static_cast<int>(sd), static_cast<int>(sv));
+            cd = __shfl(sd, 63, 64);
+            cv = __shfl(sv, 63, 64);
+        }
+    }
+}
+
+// Sequential double Kahan integration of one float channel per thread,
+// operation by operation as image.cpp:250-277 with T = double.
+__global__ void k_integral_kahan_f32(const float* __restrict__ color, int W, int H, double* __restrict__ work,
+        float* __restrict__ col) {
+    const int ch = threadIdx.x;
+    if (ch >= 3) return;
+    const size_t plane = static_cast<size_t>(W) * H;
+    const float* in = color + ch * plane;
+    double* d = work + ch * plane;
+    float* out = col + ch * plane;
+    double c = 0.0;
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            const double above = (y > 0) ? d[(y - 1) * W + x] : 0.0;
+            const double left = (x > 0) ? d[y * W + x - 1] : 0.0;
+            const double aboveLeft = (x > 0 && y > 0) ? d[(y - 1) * W + x - 1] : 0.0;
+            const double cur = static_cast<double>(in[y * W + x]);
+            const double dy = __dsub_rn(__dadd_rn(left, above), aboveLeft) - c;
+            const double t = __dadd_rn(cur, dy);
+            c = __dsub_rn(__dsub_rn(t, cur), dy);
+            d[y * W + x] = t;
+            out[y * W + x] = __double2float_rn(t);
+        }
+    }
+}
+
+void launchIntegralU8(hipStream_t s, const uint8_t* rgb, const uint16_t* depth, int W, int H, uint32_t* tmp,
+        float* col, int2* dep) {
+    hipLaunchKernelGGL(k_integral_cols_u8, dim3((W + 255) / 256), dim3(256), 0, s, rgb, depth, W, H, tmp);
+    const int jobs = H * 4;
+    hipLaunchKernelGGL(k_integral_rows, dim3((jobs * 64 + 255) / 256), dim3(256), 0, s, tmp, W, H, 0, col, dep);
+}
+
+void launchIntegralF32(hipStream_t s, const float* color, const int32_t* depth, int W, int H, uint32_t* tmp,
+        double* work, float* col, int2* dep) {
+    hipLaunchKernelGGL(k_integral_kahan_f32, dim3(1), dim3(64), 0, s, color, W, H, work, col);
+    hipLaunchKernelGGL(k_integral_cols_depth_i32, dim3((W + 255) / 256), dim3(256), 0, s, depth, W, H, tmp);
+    const int jobs = H;  // depth job only
+    hipLaunchKernelGGL(k_integral_rows, dim3((jobs * 64 + 255) / 256), dim3(256), 0, s, tmp, W, H, 3, col, dep);
+}
+
+}  // namespace cf

@@ -1,0 +1,354 @@
+// Level-wise training driver (host side of the hot path).
+//
+// Semantics follow RandomTreeTrain::train (random_tree.h:1249-1415) and
+// ImageFeatureEvaluation::evaluateBestSplits (random_tree_image.cpp:239-502):
+//   * the root is level 1; a level equal to maxDepth is not evaluated;
+//   * one seed per (tree, level) from the tree's RandomSource, one feature
+//     table per (tree, level) generated from all of the level's samples in
+//     frontier order;
+//   * children ids are ++idNode (left, right), rolled back by 2 when a side is
+//     empty (the node stays a leaf);
+//   * a child keeps growing iff its histogram is not pure and it holds at
+//     least minSampleCount samples.
+// The device keeps the level's samples contiguous per node; evaluation runs
+// either node by node (CF_MODE_PLAIN, the reference GPU path's shape: one
+// launch sequence and one host sync per node) or for every node of every
+// tree of the level together (CF_MODE_BATCHED).
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+
+#include "cf_host.h"
+
+namespace cf {
+
+namespace {
+
+constexpr int kChunk = 4096;
+
+struct ActiveTree {
+    int job;                  // index into jobs / out
+    int idNode;               // last assigned node id
+    std::vector<int> nodes;   // frontier: node indices into out[job].nodes
+    std::vector<int> counts;  // samples per frontier node
+    int begin = 0, n = 0;     // segment of the level buffer
+};
+
+struct LevelNode {
+    int active;  // index into the active list (feature table)
+    int node;    // node index in its tree
+    int begin, count;
+    int chunkBegin, chunkEnd;
+};
+
+size_t evalBudgetBytes() {
+    const char* e = std::getenv("CURFIL_EVAL_BUDGET_MB");
+    size_t mb = e ? static_cast<size_t>(std::atoll(e)) : 2048;
+    if (mb < 16) mb = 16;
+    return mb << 20;
+}
+
+bool isPure(const uint32_t* h, int C) {
+    int nz = 0;
+    for (int c = 0; c < C; c++) {
+        if (h[c] == 0) continue;
+        if (++nz > 1) return false;
+    }
+    return nz == 1;
+}
+
+}  // namespace
+
+void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int C, std::vector<TreeJob>& jobs,
+        int mode, std::vector<TreeBuild>& out, TrainStats& stats) {
+    if (cfg.horizontal_flipping) throw Error(CF_ERR_UNSUPPORTED, "horizontal flipping is not implemented");
+    if (cfg.max_depth <= 0) throw Error(CF_ERR_ARG, "maxDepth must be positive");
+    if (C < 1 || C > kMaxClasses) throw Error(CF_ERR_ARG, "number of classes must be in [1, 32]");
+    if (cfg.feature_count == 0 || cfg.thresholds == 0) throw Error(CF_ERR_ARG, "featureCount and thresholds must be positive");
+    if (mode != CF_MODE_PLAIN && mode != CF_MODE_BATCHED) throw Error(CF_ERR_ARG, "unknown training mode");
+    const int F = static_cast<int>(cfg.feature_count);
+    const int T = cfg.thresholds;
+    hipStream_t s = ctx->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+
+    out.assign(jobs.size(), TreeBuild());
+    size_t total = 0;
+    for (auto& j : jobs) total += j.roots.size();
+    DevBuf<cf_sample> level[2];
+    level[0].resize(std::max<size_t>(total, 1));
+    level[1].resize(std::max<size_t>(total, 1));
+    DevBuf<uint8_t> dir(std::max<size_t>(total, 1));
+
+    std::vector<ActiveTree> active;
+    {
+        std::vector<cf_sample> all;
+        all.reserve(total);
+        for (size_t j = 0; j < jobs.size(); j++) {
+            TreeBuild& tb = out[j];
+            cf_tree_node root{};
+            root.id = jobs[j].treeId;
+            root.level = 1;
+            root.parent = -1;
+            root.left = root.right = -1;
+            root.num_samples = static_cast<int32_t>(jobs[j].roots.size());
+            root.is_leaf = 1;
+            tb.nodes.push_back(root);
+            tb.hist.assign(C, 0u);
+            for (const cf_sample& x : jobs[j].roots) {
+                if (x.label >= C) throw Error(CF_ERR_ARG, "sample label >= number of classes");
+                if (x.image >= static_cast<uint32_t>(images->count)) throw Error(CF_ERR_ARG, "sample image out of range");
+                if (x.x >= images->W || x.y >= images->H) throw Error(CF_ERR_ARG, "sample position outside the image");
+                tb.hist[x.label] += 1;
+                if (x.flip == 2) tb.hist[x.label] += 1;
+            }
+            if (jobs[j].roots.empty()) continue;
+            ActiveTree a;
+            a.job = static_cast<int>(j);
+            a.idNode = jobs[j].treeId;
+            a.nodes.push_back(0);
+            a.counts.push_back(static_cast<int>(jobs[j].roots.size()));
+            a.begin = static_cast<int>(all.size());
+            a.n = static_cast<int>(jobs[j].roots.size());
+            all.insert(all.end(), jobs[j].roots.begin(), jobs[j].roots.end());
+            active.push_back(std::move(a));
+        }
+        if (!all.empty()) {
+            CF_HIP(hipMemcpyAsync(level[0].get(), all.data(), all.size() * sizeof(cf_sample), hipMemcpyHostToDevice, s));
+        }
+    }
+    int cur = 0;
+
+    FeatTables tabs;
+    DevBuf<FeatGenJob> dJobs;
+    DevBuf<Chunk> dChunks;
+    DevBuf<uint32_t> dNodeHist;
+    DevBuf<int> dSlotTree;
+    DevBuf<uint32_t> dG;
+    DevBuf<FeatSummary> dSumm;
+    DevBuf<NodeBest> dBest;
+    DevBuf<double> dAllScores;
+    DevBuf<SplitRec> dSplits;
+    DevBuf<int> dChunkLeft;
+    DevBuf<uint32_t> dLeftHist;
+    DevBuf<int2> dDst;
+    const size_t perSlot = static_cast<size_t>(F) * (T + 1) * C;
+    const size_t budget = evalBudgetBytes();
+    size_t maxSlots = std::max<size_t>(1, budget / (perSlot * sizeof(uint32_t)));
+
+    for (int levelNr = 1; !active.empty(); levelNr++) {
+        if (levelNr == cfg.max_depth) break;
+        stats.levels++;
+        const int numActive = static_cast<int>(active.size());
+        // ---- feature generation, one job per tree
+        std::vector<int> seeds(numActive);
+        std::vector<FeatGenJob> fj(numActive);
+        for (int a = 0; a < numActive; a++) {
+            seeds[a] = jobs[active[a].job].rs.uniformSampler(0xFFFFFF).next();
+            fj[a].samples = level[cur].get() + active[a].begin;
+            fj[a].n = active[a].n;
+            fj[a].seed = seeds[a];
+            fj[a].outOffset = a * F;
+            fj[a].pad = 0;
+        }
+        dJobs.resize(numActive);
+        CF_HIP(hipMemcpyAsync(dJobs.get(), fj.data(), numActive * sizeof(FeatGenJob), hipMemcpyHostToDevice, s));
+        launchFeatureGeneration(s, images->desc.get(), dJobs.get(), numActive, F, T, cfg.box_radius, cfg.region_size,
+                cfg.use_depth_images, seeds.data(), tabs);
+
+        // ---- nodes and chunks of the level (slot = position in `nodes`)
+        std::vector<LevelNode> nodes;
+        std::vector<Chunk> chunks;
+        std::vector<int> chunkSlotTree;
+        for (int a = 0; a < numActive; a++) {
+            int off = active[a].begin;
+            for (size_t k = 0; k < active[a].nodes.size(); k++) {
+                LevelNode ln;
+                ln.active = a;
+                ln.node = active[a].nodes[k];
+                ln.begin = off;
+                ln.count = active[a].counts[k];
+                ln.chunkBegin = static_cast<int>(chunks.size());
+                for (int b = 0; b < ln.count; b += kChunk) {
+                    Chunk c;
+                    c.slot = static_cast<int>(nodes.size());
+                    c.tree = a;
+                    c.begin = off + b;
+                    c.count = std::min(kChunk, ln.count - b);
+                    chunks.push_back(c);
+                }
+                ln.chunkEnd = static_cast<int>(chunks.size());
+                off += ln.count;
+                nodes.push_back(ln);
+                stats.evaluations += static_cast<double>(ln.count) * F * T;
+            }
+        }
+        const int numNodes = static_cast<int>(nodes.size());
+        std::vector<NodeBest> best(numNodes);
+
+        // ---- evaluation
+        auto evalSlots = [&](int s0, int s1) {
+            const int ns = s1 - s0;
+            const int c0 = nodes[s0].chunkBegin, c1 = nodes[s1 - 1].chunkEnd;
+            std::vector<Chunk> bc(chunks.begin() + c0, chunks.begin() + c1);
+            for (auto& c : bc) c.slot -= s0;
+            std::vector<uint32_t> nh(static_cast<size_t>(ns) * C);
+            std::vector<int> st(ns);
+            for (int i = 0; i < ns; i++) {
+                const LevelNode& ln = nodes[s0 + i];
+                std::memcpy(&nh[static_cast<size_t>(i) * C], &out[active[ln.active].job].hist[static_cast<size_t>(ln.node) * C],
+                        C * sizeof(uint32_t));
+                st[i] = ln.active;
+            }
+            dChunks.resize(bc.size());
+            dNodeHist.resize(nh.size());
+            dSlotTree.resize(ns);
+            dG.resize(static_cast<size_t>(ns) * perSlot);
+            dSumm.resize(static_cast<size_t>(ns) * F);
+            dBest.resize(ns);
+            CF_HIP(hipMemcpyAsync(dChunks.get(), bc.data(), bc.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
+            CF_HIP(hipMemcpyAsync(dNodeHist.get(), nh.data(), nh.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            CF_HIP(hipMemcpyAsync(dSlotTree.get(), st.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+            CF_HIP(hipMemsetAsync(dG.get(), 0, static_cast<size_t>(ns) * perSlot * sizeof(uint32_t), s));
+            launchEvalHistograms(s, images->desc.get(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()),
+                    tabs, C, dG.get());
+            launchScores(s, dG.get(), dNodeHist.get(), dSlotTree.get(), ns, tabs, C, dSumm.get(), nullptr, -1);
+            launchReduce(s, dSumm.get(), ns, F, dBest.get());
+            CF_HIP(hipMemcpyAsync(&best[s0], dBest.get(), ns * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
+            CF_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < ns; i++) {
+                if (!best[s0 + i].flag) continue;
+                // near-tie band: replay the reference's sequential scan for this node
+                dAllScores.resize(static_cast<size_t>(T) * F);
+                launchScores(s, dG.get(), dNodeHist.get(), dSlotTree.get(), ns, tabs, C, dSumm.get(), dAllScores.get(), i);
+                launchSequentialArgmax(s, dAllScores.get(), T, F, dBest.get() + i);
+                CF_HIP(hipMemcpyAsync(&best[s0 + i], dBest.get() + i, sizeof(NodeBest), hipMemcpyDeviceToHost, s));
+                CF_HIP(hipStreamSynchronize(s));
+                stats.replays++;
+            }
+        };
+        if (mode == CF_MODE_PLAIN) {
+            for (int i = 0; i < numNodes; i++) evalSlots(i, i + 1);
+        } else {
+            for (int s0 = 0; s0 < numNodes;) {
+                const int s1 = static_cast<int>(std::min<size_t>(numNodes, s0 + maxSlots));
+                evalSlots(s0, s1);
+                s0 = s1;
+            }
+        }
+
+        // ---- split records
+        std::vector<int8_t> soa(static_cast<size_t>(numActive) * kFeatParams * F);
+        std::vector<float> thrTF(static_cast<size_t>(numActive) * T * F);
+        CF_HIP(hipMemcpyAsync(soa.data(), tabs.soa.get(), soa.size(), hipMemcpyDeviceToHost, s));
+        CF_HIP(hipMemcpyAsync(thrTF.data(), tabs.thrTF.get(), thrTF.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        CF_HIP(hipStreamSynchronize(s));
+        std::vector<SplitRec> splits(numNodes);
+        for (int i = 0; i < numNodes; i++) {
+            const int a = nodes[i].active, f = best[i].feat, t = best[i].thresh;
+            SplitRec sr{};
+            for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = soa[(static_cast<size_t>(a) * kFeatParams + p) * F + f];
+            sr.threshold = thrTF[(static_cast<size_t>(a) * T + t) * F + f];
+            splits[i] = sr;
+        }
+        dSplits.resize(numNodes);
+        dChunkLeft.resize(chunks.size());
+        dLeftHist.resize(static_cast<size_t>(numNodes) * C);
+        dChunks.resize(chunks.size());
+        CF_HIP(hipMemcpyAsync(dSplits.get(), splits.data(), numNodes * sizeof(SplitRec), hipMemcpyHostToDevice, s));
+        CF_HIP(hipMemcpyAsync(dChunks.get(), chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
+        CF_HIP(hipMemsetAsync(dLeftHist.get(), 0, static_cast<size_t>(numNodes) * C * sizeof(uint32_t), s));
+        launchPartitionDirections(s, images->desc.get(), level[cur].get(), dChunks.get(), static_cast<int>(chunks.size()),
+                dSplits.get(), C, dir.get(), dChunkLeft.get(), dLeftHist.get());
+        std::vector<int> chunkLeft(chunks.size());
+        std::vector<uint32_t> leftHist(static_cast<size_t>(numNodes) * C);
+        CF_HIP(hipMemcpyAsync(chunkLeft.data(), dChunkLeft.get(), chunks.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        CF_HIP(hipMemcpyAsync(leftHist.data(), dLeftHist.get(), leftHist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        CF_HIP(hipStreamSynchronize(s));
+
+        // ---- tree growth on the host, next frontier and sample destinations
+        std::vector<int2> dst(chunks.size(), make_int2(-1, -1));
+        std::vector<ActiveTree> next;
+        int nextOff = 0;
+        int slot = 0;
+        for (int a = 0; a < numActive; a++) {
+            ActiveTree& at = active[a];
+            TreeBuild& tb = out[at.job];
+            ActiveTree na;
+            na.job = at.job;
+            na.begin = nextOff;
+            for (size_t k = 0; k < at.nodes.size(); k++, slot++) {
+                const LevelNode& ln = nodes[slot];
+                int nL = 0;
+                for (int c = ln.chunkBegin; c < ln.chunkEnd; c++) nL += chunkLeft[c];
+                const int nR = ln.count - nL;
+                const int leftId = ++at.idNode;
+                const int rightId = ++at.idNode;
+                if (nL == 0 || nR == 0) {
+                    at.idNode -= 2;
+                    continue;
+                }
+                const int pi = ln.node;
+                const int li = static_cast<int>(tb.nodes.size());
+                for (int side = 0; side < 2; side++) {
+                    cf_tree_node cn{};
+                    cn.id = side == 0 ? leftId : rightId;
+                    cn.level = tb.nodes[pi].level + 1;
+                    cn.parent = pi;
+                    cn.left = cn.right = -1;
+                    cn.num_samples = side == 0 ? nL : nR;
+                    cn.is_leaf = 1;
+                    tb.nodes.push_back(cn);
+                    for (int c = 0; c < C; c++) {
+                        const uint32_t l = leftHist[static_cast<size_t>(slot) * C + c];
+                        tb.hist.push_back(side == 0 ? l : tb.hist[static_cast<size_t>(pi) * C + c] - l);
+                    }
+                }
+                cf_tree_node& p = tb.nodes[pi];
+                p.is_leaf = 0;
+                p.left = li;
+                p.right = li + 1;
+                p.feature_id = best[slot].feat;
+                p.threshold = splits[slot].threshold;
+                p.score = best[slot].score;
+                for (int q = 0; q < kFeatParams; q++) p.feature[q] = splits[slot].feat.p[q];
+                int childBegin[2] = {-1, -1};
+                for (int side = 0; side < 2; side++) {
+                    const int ci = li + side;
+                    const int cnt = side == 0 ? nL : nR;
+                    if (!isPure(&tb.hist[static_cast<size_t>(ci) * C], C) &&
+                            cnt >= static_cast<int>(cfg.min_sample_count)) {
+                        childBegin[side] = nextOff;
+                        na.nodes.push_back(ci);
+                        na.counts.push_back(cnt);
+                        nextOff += cnt;
+                    }
+                }
+                int doneL = 0, doneR = 0;
+                for (int c = ln.chunkBegin; c < ln.chunkEnd; c++) {
+                    const int cl = chunkLeft[c];
+                    const int cr = chunks[c].count - cl;
+                    dst[c].x = childBegin[0] >= 0 ? childBegin[0] + doneL : -1;
+                    dst[c].y = childBegin[1] >= 0 ? childBegin[1] + doneR : -1;
+                    doneL += cl;
+                    doneR += cr;
+                }
+            }
+            na.n = nextOff - na.begin;
+            na.idNode = at.idNode;
+            if (!na.nodes.empty()) next.push_back(std::move(na));
+        }
+        if (nextOff > 0) {
+            dDst.resize(dst.size());
+            CF_HIP(hipMemcpyAsync(dDst.get(), dst.data(), dst.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+            launchPartitionScatter(s, level[cur].get(), dChunks.get(), static_cast<int>(chunks.size()), dir.get(),
+                    dDst.get(), level[1 - cur].get());
+            cur = 1 - cur;
+        }
+        CF_HIP(hipStreamSynchronize(s));
+        active.swap(next);
+    }
+    CF_HIP(hipStreamSynchronize(s));
+    stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace cf

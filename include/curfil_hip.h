@@ -1,0 +1,201 @@
+/*
+ * curfil_hip.h -- C ABI of the MI355X-native CURFIL hot path.
+ *
+ * Plain C, plain pointers and sizes, no exceptions across the boundary: every
+ * call returns a status (CF_OK == 0) and cf_last_error() gives the message of
+ * the last failure on that context.  Host buffers are owned by the caller;
+ * device buffers are owned by the handles.  One context per host thread (a
+ * context serialises its own calls with an internal lock).
+ *
+ * Each entry point names the reference interface it replaces
+ * (deeplearningais/curfil, src/curfil/...).
+ */
+#ifndef CURFIL_HIP_H
+#define CURFIL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CF_OK = 0,
+    CF_ERR_ARG = -1,         /* invalid argument (reference: assert / runtime_error on bad input) */
+    CF_ERR_HIP = -2,         /* HIP runtime failure (reference: cudaSafeCall, utils.h:48) */
+    CF_ERR_RUNTIME = -3,     /* algorithmic runtime_error of the reference (message says which) */
+    CF_ERR_UNSUPPORTED = -4, /* option not implemented on this path */
+    CF_ERR_NOMEM = -5        /* device memory exhausted / HBM budget exceeded */
+};
+
+typedef struct cf_ctx cf_ctx;
+typedef struct cf_images cf_images;
+typedef struct cf_forest cf_forest;
+
+/* One training pixel; the fields of Samples (random_tree_image.h:922-978) and
+ * PixelInstance (random_tree_image.h:129-392).  depth_mm <= 0 means invalid.
+ * flip: 0 NoFlip, 1 Flip, 2 Both (random_tree.h:58-60). */
+typedef struct cf_sample {
+    uint16_t x, y;
+    uint32_t image;
+    int32_t depth_mm;
+    uint8_t label, flip, pad0, pad1;
+} cf_sample;
+
+/* One node of a trained tree, in creation order (root first, then per level
+ * the children pairs of the level's nodes in frontier order).  Mirrors
+ * RandomTree (random_tree.h:499-1003) + SplitFunction (random_tree.h:89-181).
+ * feature[] is the 11-byte SoA column of ImageFeaturesAndThresholds
+ * (random_tree_image.h:702-773): type, offset1 x/y, offset2 x/y,
+ * region1 x/y, region2 x/y, channel1, channel2. */
+typedef struct cf_tree_node {
+    int32_t id, level, parent, left, right;
+    int32_t num_samples;
+    int32_t feature_id;
+    float threshold;
+    double score;
+    int8_t feature[11];
+    uint8_t is_leaf;
+    int32_t pad;
+} cf_tree_node;
+
+/* TrainingConfiguration (random_tree.h:222-273), the fields the hot path reads. */
+typedef struct cf_train_config {
+    int32_t random_seed;
+    uint32_t samples_per_image;
+    uint32_t feature_count;
+    uint32_t min_sample_count;
+    int32_t max_depth;
+    uint16_t box_radius;
+    uint16_t region_size;
+    uint16_t thresholds;
+    uint16_t pad0;
+    int32_t num_threads;
+    int32_t max_images;
+    int32_t use_depth_images;
+    int32_t horizontal_flipping;
+    int32_t subsampling_pixel_uniform; /* 0: classUniform (default) */
+    int32_t num_ignored;
+    int32_t ignored_labels[8];
+} cf_train_config;
+
+/* Training execution mode. */
+enum {
+    CF_MODE_PLAIN = 0,   /* one node per launch, host sync per node (the reference GPU path's shape) */
+    CF_MODE_BATCHED = 1  /* every open node of every tree at a level in one launch sequence */
+};
+
+/* ---- context -------------------------------------------------------------- */
+/* replaces ImageFeatureEvaluation::initDevice / selectDevice (random_tree_image_gpu.cu:1722-1760) */
+int cf_ctx_create(int device, cf_ctx** out);
+int cf_ctx_destroy(cf_ctx* ctx);
+const char* cf_last_error(const cf_ctx* ctx);
+int cf_ctx_synchronize(cf_ctx* ctx);
+
+/* ---- images (device-resident integral images) ----------------------------- */
+/* replaces RGBDImage + calculateIntegral (image.cpp:531-556) and the image
+ * cache upload (random_tree_image_gpu.cu:580-1059).  All images of one set
+ * have the same size (loadImages pads to the maximum size, image.cpp:910-955). */
+int cf_images_create(cf_ctx* ctx, int count, int width, int height, cf_images** out);
+int cf_images_destroy(cf_images* images);
+/* 8-bit RGB (interleaved [H][W][3]) + uint16 depth in mm (0 = missing). The
+ * integrals are built on the device (exact integer sums, colour rounded once
+ * to float: identical to the reference's double Kahan integration for
+ * integer input). */
+int cf_images_set_rgbd_u8(cf_images* images, int index, const uint8_t* rgb, const uint16_t* depth_mm);
+/* float colour planes [3][H][W] + int32 depth in mm (<= 0 invalid).  The
+ * colour integral follows the reference's sequential double Kahan recurrence
+ * exactly (image.cpp:236-277). */
+int cf_images_set_rgbd_f32(cf_images* images, int index, const float* color, const int32_t* depth_mm);
+/* label image [H][W] (host copy, read by pixel sampling) */
+int cf_images_set_labels(cf_images* images, int index, const uint8_t* labels);
+/* read back the integrals of one image: color [3][H][W], depth sum and valid count [H][W] */
+int cf_images_get_integrals(cf_images* images, int index, float* color, int32_t* depth, int32_t* valid);
+
+/* ---- training seams -------------------------------------------------------- */
+/* replaces ImageFeatureEvaluation::generateRandomFeatures (CPU generator,
+ * random_tree_image.cpp:577-700, sorted as in :504-566).  features_out
+ * [11][F] int8, thresholds_out [T][F] float.  Runs on the device. */
+int cf_generate_features(cf_ctx* ctx, cf_images* images, const cf_sample* samples, int n, int seed,
+        int num_features, int num_thresholds, int box_radius, int region_size, int use_depth_images,
+        int8_t* features_out, float* thresholds_out);
+
+/* feature responses (double) of n samples for F features -> out[F][n];
+ * replaces featureResponseKernel (random_tree_image_gpu.cu:1455-1545) as
+ * observed by random_tree_image_gpu_test.cu. */
+int cf_feature_responses(cf_ctx* ctx, cf_images* images, const cf_sample* samples, int n,
+        const int8_t* features, int num_features, double* out);
+
+/* per-node left/right class counters, layout [F][T][C][2] (counters(feat,
+ * thresh, label, lr), random_tree_image_gpu.cu:1069-1085); replaces
+ * calculateFeatureResponsesAndHistograms (random_tree_image_gpu.cu:1958-2135). */
+int cf_node_counters(cf_ctx* ctx, cf_images* images, const cf_sample* samples, int n,
+        const int8_t* features, const float* thresholds, int num_features, int num_thresholds,
+        int num_classes, uint32_t* counters);
+
+/* normalized information-gain scores [T][F] of one node from its counters
+ * [F][T][C][2] and node histogram [C]; replaces calculateScores
+ * (random_tree_image.cpp:188-237 / scoreKernel random_tree_image_gpu.cu:1555-1601). */
+int cf_scores(cf_ctx* ctx, const uint32_t* counters, const uint32_t* histogram, int num_features,
+        int num_thresholds, int num_classes, double* scores);
+
+/* best split of one node (plain path): counters, scores and the reference's
+ * sequential argmax with its 1e-13 tie rule (random_tree_image.cpp:445-457,
+ * random_tree.cpp:36-52), all on the device. */
+int cf_node_best_split(cf_ctx* ctx, cf_images* images, const cf_sample* samples, int n,
+        const int8_t* features, const float* thresholds, int num_features, int num_thresholds,
+        int num_classes, const uint32_t* histogram, int* best_feature, int* best_threshold, double* best_score);
+
+/* ---- forest training ------------------------------------------------------- */
+/* RandomForestImage(treeCount, configuration).train(images, numLabels)
+ * (random_forest_image.cpp:132-182): per-tree class-uniform pixel sampling on
+ * the host, then level-wise training on the device in the given mode. */
+int cf_forest_train(cf_ctx* ctx, cf_images* images, const cf_train_config* config, int num_trees,
+        int num_labels, int mode, cf_forest** out);
+/* one tree from given root samples (RandomTreeImage::doTrain,
+ * random_tree_image.cpp:714-730); rs_seed seeds the tree's RandomSource. */
+int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* images, const cf_train_config* config, int tree_id,
+        int num_classes, int rs_seed, const cf_sample* roots, int n, int mode, cf_forest** out);
+/* build a forest from node tables (e.g. imported trees); histograms [nodes][C] */
+int cf_forest_create(cf_ctx* ctx, int num_trees, const int* num_nodes, const cf_tree_node* nodes,
+        const uint32_t* histograms, int num_classes, const uint32_t* class_prior, cf_forest** out);
+int cf_forest_destroy(cf_forest* forest);
+int cf_forest_num_trees(const cf_forest* forest);
+int cf_forest_num_classes(const cf_forest* forest);
+int cf_forest_tree_size(const cf_forest* forest, int tree);
+int cf_forest_get_tree(const cf_forest* forest, int tree, cf_tree_node* nodes, uint32_t* histograms);
+int cf_forest_get_root_samples(const cf_forest* forest, int tree, cf_sample* out, int capacity);
+int cf_forest_get_prior(const cf_forest* forest, uint32_t* prior);
+/* training statistics of the last train call: evaluated feature-threshold
+ * pairs, device seconds, per-level timing count */
+int cf_forest_train_stats(const cf_forest* forest, double* evaluations, double* seconds);
+
+/* RandomForestImage::normalizeHistograms (random_forest_image.cpp:366-376)
+ * and upload of the packed forest for prediction. */
+int cf_forest_normalize(cf_forest* forest, double histogram_bias, int use_labels_prior);
+/* leaf posteriors [nodes][C] (double) of one tree, after normalize */
+int cf_forest_get_posteriors(const cf_forest* forest, int tree, double* out);
+
+/* ---- prediction ------------------------------------------------------------- */
+/* RandomForestImage::predict (random_forest_image.cpp:258-352) with the CPU
+ * branch's arithmetic (:307-345): probs [C][H][W] float, labels [H][W] uint8.
+ * Either output may be NULL. */
+int cf_predict(cf_ctx* ctx, cf_forest* forest, cf_images* images, int index, float* probs, uint8_t* labels);
+/* several frames of one image set in one launch sequence; outputs stacked per frame */
+int cf_predict_batch(cf_ctx* ctx, cf_forest* forest, cf_images* images, const int* indices, int count,
+        float* probs, uint8_t* labels);
+
+/* ---- synthetic workload (benchmark / tests; no counterpart in the reference) -- */
+/* One labelled RGB-D frame: Voronoi labels (site k -> label k mod (classes+1),
+ * 0 = void), per-class colour means from class_seed plus N(0, 20) noise,
+ * per-cell tilted depth planes in [500, 5000] mm with a fraction zero_frac of
+ * missing (0) pixels.  rgb [H][W][3], depth [H][W], labels [H][W]. */
+int cf_synth_rgbd(uint32_t seed, uint32_t class_seed, int width, int height, int sites, int classes,
+        double zero_frac, uint8_t* rgb, uint16_t* depth, uint8_t* labels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CURFIL_HIP_H */

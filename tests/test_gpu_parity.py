@@ -1,0 +1,279 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: bit-exact for integrals, responses (double), counters, feature tables,
+chosen splits, tree structure and labels; scores compared against the oracle
+(FP64, glibc log2) with the maximum difference stated; posteriors within
+1e-5 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import curfil_amd as cf
+import oracle as orc
+from helpers import (CONFIG0, GOLDEN_FEATURE_HASH, boost_feature_hash, config0_dataset, golden_hash_dataset,
+                     random_features, random_samples, random_u8_dataset)
+
+pytestmark = pytest.mark.gpu
+
+
+def _images_u8(ctx, rgb, depth, labels=None):
+    n, H, W, _ = rgb.shape
+    im = cf.Images(ctx, n, W, H)
+    for i in range(n):
+        im.set_rgbd_u8(i, rgb[i], depth[i])
+        if labels is not None:
+            im.set_labels(i, labels[i])
+    return im
+
+
+def _images_f32(ctx, color, depth):
+    n, _, H, W = color.shape
+    im = cf.Images(ctx, n, W, H)
+    for i in range(n):
+        im.set_rgbd_f32(i, color[i], depth[i])
+    return im
+
+
+def _same_float_bits(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint32 if a.dtype == np.float32 else np.uint64),
+                          np.asarray(b).view(np.uint32 if b.dtype == np.float32 else np.uint64))
+
+
+# ------------------------------------------------------------------ integrals
+@pytest.mark.parametrize("W,H", [(160, 120), (640, 480), (37, 5)])
+def test_integrals_u8_bit_exact(ctx, W, H):
+    rgb, depth, _ = random_u8_dataset(2, W, H, seed=W)
+    im = _images_u8(ctx, rgb, depth)
+    ds = orc.Dataset.from_u8(rgb, depth)
+    for i in range(2):
+        col, dep, val = im.integrals(i)
+        ocol, odep, oval = ds.integrals(i)
+        assert _same_float_bits(col, ocol)
+        assert np.array_equal(dep, odep) and np.array_equal(val, oval)
+
+
+def test_integrals_f32_kahan_bit_exact(ctx):
+    rng = np.random.default_rng(3)
+    color = (rng.integers(0, 1000, (1, 3, 48, 64)).astype(np.float32) / np.float32(100.0))
+    depth = rng.integers(0, 5000, (1, 48, 64)).astype(np.int32)
+    im = _images_f32(ctx, color, depth)
+    col, dep, val = im.integrals(0)
+    ocol, odep, oval = orc.Dataset(color, depth).integrals(0)
+    assert _same_float_bits(col, ocol)
+    assert np.array_equal(dep, odep) and np.array_equal(val, oval)
+
+
+def test_depth_integral_wraps_but_box_sums_stay_exact(ctx):
+    # 1280x960 at 5000 mm overflows 32-bit integral entries (the reference's int
+    # integral overflows here); box sums from the wrapped entries are exact.
+    W, H = 1280, 960
+    rgb = np.zeros((1, H, W, 3), np.uint8)
+    depth = np.full((1, H, W), 5000, np.uint16)
+    im = _images_u8(ctx, rgb, depth)
+    _, dep, val = im.integrals(0)
+    s = (np.int64(dep[H - 1, W - 1]) - dep[H - 11, W - 1] - dep[H - 1, W - 11] + dep[H - 11, W - 11]) % (1 << 32)
+    assert s == 100 * 5000
+    f = np.array([0, 0, 0, 5, 5, 3, 3, 3, 3, 0, 0], np.int8)[:, None]
+    samples = cf.make_samples([W - 20], [H - 20], [0], [5000], [0])
+    r = cf.feature_responses(ctx, im, samples, f)
+    assert r[0, 0] == 0.0
+
+
+# ------------------------------------------------------------------ responses / counters known answers
+def test_known_answer_complex_case(ctx):
+    from test_oracle_pins import COMPLEX_COUNTERS, COMPLEX_RESPONSES, complex_case
+    color, depth, feats, thr, samples = complex_case()
+    im = _images_f32(ctx, color, depth)
+    r = cf.feature_responses(ctx, im, samples, feats)
+    assert np.array_equal(np.isnan(r), np.isnan(COMPLEX_RESPONSES))
+    assert np.array_equal(r[~np.isnan(r)], COMPLEX_RESPONSES[~np.isnan(COMPLEX_RESPONSES)])
+    c = cf.node_counters(ctx, im, samples, feats, thr, 2)   # [F][T][C][2]
+    for (f, t), per_label in COMPLEX_COUNTERS.items():
+        for label, lr in enumerate(per_label):
+            assert tuple(c[f, t, label]) == lr
+
+
+def test_known_answer_depth_simple(ctx):
+    W, H = 16, 20
+    depth = np.zeros((1, H, W), np.int32)
+    depth[0, 3, 7], depth[0, 4, 7], depth[0, 4, 8], depth[0, 2, 3] = 1500, 1700, 4500, 3900
+    im = _images_f32(ctx, np.zeros((1, 3, H, W), np.float32), depth)
+    f = np.array([0, 1, 1, -3, -1, 2, 2, 1, 1, 0, 0], np.int8)[:, None]
+    thr = ((np.arange(100) - 50) / np.float32(10.0)).astype(np.float32)[:, None]
+    s = cf.make_samples([6, 6], [3, 3], [0, 0], [1000, 1000], [0, 1])
+    r = cf.feature_responses(ctx, im, s, f)
+    assert abs(r[0, 0] - ((1.5 + 1.7 + 4.5) / 3 - 3.9)) < 1e-12
+    c = cf.node_counters(ctx, im, s, f, thr, 2)
+    for label in range(2):
+        assert np.array_equal(c[0, :, label, 0], (np.arange(100) >= 37).astype(np.uint32))
+        assert np.all(c[0, :, label, 0] + c[0, :, label, 1] == 1)
+
+
+# ------------------------------------------------------------------ random parity
+@pytest.fixture(scope="module")
+def rand_data(ctx):
+    rgb, depth, labels = random_u8_dataset(3, 160, 120, seed=11)
+    im = _images_u8(ctx, rgb, depth, labels)
+    ds = orc.Dataset.from_u8(rgb, depth, labels)
+    return rgb, depth, labels, im, ds
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_responses_bit_exact(ctx, rand_data, seed):
+    rgb, depth, labels, im, ds = rand_data
+    rng = np.random.default_rng(seed)
+    feats = random_features(rng, 64, box_radius=100 if seed else 40, region_size=20)
+    s = random_samples(rng, 500, 3, 160, 120, depth.astype(np.int32))
+    s["flip"] = rng.integers(0, 2, len(s))
+    r = cf.feature_responses(ctx, im, s, feats)
+    o = ds.responses(s, feats)
+    assert np.array_equal(np.isnan(r), np.isnan(o))
+    assert _same_float_bits(np.nan_to_num(r), np.nan_to_num(o))
+
+
+@pytest.mark.parametrize("seed,n,T", [(0, 1, 5), (1, 777, 10), (2, 9000, 32), (3, 300, 1)])
+def test_random_counters_exact(ctx, rand_data, seed, n, T):
+    rgb, depth, labels, im, ds = rand_data
+    rng = np.random.default_rng(seed)
+    F = 40
+    feats = random_features(rng, F)
+    s = random_samples(rng, n, 3, 160, 120, depth.astype(np.int32), classes=6)
+    # thresholds: float-rounded responses of random samples, plus duplicates
+    resp = ds.responses(random_samples(rng, T, 3, 160, 120, depth.astype(np.int32)), feats)
+    thr = np.nan_to_num(resp.T, nan=0.0).astype(np.float32)
+    if T > 2:
+        thr[1] = thr[0]
+    c = cf.node_counters(ctx, im, s, feats, thr, 6)
+    o = ds.node_counters(s, feats, thr, 6)          # [C][F][T][2]
+    assert np.array_equal(c, o.transpose(1, 2, 0, 3))
+
+
+def test_scores_against_oracle(ctx, rand_data):
+    rgb, depth, labels, im, ds = rand_data
+    rng = np.random.default_rng(5)
+    F, T, C = 60, 16, 6
+    feats = random_features(rng, F)
+    s = random_samples(rng, 2000, 3, 160, 120, depth.astype(np.int32), labels=labels)
+    resp = ds.responses(random_samples(rng, T, 3, 160, 120, depth.astype(np.int32)), feats)
+    thr = np.nan_to_num(resp.T, nan=0.0).astype(np.float32)
+    o_cnt = ds.node_counters(s, feats, thr, C)
+    hist = np.bincount(s["label"], minlength=C).astype(np.uint32)
+    o = orc.scores_from_counters(o_cnt, hist)
+    g = cf.scores(ctx, o_cnt.transpose(1, 2, 0, 3), hist)
+    err = np.abs(g - o).max()
+    print(f"max |score_gpu - score_oracle| = {err:.3e}; bit-equal fraction {np.mean(g == o):.4f}")
+    assert err <= 4e-16
+
+
+def test_node_best_split_matches_oracle(ctx, rand_data):
+    rgb, depth, labels, im, ds = rand_data
+    rng = np.random.default_rng(9)
+    for trial in range(12):
+        F, T, C = int(rng.integers(1, 50)), int(rng.integers(1, 21)), 6
+        n = int(rng.integers(2, 3000))
+        feats = random_features(rng, F)
+        s = random_samples(rng, n, 3, 160, 120, depth.astype(np.int32), labels=labels)
+        resp = ds.responses(random_samples(rng, T, 3, 160, 120, depth.astype(np.int32)), feats)
+        thr = np.nan_to_num(resp.T, nan=0.0).astype(np.float32)
+        hist = np.bincount(s["label"], minlength=C).astype(np.uint32)
+        o = orc.scores_from_counters(ds.node_counters(s, feats, thr, C), hist)
+        of, ot, osc = orc.argmax(o)
+        gf, gt, gsc = cf.node_best_split(ctx, im, s, feats, thr, C, hist)
+        assert (gf, gt) == (of, ot), trial
+        assert abs(gsc - osc) <= 4e-16
+
+
+# ------------------------------------------------------------------ feature generation
+def test_golden_feature_hash_on_device(ctx):
+    color, depth, samples = golden_hash_dataset()
+    im = _images_f32(ctx, color, depth)
+    feats, thr = cf.generate_features(ctx, im, samples, seed=9, num_features=500, num_thresholds=50,
+                                      box_radius=127, region_size=16)
+    assert boost_feature_hash(feats, thr) == GOLDEN_FEATURE_HASH
+
+
+@pytest.mark.parametrize("seed,n,F,T", [(17, 4000, 100, 10), (3, 1, 20, 4), (123456, 50, 300, 32), (99, 2500, 2000, 20)])
+def test_feature_generation_matches_oracle(ctx, seed, n, F, T):
+    rgb, depth, labels = config0_dataset()
+    im = _images_u8(ctx, rgb, depth, labels)
+    ds = orc.Dataset.from_u8(rgb, depth, labels)
+    rng = np.random.default_rng(seed)
+    s = random_samples(rng, n, 4, 160, 120, depth.astype(np.int32), labels=labels)
+    s = s[s["depth_mm"] > 0] if n > 1 else s
+    if len(s) == 0 or s["depth_mm"].min() <= 0:
+        s = s[:0]
+        s = random_samples(rng, 1, 4, 160, 120, np.full_like(depth, 1000, dtype=np.int32))
+    gf, gt = cf.generate_features(ctx, im, s, seed, F, T, 60, 10)
+    of, ot = ds.generate_features(s, seed, F, T, 60, 10)
+    assert np.array_equal(gf, of)
+    assert _same_float_bits(gt, ot)
+
+
+# ------------------------------------------------------------------ training
+@pytest.fixture(scope="module")
+def config0(ctx):
+    rgb, depth, labels = config0_dataset()
+    im = _images_u8(ctx, rgb, depth, labels)
+    ds = orc.Dataset.from_u8(rgb, depth, labels)
+    return rgb, depth, labels, im, ds
+
+
+def _assert_trees_equal(gn, gh, on, oh):
+    assert len(gn) == len(on)
+    for k in ("id", "level", "parent", "left", "right", "num_samples", "is_leaf"):
+        assert np.array_equal(gn[k], on[k]), k
+    inner = gn["is_leaf"] == 0
+    assert np.array_equal(gn["feature_id"][inner], on["feature_id"][inner])
+    assert np.array_equal(gn["feature"][inner], on["feature"][inner])
+    assert _same_float_bits(gn["threshold"][inner], on["threshold"][inner])
+    assert np.abs(gn["score"][inner] - on["score"][inner]).max(initial=0) <= 4e-16
+    assert np.array_equal(gh, oh)
+
+
+@pytest.mark.parametrize("mode", [cf.MODE_PLAIN, cf.MODE_BATCHED])
+def test_forest_training_matches_oracle(ctx, config0, mode):
+    rgb, depth, labels, im, ds = config0
+    cfg = cf.TrainingConfiguration(**CONFIG0)
+    forest = cf.RandomForestImage.train(ctx, im, cfg, tree_count=1, num_labels=6, mode=mode)
+    ocfg = orc.config(**{k: v for k, v in CONFIG0.items()})
+    of = ds.train_forest(ocfg, 1, 6)
+    assert np.array_equal(forest.root_samples(0), of.roots(0))
+    assert np.array_equal(forest.prior(), of.prior())
+    gn, gh = forest.tree(0)
+    on, oh = of.tree(0)
+    print(f"mode {mode}: {len(gn)} nodes, depth {gn['level'].max()}, stats {forest.train_stats()}")
+    _assert_trees_equal(gn, gh, on, oh)
+
+
+def test_three_tree_forest_and_prediction_match_oracle(ctx, config0):
+    rgb, depth, labels, im, ds = config0
+    params = dict(CONFIG0)
+    params.update(samplesPerImage=3000, maxDepth=10, featureCount=200, thresholds=20)
+    cfg = cf.TrainingConfiguration(**params)
+    forest = cf.RandomForestImage.train(ctx, im, cfg, tree_count=3, num_labels=6, mode=cf.MODE_BATCHED)
+    of = ds.train_forest(orc.config(**params), 3, 6)
+    trees, posts = [], []
+    forest.normalizeHistograms(0.0, True)
+    prior = of.prior()
+    for t in range(3):
+        gn, gh = forest.tree(t)
+        on, oh = of.tree(t)
+        _assert_trees_equal(gn, gh, on, oh)
+        post = np.array([orc.normalize_histogram(h, prior, 0.0, True) for h in oh])
+        assert np.array_equal(forest.posteriors(t), post)
+        trees.append(on)
+        posts.append(post)
+    for img in range(4):
+        gl, gp = forest.predict(im, img)
+        ol, op = ds.predict(img, trees, posts, forest.num_classes)
+        assert np.array_equal(gl, ol)
+        assert np.array_equal(np.isnan(gp), np.isnan(op))
+        assert _same_float_bits(np.nan_to_num(gp), np.nan_to_num(op))
+    bl, bp = forest.predict_batch(im, [3, 1, 2])
+    for k, img in enumerate([3, 1, 2]):
+        sl, sp = forest.predict(im, img)
+        assert np.array_equal(bl[k], sl) and _same_float_bits(np.nan_to_num(bp[k]), np.nan_to_num(sp))
+    acc = np.mean(forest.predict(im, 0)[0][labels[0] > 0] == labels[0][labels[0] > 0])
+    print(f"training-image pixel accuracy (no void): {acc:.3f}")
