@@ -99,14 +99,20 @@ def _ptr(a):
 
 
 class Context:
-    """One HIP device + stream (cf_ctx)."""
+    """One HIP device + stream (cf_ctx).  Handles created on it (images,
+    forests) are released before the context itself."""
 
     def __init__(self, device=0):
+        import weakref
         h = ctypes.c_void_p()
         st = lib.cf_ctx_create(device, ctypes.byref(h))
         if st != CF_OK:
             raise CurfilError(st, f"cf_ctx_create(device={device}) failed")
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def check(self, st):
         if st != CF_OK:
@@ -117,6 +123,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             lib.cf_ctx_destroy(self.h)
             self.h = None
 
@@ -135,6 +143,7 @@ class Images:
         h = ctypes.c_void_p()
         ctx.check(lib.cf_images_create(ctx.h, count, width, height, ctypes.byref(h)))
         self.h = h
+        ctx._adopt(self)
 
     def set_rgbd_u8(self, index, rgb, depth):
         rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
@@ -284,6 +293,7 @@ class RandomForestImage:
 
     def __init__(self, ctx, h):
         self.ctx, self.h = ctx, h
+        ctx._adopt(self)
 
     @classmethod
     def train(cls, ctx, images, configuration, tree_count, num_labels, mode=MODE_BATCHED):

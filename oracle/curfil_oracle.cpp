@@ -19,12 +19,12 @@
 //     4123659995 for the 10000th output of seed 5489.
 //   * integrals: image_test.cpp:87-279 known answers, brute force.
 //   * responses / counters: random_tree_image_gpu_test.cu:318-866 known answers.
-//   * scores: random_tree_image_gpu_test.cu:106-240 known answers and an
-//     exact-rational (Python fractions + mpmath-free) recomputation.
+//   * scores: random_tree_image_gpu_test.cu:106-240 known answers and a
+//     50-digit decimal recomputation (tests/test_oracle_pins.py).
 //   * argmax tie rule: constructed tie / near-tie cases.
 //   * normalizeHistogram: random_tree_test.cpp:46-154 known answers.
-//   * class-uniform sampling, level loop, prediction: invariants only
-//     (parity unpinned beyond those invariants).
+//   * class-uniform sampling, level loop, prediction: invariants and the
+//     reference's own end-to-end behaviour only (parity unpinned beyond them).
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).
 
@@ -127,24 +127,6 @@ struct Reservoir {
 // Images: RGBDImage integral in double with one running Kahan term per
 // channel (image.cpp:236-277), stored as float (colour) / int (depth, valid).
 // ---------------------------------------------------------------------------
-template<class T>
-static void integrate(std::vector<double>& d, int W, int H) {
-    // image.cpp:250-277 with T the stored type; `d` holds the channel as double
-    // (for int channels the reference works on ints and converts each step).
-    double c = 0.0;
-    for (int y = 0; y < H; ++y) {
-        for (int x = 0; x < W; ++x) {
-            T above = (y > 0) ? static_cast<T>(d[(y - 1) * W + x]) : T();
-            T left = (x > 0) ? static_cast<T>(d[y * W + x - 1]) : T();
-            T aboveLeft = (x > 0 && y > 0) ? static_cast<T>(d[(y - 1) * W + x - 1]) : T();
-            double dy = (left + above - aboveLeft) - c;
-            double t = static_cast<T>(d[y * W + x]) + dy;
-            c = (t - static_cast<T>(d[y * W + x])) - dy;
-            d[y * W + x] = static_cast<T>(t);
-        }
-    }
-}
-
 struct Image {
     int W = 0, H = 0;
     std::vector<float> col;   // [3][H][W] integral
@@ -670,10 +652,11 @@ static int trainTree(const Dataset& ds, int treeId, const std::vector<Sample>& s
                 p.score = best[i].s;
                 for (int k = 0; k < 11; k++) p.feature[k] = feats[static_cast<size_t>(k) * F + best[i].f];
                 const int li = static_cast<int>(out.nodes.size());
+                const int childLevel = p.level + 1;  // read before push_back invalidates `p`
                 for (int side = 0; side < 2; side++) {
                     NodeOut c{};
                     c.id = side == 0 ? leftId : rightId;
-                    c.level = p.level + 1;
+                    c.level = childLevel;
                     c.parent = n.index;
                     c.left = c.right = -1;
                     c.isLeaf = 1;
