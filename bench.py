@@ -1,0 +1,214 @@
+"""Benchmark of the CURFIL hot path on one MI355X.
+
+Workload (the plain full-frame configuration): 800 synthetic RGB-D frames of
+640x480 (30 Voronoi sites, 4 classes + void, 8% missing depth), 3 trees to
+depth 18, 4000 class-uniform samples per image and tree (void ignored),
+2000 candidate features (offsets <= 120 px, regions <= 20 px) x 20
+thresholds, minSampleCount 50.
+
+A step is one full training of the 3-tree forest (feature generation,
+split evaluation and partition of every level) from the resident integral
+images and the sampled root pixels; `value` is feature-threshold
+evaluations per second of the batched level-wise trainer.  Also reported:
+the plain per-node path on the same first tree (depth 12), dense prediction
+of all 800 frames (batched, Mpix/s) with single-frame latency percentiles,
+the roofline of the split-evaluation kernel, and the CPU oracle timed on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+BYTES_PER_RESPONSE = 48.0      # 2 boxes x 4 corners: colour 4 B, depth 8 B (sum + count); 50/50 mix
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=800)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
+              f"this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import curfil_amd as cf
+
+    W, H, N = 640, 480, args.frames
+    cfg_kw = dict(randomSeed=4711, samplesPerImage=12000, featureCount=2000, minSampleCount=50, maxDepth=18,
+                  boxRadius=120, regionSize=20, thresholds=20, ignored_labels=(0,))
+    trees = 3
+    ctx = cf.Context(0)
+
+    t0 = time.time()
+    rgb, depth, labels = cf.synth_dataset(N, W, H, sites=30, classes=4, zero_frac=0.08, seed0=4711)
+    log(f"synthesised {N} frames {W}x{H} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    images = cf.Images(ctx, N, W, H)
+    for i in range(N):
+        images.set_rgbd_u8(i, rgb[i], depth[i])
+        images.set_labels(i, labels[i])
+    ctx.synchronize()
+    log(f"uploaded + integrated {N} frames in {time.time() - t0:.1f}s")
+
+    cfg = cf.TrainingConfiguration(**cfg_kw)
+    t0 = time.time()
+    forest = cf.RandomForestImage.sample(ctx, images, cfg, trees, num_labels=5)
+    roots = [len(forest.root_samples(t)) for t in range(trees)]
+    log(f"sampled root pixels {roots} in {time.time() - t0:.1f}s")
+
+    for w in range(args.warmup):
+        t0 = time.time()
+        forest.grow(cfg, cf.MODE_BATCHED)
+        log(f"warmup step {w}: {time.time() - t0:.2f}s")
+
+    ctx.set_profiling(True)
+    step_times, profiles = [], []
+    for k in range(args.steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        forest.grow(cfg, cf.MODE_BATCHED)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        step_times.append(dt)
+        profiles.append(forest.profile())
+        log(f"step {k}: {dt:.3f}s  {profiles[-1]}")
+    ctx.set_profiling(False)
+    prof = profiles[-1]
+    evals = prof["evaluations"]
+    total_time = sum(step_times)
+    value = evals * len(step_times) / total_time
+    nodes = [len(forest.tree(t)[0]) for t in range(trees)]
+
+    # roofline of the split-evaluation (histogram) kernel, from its HIP-event time
+    eval_s = sum(p["eval_kernel_ms"] for p in profiles) / 1e3
+    responses = sum(p["responses"] for p in profiles)
+    achieved = responses * BYTES_PER_RESPONSE / eval_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_eval_hist", "launches": int(sum(p["eval_launches"] for p in profiles)),
+                "avg_launch_ms": round(sum(p["eval_kernel_ms"] for p in profiles) /
+                                       max(1, sum(p["eval_launches"] for p in profiles)), 4),
+                "bytes_per_response": BYTES_PER_RESPONSE}
+
+    # plain per-node path vs batched on the same single tree (tree 0, depth 12)
+    cfg1 = cf.TrainingConfiguration(**dict(cfg_kw, samplesPerImage=4000, maxDepth=12))
+    f1 = cf.RandomForestImage.sample(ctx, images, cfg1, 1, num_labels=5)
+    t0 = time.perf_counter()
+    f1.grow(cfg1, cf.MODE_PLAIN)
+    plain_s = time.perf_counter() - t0
+    plain_evals = f1.profile()["evaluations"]
+    plain_nodes = f1.profile()["nodes"]
+    t0 = time.perf_counter()
+    f1.grow(cfg1, cf.MODE_BATCHED)
+    batched1_s = time.perf_counter() - t0
+    log(f"plain (1 tree, depth 12, {int(plain_nodes)} nodes): {plain_s:.2f}s; batched: {batched1_s:.2f}s")
+
+    # dense prediction with the trained forest
+    forest.normalizeHistograms(0.0, True)
+    batch = 64
+    forest.predict_batch(images, list(range(min(batch, N))), probabilities=False)   # warm (forest upload)
+    t0 = time.perf_counter()
+    for b0 in range(0, N, batch):
+        forest.predict_batch(images, list(range(b0, min(N, b0 + batch))), probabilities=False)
+    pred_s = time.perf_counter() - t0
+    mpix = N * W * H / pred_s / 1e6
+    lat = []
+    for i in range(min(N, 100)):
+        t1 = time.perf_counter()
+        forest.predict(images, i, probabilities=True)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat = np.array(lat)
+    log(f"predict: {mpix:.1f} Mpix/s batched ({N / pred_s:.1f} frames/s); single-frame p50 "
+        f"{np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(forest, rgb, depth, labels, cfg_kw, args.cpu_sample)
+
+    out = {
+        "metric": "feature-threshold evals/s (train)",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": len(step_times),
+        "warmup": args.warmup,
+        "ms_per_step": total_time / len(step_times) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain full-frame training: {N} frames {W}x{H}, {trees} trees depth 18, "
+                               f"4000 samples/image/tree, F=2000 x T=20, boxRadius 120, regionSize 20, "
+                               f"minSampleCount 50, 4 classes + void",
+                   "mode": "batched level-wise", "evaluations_per_step": evals, "nodes_per_tree": nodes,
+                   "root_samples_per_tree": roots[0]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "phase_ms": {k: round(prof[k], 1) for k in ("featgen_ms", "eval_kernel_ms", "score_ms", "partition_ms")},
+        "plain_vs_batched": {"workload": "tree 0 only, depth 12", "plain_evals_per_s": plain_evals / plain_s,
+                             "batched_evals_per_s": plain_evals / batched1_s,
+                             "speedup": plain_s / batched1_s, "nodes": int(plain_nodes)},
+        "predict": {"frames": N, "trees": trees, "batch": batch, "mpix_per_s": mpix, "frames_per_s": N / pred_s,
+                    "single_frame_p50_ms": float(np.percentile(lat, 50)),
+                    "single_frame_p99_ms": float(np.percentile(lat, 99)),
+                    "note": "batched = labels to host; single frame = posteriors + labels to host"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(forest, rgb, depth, labels, cfg_kw, sample):
+    """The oracle (scalar C++ port of the reference CPU path) evaluating the
+    root node of tree 0 on its first `sample` root samples: all 2000 x 20
+    feature-threshold pairs, one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+
+    roots = forest.root_samples(0)[:sample].copy()
+    used = np.unique(roots["image"])
+    remap = np.zeros(int(used.max()) + 1, np.uint32)
+    remap[used] = np.arange(len(used))
+    roots["image"] = remap[roots["image"]]
+    ds = orc.Dataset.from_u8(rgb[used], depth[used], labels[used])
+    F, T = cfg_kw["featureCount"], cfg_kw["thresholds"]
+    feats, thr = ds.generate_features(roots, 12345, F, T, cfg_kw["boxRadius"], cfg_kw["regionSize"])
+    t0 = time.perf_counter()
+    ds.node_counters(roots, feats, thr, 5)
+    dt = time.perf_counter() - t0
+    ev = len(roots) * F * T
+    log(f"cpu baseline: {ev:.3e} evals in {dt:.2f}s")
+    return {"value": ev / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"root node of tree 0: first {len(roots)} root samples x {F} features x {T} thresholds "
+                      f"({ev:.2e} evaluations), 1 thread, {cpu_model()}"}
+
+
+if __name__ == "__main__":
+    main()

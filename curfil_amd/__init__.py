@@ -72,6 +72,8 @@ def _load():
         "cf_scores": [P, P, P, I, I, I, P],
         "cf_node_best_split": [P, P, P, I, P, P, I, I, I, P, P, P, P],
         "cf_forest_train": [P, P, P, I, I, I, ctypes.POINTER(P)],
+        "cf_forest_sample": [P, P, P, I, I, ctypes.POINTER(P)], "cf_forest_grow": [P, P, P, P, I],
+        "cf_ctx_set_profiling": [P, I], "cf_forest_profile": [P, P, I],
         "cf_tree_train_from_samples": [P, P, P, I, I, I, P, I, I, ctypes.POINTER(P)],
         "cf_forest_create": [P, I, P, P, P, I, P, ctypes.POINTER(P)],
         "cf_forest_destroy": [P], "cf_forest_num_trees": [P], "cf_forest_num_classes": [P],
@@ -120,6 +122,9 @@ class Context:
 
     def synchronize(self):
         self.check(lib.cf_ctx_synchronize(self.h))
+
+    def set_profiling(self, enable=True):
+        self.check(lib.cf_ctx_set_profiling(self.h, int(enable)))
 
     def close(self):
         if self.h:
@@ -302,6 +307,29 @@ class RandomForestImage:
         ctx.check(lib.cf_forest_train(ctx.h, images.h, ctypes.byref(c), tree_count, num_labels, mode,
                                       ctypes.byref(h)))
         return cls(ctx, h)
+
+    @classmethod
+    def sample(cls, ctx, images, configuration, tree_count, num_labels):
+        """Pixel sampling of every tree; grow() then trains from these roots."""
+        h = ctypes.c_void_p()
+        c = configuration.to_c()
+        ctx.check(lib.cf_forest_sample(ctx.h, images.h, ctypes.byref(c), tree_count, num_labels, ctypes.byref(h)))
+        f = cls(ctx, h)
+        f._images = images
+        return f
+
+    def grow(self, configuration, mode=MODE_BATCHED, images=None):
+        c = configuration.to_c()
+        im = images if images is not None else self._images
+        self.ctx.check(lib.cf_forest_grow(self.ctx.h, im.h, self.h, ctypes.byref(c), mode))
+
+    PROFILE_KEYS = ("evaluations", "seconds", "eval_kernel_ms", "eval_launches", "responses", "featgen_ms",
+                    "score_ms", "partition_ms", "levels", "replays", "nodes")
+
+    def profile(self):
+        out = np.zeros(len(self.PROFILE_KEYS), np.float64)
+        lib.cf_forest_profile(self.h, _ptr(out), len(out))
+        return dict(zip(self.PROFILE_KEYS, out.tolist()))
 
     @classmethod
     def train_tree_from_samples(cls, ctx, images, configuration, tree_id, num_classes, rs_seed, roots,

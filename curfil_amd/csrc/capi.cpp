@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "cf_host.h"
 
@@ -394,7 +395,7 @@ int cf_scores(cf_ctx* ctx, const uint32_t* counters, const uint32_t* histogram, 
     });
 }
 
-int cf_forest_train(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int numTrees, int numLabels, int mode,
+int cf_forest_sample(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int numTrees, int numLabels,
         cf_forest** out) {
     return guarded(ctx, [&] {
         require(im && cfg && out, "null argument");
@@ -408,19 +409,78 @@ int cf_forest_train(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int 
         fo->prior = labelPrior(im, numLabels);
         fo->C = static_cast<int>(fo->prior.size());
         // RandomForestImage::train: SEED from the configured seed, tree t uses
-        // RandomSource(SEED + t) (random_forest_image.cpp:150-173).
+        // RandomSource(SEED + t) (random_forest_image.cpp:150-173).  Trees
+        // sample independently, so they sample concurrently.
         HostRandomSource master(cfg->random_seed);
         const int SEED = master.uniformSampler(0xFFFF).next();
-        std::vector<TreeJob> jobs;
+        fo->roots.resize(numTrees);
+        fo->rsSeeds.resize(numTrees);
+        fo->treeIds.resize(numTrees);
+        std::vector<std::string> errors(numTrees);
+        std::vector<std::thread> threads;
         for (int t = 0; t < numTrees; t++) {
-            TreeJob j{t, HostRandomSource(SEED + t), {}};
-            j.roots = sampleClassUniform(im, fo->prior, *cfg, j.rs, cfg->samples_per_image / numTrees);
-            fo->roots.push_back(j.roots);
-            jobs.push_back(std::move(j));
+            threads.emplace_back([&, t] {
+                try {
+                    HostRandomSource rs(SEED + t);
+                    fo->roots[t] = sampleClassUniform(im, fo->prior, *cfg, rs, cfg->samples_per_image / numTrees);
+                    fo->rsSeeds[t] = rs.seed();
+                    fo->treeIds[t] = t;
+                } catch (const std::exception& e) {
+                    errors[t] = e.what();
+                }
+            });
         }
-        trainTrees(ctx, im, *cfg, fo->C, jobs, mode, fo->trees, fo->stats);
+        for (auto& th : threads) th.join();
+        for (const auto& e : errors) {
+            if (!e.empty()) throw Error(CF_ERR_RUNTIME, e);
+        }
         *out = fo.release();
     });
+}
+
+int cf_forest_grow(cf_ctx* ctx, cf_images* im, cf_forest* fo, const cf_train_config* cfg, int mode) {
+    return guarded(ctx, [&] {
+        require(im && fo && cfg, "null argument");
+        require(!fo->roots.empty(), "forest has no sampled roots");
+        std::vector<TreeJob> jobs;
+        for (size_t t = 0; t < fo->roots.size(); t++) {
+            jobs.push_back(TreeJob{fo->treeIds[t], HostRandomSource(fo->rsSeeds[t]), fo->roots[t]});
+        }
+        fo->stats = TrainStats();
+        fo->trees.clear();
+        fo->post.clear();
+        fo->uploaded = false;
+        trainTrees(ctx, im, *cfg, fo->C, jobs, mode, fo->trees, fo->stats);
+    });
+}
+
+int cf_forest_train(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int numTrees, int numLabels, int mode,
+        cf_forest** out) {
+    if (!out) return CF_ERR_ARG;
+    cf_forest* fo = nullptr;
+    int st = cf_forest_sample(ctx, im, cfg, numTrees, numLabels, &fo);
+    if (st != CF_OK) return st;
+    st = cf_forest_grow(ctx, im, fo, cfg, mode);
+    if (st != CF_OK) {
+        cf_forest_destroy(fo);
+        return st;
+    }
+    *out = fo;
+    return CF_OK;
+}
+
+int cf_ctx_set_profiling(cf_ctx* ctx, int enable) {
+    return guarded(ctx, [&] { ctx->profiling = enable != 0; });
+}
+
+int cf_forest_profile(const cf_forest* fo, double* out, int n) {
+    if (!fo || !out) return CF_ERR_ARG;
+    const TrainStats& st = fo->stats;
+    const double v[] = {st.evaluations, st.seconds, st.evalKernelMs, st.evalLaunches, st.responses, st.featGenMs,
+            st.scoreMs, st.partitionMs, static_cast<double>(st.levels), static_cast<double>(st.replays), st.nodes};
+    const int k = static_cast<int>(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < k; i++) out[i] = v[i];
+    return k;
 }
 
 int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int treeId, int C, int rsSeed,
@@ -436,6 +496,8 @@ int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* im, const cf_train_config
         std::vector<TreeJob> jobs;
         TreeJob j{treeId, HostRandomSource(rsSeed), std::vector<cf_sample>(roots, roots + n)};
         fo->roots.push_back(j.roots);
+        fo->treeIds.push_back(treeId);
+        fo->rsSeeds.push_back(rsSeed);
         jobs.push_back(std::move(j));
         trainTrees(ctx, im, *cfg, C, jobs, mode, fo->trees, fo->stats);
         *out = fo.release();

@@ -19,6 +19,11 @@ struct TrainStats {
     double seconds = 0;      // wall time of the level loop (device work included)
     int levels = 0;
     int replays = 0;         // nodes resolved by the sequential argmax replay
+    double nodes = 0;        // evaluated nodes
+    double responses = 0;    // (sample, feature) responses binned by the histogram kernel
+    // device time per phase (ms), filled when the context profiles
+    double evalKernelMs = 0, evalLaunches = 0;
+    double featGenMs = 0, scoreMs = 0, partitionMs = 0;
 };
 
 // One tree to grow: its id, random source state and root samples.
@@ -49,6 +54,8 @@ struct cf_forest {
     int C = 0;
     std::vector<cf::TreeBuild> trees;
     std::vector<std::vector<cf_sample>> roots;
+    std::vector<int> treeIds;
+    std::vector<int> rsSeeds;  // RandomSource seed counter of each tree after pixel sampling
     std::vector<uint32_t> prior;
     std::vector<std::vector<double>> post;  // normalized histograms [nodes][C] per tree
     bool uploaded = false;

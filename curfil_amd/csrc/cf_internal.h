@@ -82,6 +82,7 @@ struct cf_ctx {
     hipStream_t stream = nullptr;
     std::string error;
     std::recursive_mutex mu;
+    bool profiling = false;
     // scratch buffers reused across calls
     cf::DevBuf<uint8_t> scratch0, scratch1, scratch2;
 };

@@ -41,6 +41,40 @@ struct LevelNode {
     int chunkBegin, chunkEnd;
 };
 
+// Device time of the work enqueued between begin() and end() on the stream.
+class PhaseTimer {
+public:
+    PhaseTimer(hipStream_t s, bool on) : s_(s), on_(on) {
+        if (on_) {
+            CF_HIP(hipEventCreate(&a_));
+            CF_HIP(hipEventCreate(&b_));
+        }
+    }
+    ~PhaseTimer() {
+        if (on_) {
+            (void)hipEventDestroy(a_);
+            (void)hipEventDestroy(b_);
+        }
+    }
+    void begin() {
+        if (on_) CF_HIP(hipEventRecord(a_, s_));
+    }
+    // adds the elapsed milliseconds to `acc` (synchronises on the end event)
+    void end(double& acc) {
+        if (!on_) return;
+        CF_HIP(hipEventRecord(b_, s_));
+        CF_HIP(hipEventSynchronize(b_));
+        float ms = 0;
+        CF_HIP(hipEventElapsedTime(&ms, a_, b_));
+        acc += ms;
+    }
+
+private:
+    hipStream_t s_;
+    bool on_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
 size_t evalBudgetBytes() {
     const char* e = std::getenv("CURFIL_EVAL_BUDGET_MB");
     size_t mb = e ? static_cast<size_t>(std::atoll(e)) : 2048;
@@ -131,6 +165,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<int> dChunkLeft;
     DevBuf<uint32_t> dLeftHist;
     DevBuf<int2> dDst;
+    PhaseTimer timer(s, ctx->profiling);
     const size_t perSlot = static_cast<size_t>(F) * (T + 1) * C;
     const size_t budget = evalBudgetBytes();
     size_t maxSlots = std::max<size_t>(1, budget / (perSlot * sizeof(uint32_t)));
@@ -152,8 +187,10 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         }
         dJobs.resize(numActive);
         CF_HIP(hipMemcpyAsync(dJobs.get(), fj.data(), numActive * sizeof(FeatGenJob), hipMemcpyHostToDevice, s));
+        timer.begin();
         launchFeatureGeneration(s, images->desc.get(), dJobs.get(), numActive, F, T, cfg.box_radius, cfg.region_size,
                 cfg.use_depth_images, seeds.data(), tabs);
+        timer.end(stats.featGenMs);
 
         // ---- nodes and chunks of the level (slot = position in `nodes`)
         std::vector<LevelNode> nodes;
@@ -180,6 +217,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 off += ln.count;
                 nodes.push_back(ln);
                 stats.evaluations += static_cast<double>(ln.count) * F * T;
+                stats.responses += static_cast<double>(ln.count) * F;
+                stats.nodes += 1;
             }
         }
         const int numNodes = static_cast<int>(nodes.size());
@@ -209,10 +248,15 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             CF_HIP(hipMemcpyAsync(dNodeHist.get(), nh.data(), nh.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
             CF_HIP(hipMemcpyAsync(dSlotTree.get(), st.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
             CF_HIP(hipMemsetAsync(dG.get(), 0, static_cast<size_t>(ns) * perSlot * sizeof(uint32_t), s));
+            timer.begin();
             launchEvalHistograms(s, images->desc.get(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()),
                     tabs, C, dG.get());
+            timer.end(stats.evalKernelMs);
+            stats.evalLaunches += 1;
+            timer.begin();
             launchScores(s, dG.get(), dNodeHist.get(), dSlotTree.get(), ns, tabs, C, dSumm.get(), nullptr, -1);
             launchReduce(s, dSumm.get(), ns, F, dBest.get());
+            timer.end(stats.scoreMs);
             CF_HIP(hipMemcpyAsync(&best[s0], dBest.get(), ns * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
             for (int i = 0; i < ns; i++) {
@@ -257,8 +301,10 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         CF_HIP(hipMemcpyAsync(dSplits.get(), splits.data(), numNodes * sizeof(SplitRec), hipMemcpyHostToDevice, s));
         CF_HIP(hipMemcpyAsync(dChunks.get(), chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
         CF_HIP(hipMemsetAsync(dLeftHist.get(), 0, static_cast<size_t>(numNodes) * C * sizeof(uint32_t), s));
+        timer.begin();
         launchPartitionDirections(s, images->desc.get(), level[cur].get(), dChunks.get(), static_cast<int>(chunks.size()),
                 dSplits.get(), C, dir.get(), dChunkLeft.get(), dLeftHist.get());
+        timer.end(stats.partitionMs);
         std::vector<int> chunkLeft(chunks.size());
         std::vector<uint32_t> leftHist(static_cast<size_t>(numNodes) * C);
         CF_HIP(hipMemcpyAsync(chunkLeft.data(), dChunkLeft.get(), chunks.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -340,8 +386,10 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         if (nextOff > 0) {
             dDst.resize(dst.size());
             CF_HIP(hipMemcpyAsync(dDst.get(), dst.data(), dst.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+            timer.begin();
             launchPartitionScatter(s, level[cur].get(), dChunks.get(), static_cast<int>(chunks.size()), dir.get(),
                     dDst.get(), level[1 - cur].get());
+            timer.end(stats.partitionMs);
             cur = 1 - cur;
         }
         CF_HIP(hipStreamSynchronize(s));

@@ -153,6 +153,21 @@ int cf_node_best_split(cf_ctx* ctx, cf_images* images, const cf_sample* samples,
  * the host, then level-wise training on the device in the given mode. */
 int cf_forest_train(cf_ctx* ctx, cf_images* images, const cf_train_config* config, int num_trees,
         int num_labels, int mode, cf_forest** out);
+/* the two halves of cf_forest_train: pixel sampling of every tree (host,
+ * one thread per tree; RandomTreeImage::train up to doTrain,
+ * random_tree_image.cpp:746-812), then growing all trees from the stored
+ * roots on the device.  grow may be called repeatedly; every call restarts
+ * each tree's RandomSource where sampling left it, so results repeat. */
+int cf_forest_sample(cf_ctx* ctx, cf_images* images, const cf_train_config* config, int num_trees,
+        int num_labels, cf_forest** out);
+int cf_forest_grow(cf_ctx* ctx, cf_images* images, cf_forest* forest, const cf_train_config* config, int mode);
+/* per-phase device timing of training (hipEvents; the reference's utils::Profile,
+ * utils.h:159-203, and per-node timers, random_tree.h:634-679) */
+int cf_ctx_set_profiling(cf_ctx* ctx, int enable);
+/* out[0..10]: evaluations, wall seconds, histogram-kernel ms, histogram launches,
+ * responses, feature-generation ms, score+argmax ms, partition ms, levels,
+ * sequential-replay nodes, evaluated nodes.  Returns the number of entries. */
+int cf_forest_profile(const cf_forest* forest, double* out, int n);
 /* one tree from given root samples (RandomTreeImage::doTrain,
  * random_tree_image.cpp:714-730); rs_seed seeds the tree's RandomSource. */
 int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* images, const cf_train_config* config, int tree_id,
