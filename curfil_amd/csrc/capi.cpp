@@ -70,7 +70,7 @@ void singleNodeBins(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n,
     const size_t perSlot = static_cast<size_t>(tabs.F) * (tabs.T + 1) * C;
     dG.resize(perSlot);
     CF_HIP(hipMemsetAsync(dG.get(), 0, perSlot * sizeof(uint32_t), ctx->stream));
-    launchEvalHistograms(ctx->stream, im->desc.get(), dS.get(), dC.get(), static_cast<int>(chunks.size()), tabs, C,
+    launchEvalHistograms(ctx->stream, im->set(), dS.get(), dC.get(), static_cast<int>(chunks.size()), tabs, C,
             dG.get());
 }
 
@@ -114,6 +114,8 @@ void uploadForest(cf_forest* fo) {
 }
 
 }  // namespace
+
+cf::ImageSet cf_images::set() const { return cf::ImageSet{col.get(), dep.get(), W, H}; }
 
 extern "C" {
 

@@ -44,40 +44,50 @@ __device__ __forceinline__ float depthToFloat(int32_t mm) {
     return static_cast<float>(mm) / 1000.0f;
 }
 
-// XY::normalize (random_tree_image.h:82-87): int -> float division -> truncation.
-__device__ __forceinline__ int normalizeCoord(int v, float d) {
-    return static_cast<int>(static_cast<float>(v) / d);
+// XY::normalize (random_tree_image.h:82-87) is (int)((float)v / d).  For the
+// operands that occur here (|v| <= 127, d = mm / 1000.0f with mm <= 50000)
+// that equals (int)(float)(v * rinv) with rinv = 1.0 / (double)d: the product
+// is within 2^-52 (relative) of v/d, while a quotient v/d of a 7-bit integer by
+// a 24-bit float lies at least 2^-49 (relative) away from every float rounding
+// midpoint and is never one itself, so both round to the same float.
+// tests/test_numerics.py checks all 12.75M (v, mm) pairs.
+__host__ __device__ __forceinline__ double depthReciprocal(float d) {
+    return 1.0 / static_cast<double>(d);
+}
+__device__ __forceinline__ int normalizeCoord(int v, double rinv) {
+    return static_cast<int>(static_cast<float>(static_cast<double>(v) * rinv));
 }
 
-// averageRegionColor (random_tree_image.h:222-262): box (lx,rx] x (uy,ly].
-__device__ __forceinline__ double boxColor(const DevImage& im, int ch, int px, int py, int ox, int oy, int rx,
-        int ry) {
+// averageRegionColor (random_tree_image.h:222-262): box (lx,rx] x (uy,ly] of
+// one colour plane c (W x H).
+__device__ __forceinline__ double boxColor(const float* __restrict__ c, int W, int H, int px, int py, int ox, int oy,
+        int rx, int ry) {
     const int w = rx > 1 ? rx : 1;
     const int h = ry > 1 ? ry : 1;
     const int x = px + ox, y = py + oy;
     const int lx = x - w, hx = x + w, uy = y - h, ly = y + h;
-    if (lx < 0 || hx >= im.W || uy < 0 || ly >= im.H) return cf_nan();
-    const float* c = im.col + static_cast<size_t>(ch) * im.W * im.H;
-    const double lowerRight = c[ly * im.W + hx];
-    const double lowerLeft = c[ly * im.W + lx];
-    const double upperRight = c[uy * im.W + hx];
-    const double upperLeft = c[uy * im.W + lx];
+    if (lx < 0 || hx >= W || uy < 0 || ly >= H) return cf_nan();
+    const double lowerRight = c[ly * W + hx];
+    const double lowerLeft = c[ly * W + lx];
+    const double upperRight = c[uy * W + hx];
+    const double upperLeft = c[uy * W + lx];
     return (lowerRight - upperRight) + (upperLeft - lowerLeft);
 }
 
 // averageRegionDepth (random_tree_image.h:268-316).  Sums use wrapping
 // 32-bit arithmetic: the box sum is exact whenever it fits in int32, even if
 // single integral entries wrapped (they are stored modulo 2^32).
-__device__ __forceinline__ double boxDepth(const DevImage& im, int px, int py, int ox, int oy, int rx, int ry) {
+__device__ __forceinline__ double boxDepth(const int2* __restrict__ dep, int W, int H, int px, int py, int ox, int oy,
+        int rx, int ry) {
     const int w = rx > 1 ? rx : 1;
     const int h = ry > 1 ? ry : 1;
     const int x = px + ox, y = py + oy;
     const int lx = x - w, hx = x + w, uy = y - h, ly = y + h;
-    if (lx < 0 || hx >= im.W || uy < 0 || ly >= im.H) return cf_nan();
-    const int2 lr = im.dep[ly * im.W + hx];
-    const int2 ll = im.dep[ly * im.W + lx];
-    const int2 ur = im.dep[uy * im.W + hx];
-    const int2 ul = im.dep[uy * im.W + lx];
+    if (lx < 0 || hx >= W || uy < 0 || ly >= H) return cf_nan();
+    const int2 lr = dep[ly * W + hx];
+    const int2 ll = dep[ly * W + lx];
+    const int2 ur = dep[uy * W + hx];
+    const int2 ul = dep[uy * W + lx];
     const int numValid = static_cast<int>((static_cast<uint32_t>(lr.y) - static_cast<uint32_t>(ur.y)) +
             (static_cast<uint32_t>(ul.y) - static_cast<uint32_t>(ll.y)));
     if (numValid == 0) return cf_nan();
@@ -87,30 +97,37 @@ __device__ __forceinline__ double boxDepth(const DevImage& im, int px, int py, i
     return feat / static_cast<double>(numValid);
 }
 
-// calculateColorFeature / calculateDepthFeature (random_tree_image.h:564-623).
-__device__ __forceinline__ double featureResponse(const DevImage& im, const int8_t* p, int px, int py,
-        int32_t depthMm, bool flip) {
-    if (!(depthMm > 0)) return cf_nan();
-    const float d = depthToFloat(depthMm);
+// calculateColorFeature / calculateDepthFeature (random_tree_image.h:564-623)
+// for a pixel with valid depth; rinv = depthReciprocal(Depth::getFloatValue()).
+__device__ __forceinline__ double featureResponseR(const float* __restrict__ col, const int2* __restrict__ dep, int W,
+        int H, const int8_t* p, int px, int py, double rinv, bool flip) {
     const int o1x = flip ? -p[P_O1X] : p[P_O1X];
     const int o2x = flip ? -p[P_O2X] : p[P_O2X];
     double a, b;
     if (p[P_TYPE] == T_DEPTH) {
-        a = boxDepth(im, px, py, normalizeCoord(o1x, d), normalizeCoord(p[P_O1Y], d),
-                normalizeCoord(p[P_R1X], d), normalizeCoord(p[P_R1Y], d));
+        a = boxDepth(dep, W, H, px, py, normalizeCoord(o1x, rinv), normalizeCoord(p[P_O1Y], rinv),
+                normalizeCoord(p[P_R1X], rinv), normalizeCoord(p[P_R1Y], rinv));
         if (__builtin_isnan(a)) return a;
-        b = boxDepth(im, px, py, normalizeCoord(o2x, d), normalizeCoord(p[P_O2Y], d),
-                normalizeCoord(p[P_R2X], d), normalizeCoord(p[P_R2Y], d));
+        b = boxDepth(dep, W, H, px, py, normalizeCoord(o2x, rinv), normalizeCoord(p[P_O2Y], rinv),
+                normalizeCoord(p[P_R2X], rinv), normalizeCoord(p[P_R2Y], rinv));
         if (__builtin_isnan(b)) return b;
     } else {
-        a = boxColor(im, static_cast<uint8_t>(p[P_C1]), px, py, normalizeCoord(o1x, d), normalizeCoord(p[P_O1Y], d),
-                normalizeCoord(p[P_R1X], d), normalizeCoord(p[P_R1Y], d));
+        const size_t plane = static_cast<size_t>(W) * H;
+        a = boxColor(col + static_cast<uint8_t>(p[P_C1]) * plane, W, H, px, py, normalizeCoord(o1x, rinv),
+                normalizeCoord(p[P_O1Y], rinv), normalizeCoord(p[P_R1X], rinv), normalizeCoord(p[P_R1Y], rinv));
         if (__builtin_isnan(a)) return a;
-        b = boxColor(im, static_cast<uint8_t>(p[P_C2]), px, py, normalizeCoord(o2x, d), normalizeCoord(p[P_O2Y], d),
-                normalizeCoord(p[P_R2X], d), normalizeCoord(p[P_R2Y], d));
+        b = boxColor(col + static_cast<uint8_t>(p[P_C2]) * plane, W, H, px, py, normalizeCoord(o2x, rinv),
+                normalizeCoord(p[P_O2Y], rinv), normalizeCoord(p[P_R2X], rinv), normalizeCoord(p[P_R2Y], rinv));
         if (__builtin_isnan(b)) return b;
     }
     return a - b;
+}
+
+// The same from the pixel's depth in mm, NaN if the depth is invalid.
+__device__ __forceinline__ double featureResponse(const DevImage& im, const int8_t* p, int px, int py,
+        int32_t depthMm, bool flip) {
+    if (!(depthMm > 0)) return cf_nan();
+    return featureResponseR(im.col, im.dep, im.W, im.H, p, px, py, depthReciprocal(depthToFloat(depthMm)), flip);
 }
 
 // Smallest float >= r (NaN stays NaN).  For a float threshold t:

@@ -75,6 +75,13 @@ private:
     size_t n_ = 0, cap_ = 0;
 };
 
+// Uniform-size image set: image i's planes start at col + i*3*W*H and dep + i*W*H.
+struct ImageSet {
+    const float* col;
+    const int2* dep;
+    int W, H;
+};
+
 }  // namespace cf
 
 struct cf_ctx {
@@ -97,6 +104,7 @@ struct cf_images {
     std::vector<uint8_t> labelsSet;
     std::vector<int32_t> depthHost; // host raw depth mm [count][H][W] (pixel validity for sampling)
     std::vector<uint8_t> depthSet;
+    cf::ImageSet set() const;
 };
 
 namespace cf {
@@ -162,7 +170,7 @@ struct FeatSummary {
 };
 
 // histogram bins G[slot][F][T+1][C] (interval histograms)
-void launchEvalHistograms(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
+void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const Chunk* chunks,
         int numChunks, const FeatTables& tabs, int C, uint32_t* G);
 void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
         const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot);

@@ -31,36 +31,86 @@ namespace cf {
 
 constexpr int kEvalThreads = 256;
 constexpr int kFeatTile = 16;
+constexpr int kSamplesPerThread = 4;   // samples held in registers per pass
+constexpr int kPass = kEvalThreads * kSamplesPerThread;
 
-__global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const DevImage* __restrict__ imgs,
-        const cf_sample* __restrict__ samples, const Chunk* __restrict__ chunks, const FeatRec* __restrict__ feat,
-        const float* __restrict__ thrFT, int F, int T, int C, uint32_t* __restrict__ G) {
+// One training pixel held in registers for the feature loop.
+struct RegSample {
+    double rinv;   // depthReciprocal(depth in metres)
+    int img;
+    int xy;        // x | y << 16
+    int lab;       // label | flip << 8 | (invalid depth) << 10;  < 0: empty slot
+};
+
+// Interval of q among the T thresholds (thresholds held in registers; the
+// loop bound is a compile-time constant so the compare chain unrolls).
+template<int TM>
+__device__ __forceinline__ int binOf(float q, const float (&th)[TM], int T) {
+    if (__builtin_isnan(q)) return T;
+    int b = 0;
+#pragma unroll
+    for (int t = 0; t < TM; t++) b += (t < T && th[t] < q) ? 1 : 0;
+    return b;
+}
+
+// All images of a set share W x H and are stored back to back, so a sample's
+// integrals are found from its image index.
+template<int TM>
+__global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const float* __restrict__ colBase,
+        const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
+        const Chunk* __restrict__ chunks, const FeatRec* __restrict__ feat, const float* __restrict__ thrFT, int F,
+        int T, int C, uint32_t* __restrict__ G) {
     extern __shared__ uint32_t h[];
     const Chunk ch = chunks[blockIdx.x];
     const int f0 = blockIdx.y * kFeatTile;
     const int nf = min(kFeatTile, F - f0);
     const int binsPerF = (T + 1) * C;
     const int nb = nf * binsPerF;
+    const size_t plane = static_cast<size_t>(W) * H;
     for (int k = threadIdx.x; k < nb; k += kEvalThreads) h[k] = 0;
     __syncthreads();
     const FeatRec* fp = feat + static_cast<size_t>(ch.tree) * F + f0;
     const float* tp = thrFT + (static_cast<size_t>(ch.tree) * F + f0) * T;
-    for (int i = threadIdx.x; i < ch.count; i += kEvalThreads) {
-        const cf_sample s = samples[ch.begin + i];
-        const DevImage im = imgs[s.image];
+    for (int base = 0; base < ch.count; base += kPass) {
+        RegSample rs[kSamplesPerThread];
+#pragma unroll
+        for (int k = 0; k < kSamplesPerThread; k++) {
+            const int i = base + k * kEvalThreads + threadIdx.x;
+            rs[k].lab = -1;
+            if (i < ch.count) {
+                const cf_sample s = samples[ch.begin + i];
+                const bool valid = s.depth_mm > 0;
+                rs[k].rinv = valid ? depthReciprocal(depthToFloat(s.depth_mm)) : 0.0;
+                rs[k].img = static_cast<int>(s.image);
+                rs[k].xy = s.x | (s.y << 16);
+                rs[k].lab = s.label | (s.flip << 8) | ((valid ? 0 : 1) << 10);
+            }
+        }
         for (int fi = 0; fi < nf; fi++) {
             const FeatRec fr = fp[fi];
-            const float* th = tp + static_cast<size_t>(fi) * T;
-            for (int pass = 0; pass < (s.flip == 2 ? 2 : 1); pass++) {
-                const bool flip = (s.flip == 1) || (pass == 1);
-                const double r = featureResponse(im, fr.p, s.x, s.y, s.depth_mm, flip);
-                const float q = roundUpToFloat(r);
-                int b = T;
-                if (!__builtin_isnan(q)) {
-                    b = 0;
-                    for (int t = 0; t < T; t++) b += (th[t] < q) ? 1 : 0;
+            float th[TM];
+            const float* tf = tp + static_cast<size_t>(fi) * T;
+#pragma unroll
+            for (int t = 0; t < TM; t++) th[t] = (t < T) ? tf[t] : 0.0f;
+            uint32_t* hf = h + fi * binsPerF;
+#pragma unroll
+            for (int k = 0; k < kSamplesPerThread; k++) {
+                if (rs[k].lab < 0) continue;
+                const int label = rs[k].lab & 0xff;
+                const int flipSetting = (rs[k].lab >> 8) & 3;
+                const bool valid = (rs[k].lab >> 10) == 0;
+                const float* col = colBase + static_cast<size_t>(rs[k].img) * 3 * plane;
+                const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
+                const int px = rs[k].xy & 0xffff, py = rs[k].xy >> 16;
+                for (int pass = 0; pass < (flipSetting == 2 ? 2 : 1); pass++) {
+                    const bool flip = (flipSetting == 1) || (pass == 1);
+                    int b = T;
+                    if (valid) {
+                        const double r = featureResponseR(col, dep, W, H, fr.p, px, py, rs[k].rinv, flip);
+                        b = binOf<TM>(roundUpToFloat(r), th, T);
+                    }
+                    atomicAdd(&hf[b * C + label], 1u);
                 }
-                atomicAdd(&h[fi * binsPerF + b * C + s.label], 1u);
             }
         }
     }
@@ -326,14 +376,32 @@ __global__ void __launch_bounds__(256) k_responses(const DevImage* __restrict__ 
     out[i] = featureResponse(imgs[s.image], feats[f].p, s.x, s.y, s.depth_mm, s.flip == 1);
 }
 
-void launchEvalHistograms(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
+void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const Chunk* chunks,
         int numChunks, const FeatTables& tabs, int C, uint32_t* G) {
     if (numChunks == 0) return;
     const int tiles = (tabs.F + kFeatTile - 1) / kFeatTile;
     const size_t lds = static_cast<size_t>(kFeatTile) * (tabs.T + 1) * C * sizeof(uint32_t);
     if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
-    hipLaunchKernelGGL(k_eval_hist, dim3(numChunks, tiles), dim3(kEvalThreads), lds, s, imgs, samples, chunks,
-            tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    const dim3 grid(numChunks, tiles), block(kEvalThreads);
+    if (tabs.T <= 8) {
+        hipLaunchKernelGGL(k_eval_hist<8>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
+                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    } else if (tabs.T <= 16) {
+        hipLaunchKernelGGL(k_eval_hist<16>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
+                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    } else if (tabs.T <= 24) {
+        hipLaunchKernelGGL(k_eval_hist<24>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
+                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    } else if (tabs.T <= 32) {
+        hipLaunchKernelGGL(k_eval_hist<32>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
+                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    } else if (tabs.T <= 64) {
+        hipLaunchKernelGGL(k_eval_hist<64>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
+                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    } else {
+        hipLaunchKernelGGL(k_eval_hist<kMaxThresholds>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
+                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+    }
 }
 
 void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,

@@ -249,7 +249,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             CF_HIP(hipMemcpyAsync(dSlotTree.get(), st.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
             CF_HIP(hipMemsetAsync(dG.get(), 0, static_cast<size_t>(ns) * perSlot * sizeof(uint32_t), s));
             timer.begin();
-            launchEvalHistograms(s, images->desc.get(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()),
+            launchEvalHistograms(s, images->set(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()),
                     tabs, C, dG.get());
             timer.end(stats.evalKernelMs);
             stats.evalLaunches += 1;
