@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcurfil_hip.so")
+LIB_PATH = os.environ.get("CURFIL_LIB") or os.path.join(_HERE, "libcurfil_hip.so")
 
 CF_OK = 0
 CF_ERR_ARG = -1

@@ -6,6 +6,7 @@ are bit-exact restatements of IEEE operation sequences).
 """
 import concurrent.futures
 import os
+import re
 import subprocess
 import sys
 
@@ -45,6 +46,23 @@ DEVICE = [
 ]
 
 
+# A scalar-memory load whose immediate offset is not a multiple of 4 means the
+# compiler built a byte-misaligned scalar base address; the scalar unit drops
+# the low address bits, so such a load reads the wrong dwords.  Reject it.
+_MISALIGNED_SMEM = re.compile(r"^\s*s_(?:load|buffer_load)_dword\w*\s+\S+,\s*s\[[0-9:]+\],\s*(-?0x[0-9a-f]+|-?[0-9]+)\s*$",
+                              re.M)
+
+
+def _check_device_asm(path):
+    cmd = [HIPCC] + COMMON + DEVICE + ["--offload-device-only", "-S", "-x", "hip", path, "-o", "-"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"device assembly failed: {' '.join(cmd)}\n{r.stderr}")
+    bad = [m.group(0).strip() for m in _MISALIGNED_SMEM.finditer(r.stdout) if int(m.group(1), 0) % 4 != 0]
+    if bad:
+        raise RuntimeError(f"{os.path.basename(path)}: misaligned scalar loads generated: {bad[:4]}")
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
@@ -52,6 +70,8 @@ def _compile(src):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if src.endswith(".hip"):
+        _check_device_asm(path)
     return obj, r.stderr
 
 

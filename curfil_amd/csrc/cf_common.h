@@ -28,6 +28,35 @@ struct __align__(16) FeatRec {
     int8_t p[16];
 };
 
+// A feature's parameters unpacked to ints.  Kernels read a FeatRec as one
+// aligned uint4 and unpack it with shifts: byte-indexed access to the record
+// let the compiler form a misaligned scalar base address (record + 5) with
+// signed immediate offsets, which the scalar unit does not add up as intended.
+struct Feat {
+    int type, o1x, o1y, o2x, o2y, r1x, r1y, r2x, r2y, c1, c2;
+};
+
+__host__ __device__ __forceinline__ int sbyte(uint32_t w, int k) {
+    return static_cast<int>(static_cast<int8_t>((w >> (8 * k)) & 0xffu));
+}
+
+__device__ __forceinline__ Feat loadFeat(const FeatRec* rec) {
+    const uint4 w = *reinterpret_cast<const uint4*>(rec);
+    Feat f;
+    f.type = sbyte(w.x, 0);
+    f.o1x = sbyte(w.x, 1);
+    f.o1y = sbyte(w.x, 2);
+    f.o2x = sbyte(w.x, 3);
+    f.o2y = sbyte(w.y, 0);
+    f.r1x = sbyte(w.y, 1);
+    f.r1y = sbyte(w.y, 2);
+    f.r2x = sbyte(w.y, 3);
+    f.r2y = sbyte(w.z, 0);
+    f.c1 = static_cast<int>((w.z >> 8) & 0xffu);
+    f.c2 = static_cast<int>((w.z >> 16) & 0xffu);
+    return f;
+}
+
 // Device view of one integral image.  col: [3][H][W] float; dep: [H][W]
 // int2 {sum of depth mm (mod 2^32), count of valid pixels}.
 struct DevImage {
@@ -100,34 +129,34 @@ __device__ __forceinline__ double boxDepth(const int2* __restrict__ dep, int W, 
 // calculateColorFeature / calculateDepthFeature (random_tree_image.h:564-623)
 // for a pixel with valid depth; rinv = depthReciprocal(Depth::getFloatValue()).
 __device__ __forceinline__ double featureResponseR(const float* __restrict__ col, const int2* __restrict__ dep, int W,
-        int H, const int8_t* p, int px, int py, double rinv, bool flip) {
-    const int o1x = flip ? -p[P_O1X] : p[P_O1X];
-    const int o2x = flip ? -p[P_O2X] : p[P_O2X];
+        int H, const Feat& f, int px, int py, double rinv, bool flip) {
+    const int o1x = flip ? -f.o1x : f.o1x;
+    const int o2x = flip ? -f.o2x : f.o2x;
     double a, b;
-    if (p[P_TYPE] == T_DEPTH) {
-        a = boxDepth(dep, W, H, px, py, normalizeCoord(o1x, rinv), normalizeCoord(p[P_O1Y], rinv),
-                normalizeCoord(p[P_R1X], rinv), normalizeCoord(p[P_R1Y], rinv));
+    if (f.type == T_DEPTH) {
+        a = boxDepth(dep, W, H, px, py, normalizeCoord(o1x, rinv), normalizeCoord(f.o1y, rinv),
+                normalizeCoord(f.r1x, rinv), normalizeCoord(f.r1y, rinv));
         if (__builtin_isnan(a)) return a;
-        b = boxDepth(dep, W, H, px, py, normalizeCoord(o2x, rinv), normalizeCoord(p[P_O2Y], rinv),
-                normalizeCoord(p[P_R2X], rinv), normalizeCoord(p[P_R2Y], rinv));
+        b = boxDepth(dep, W, H, px, py, normalizeCoord(o2x, rinv), normalizeCoord(f.o2y, rinv),
+                normalizeCoord(f.r2x, rinv), normalizeCoord(f.r2y, rinv));
         if (__builtin_isnan(b)) return b;
     } else {
         const size_t plane = static_cast<size_t>(W) * H;
-        a = boxColor(col + static_cast<uint8_t>(p[P_C1]) * plane, W, H, px, py, normalizeCoord(o1x, rinv),
-                normalizeCoord(p[P_O1Y], rinv), normalizeCoord(p[P_R1X], rinv), normalizeCoord(p[P_R1Y], rinv));
+        a = boxColor(col + static_cast<size_t>(f.c1) * plane, W, H, px, py, normalizeCoord(o1x, rinv),
+                normalizeCoord(f.o1y, rinv), normalizeCoord(f.r1x, rinv), normalizeCoord(f.r1y, rinv));
         if (__builtin_isnan(a)) return a;
-        b = boxColor(col + static_cast<uint8_t>(p[P_C2]) * plane, W, H, px, py, normalizeCoord(o2x, rinv),
-                normalizeCoord(p[P_O2Y], rinv), normalizeCoord(p[P_R2X], rinv), normalizeCoord(p[P_R2Y], rinv));
+        b = boxColor(col + static_cast<size_t>(f.c2) * plane, W, H, px, py, normalizeCoord(o2x, rinv),
+                normalizeCoord(f.o2y, rinv), normalizeCoord(f.r2x, rinv), normalizeCoord(f.r2y, rinv));
         if (__builtin_isnan(b)) return b;
     }
     return a - b;
 }
 
 // The same from the pixel's depth in mm, NaN if the depth is invalid.
-__device__ __forceinline__ double featureResponse(const DevImage& im, const int8_t* p, int px, int py,
+__device__ __forceinline__ double featureResponse(const DevImage& im, const Feat& f, int px, int py,
         int32_t depthMm, bool flip) {
     if (!(depthMm > 0)) return cf_nan();
-    return featureResponseR(im.col, im.dep, im.W, im.H, p, px, py, depthReciprocal(depthToFloat(depthMm)), flip);
+    return featureResponseR(im.col, im.dep, im.W, im.H, f, px, py, depthReciprocal(depthToFloat(depthMm)), flip);
 }
 
 // Smallest float >= r (NaN stays NaN).  For a float threshold t:

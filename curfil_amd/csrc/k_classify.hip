@@ -41,9 +41,10 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
         int node = forest.treeRoot[t];
         int ch = forest.child[node];
         while (ch >= 0) {
-            const SplitRec sp = forest.nodes[node];
-            const double r = featureResponse(im, sp.feat.p, x, y, depth, false);
-            node = (r <= static_cast<double>(sp.threshold)) ? ch : ch + 1;
+            const Feat f = loadFeat(&forest.nodes[node].feat);
+            const float threshold = forest.nodes[node].threshold;
+            const double r = featureResponse(im, f, x, y, depth, false);
+            node = (r <= static_cast<double>(threshold)) ? ch : ch + 1;
             ch = forest.child[node];
         }
         const double* post = forest.post + static_cast<size_t>(-ch - 1) * C;

@@ -87,29 +87,33 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const float* __restr
             }
         }
         for (int fi = 0; fi < nf; fi++) {
-            const FeatRec fr = fp[fi];
+            const Feat fr = loadFeat(fp + fi);
             float th[TM];
             const float* tf = tp + static_cast<size_t>(fi) * T;
 #pragma unroll
             for (int t = 0; t < TM; t++) th[t] = (t < T) ? tf[t] : 0.0f;
-            uint32_t* hf = h + fi * binsPerF;
+            const int hoff = fi * binsPerF;
 #pragma unroll
             for (int k = 0; k < kSamplesPerThread; k++) {
-                if (rs[k].lab < 0) continue;
-                const int label = rs[k].lab & 0xff;
-                const int flipSetting = (rs[k].lab >> 8) & 3;
-                const bool valid = (rs[k].lab >> 10) == 0;
-                const float* col = colBase + static_cast<size_t>(rs[k].img) * 3 * plane;
-                const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
-                const int px = rs[k].xy & 0xffff, py = rs[k].xy >> 16;
-                for (int pass = 0; pass < (flipSetting == 2 ? 2 : 1); pass++) {
-                    const bool flip = (flipSetting == 1) || (pass == 1);
-                    int b = T;
+                const int lab = rs[k].lab;
+                if (lab >= 0) {
+                    const int label = lab & 0xff;
+                    const int flipSetting = (lab >> 8) & 3;
+                    const bool valid = (lab >> 10) == 0;
+                    const float* col = colBase + static_cast<size_t>(rs[k].img) * 3 * plane;
+                    const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
+                    const int px = rs[k].xy & 0xffff, py = rs[k].xy >> 16;
+                    int b1 = T, b2 = T;
                     if (valid) {
-                        const double r = featureResponseR(col, dep, W, H, fr.p, px, py, rs[k].rinv, flip);
-                        b = binOf<TM>(roundUpToFloat(r), th, T);
+                        const double r1 = featureResponseR(col, dep, W, H, fr, px, py, rs[k].rinv, flipSetting == 1);
+                        b1 = binOf<TM>(roundUpToFloat(r1), th, T);
+                        if (flipSetting == 2) {
+                            const double r2 = featureResponseR(col, dep, W, H, fr, px, py, rs[k].rinv, true);
+                            b2 = binOf<TM>(roundUpToFloat(r2), th, T);
+                        }
                     }
-                    atomicAdd(&hf[b * C + label], 1u);
+                    atomicAdd(&h[hoff + b1 * C + label], 1u);
+                    if (flipSetting == 2) atomicAdd(&h[hoff + b2 * C + label], 1u);
                 }
             }
         }
@@ -373,7 +377,7 @@ __global__ void __launch_bounds__(256) k_responses(const DevImage* __restrict__ 
     if (i >= n * F) return;
     const int f = i / n, k = i % n;
     const cf_sample s = samples[k];
-    out[i] = featureResponse(imgs[s.image], feats[f].p, s.x, s.y, s.depth_mm, s.flip == 1);
+    out[i] = featureResponse(imgs[s.image], loadFeat(feats + f), s.x, s.y, s.depth_mm, s.flip == 1);
 }
 
 void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const Chunk* chunks,

@@ -187,6 +187,7 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
             break;
         }
         const FeatRec fr = myCand[k];
+        const Feat ff = loadFeat(myCand + k);
         int pos = head;
         int t = 0, tries = 10;  // meaningful in lane 0 only
         int status = 0;
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
             {
                 const int d = sh.dq[(pos + tid) % kDrawQueue];
                 const cf_sample s = job.samples[d];
-                const double r = featureResponse(imgs[s.image], fr.p, s.x, s.y, s.depth_mm, false);
+                const double r = featureResponse(imgs[s.image], ff, s.x, s.y, s.depth_mm, false);
                 sh.resp[tid] = static_cast<float>(r);
             }
             __syncthreads();

@@ -18,13 +18,14 @@ __global__ void __launch_bounds__(256) k_part_dir(const DevImage* __restrict__ i
     __shared__ int total;
     const Chunk ch = chunks[blockIdx.x];
     const SplitRec sp = splits[ch.slot];
+    const Feat f = loadFeat(&splits[ch.slot].feat);
     if (threadIdx.x < kMaxClasses) hist[threadIdx.x] = 0;
     if (threadIdx.x == 0) total = 0;
     __syncthreads();
     int cnt = 0;
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
         const cf_sample s = samples[ch.begin + i];
-        const double r = featureResponse(imgs[s.image], sp.feat.p, s.x, s.y, s.depth_mm, s.flip == 1);
+        const double r = featureResponse(imgs[s.image], f, s.x, s.y, s.depth_mm, s.flip == 1);
         const bool left = r <= static_cast<double>(sp.threshold);
         dir[ch.begin + i] = left ? 1 : 0;
         if (left) {
