@@ -43,9 +43,25 @@ void checkSamples(const cf_images* im, const cf_sample* s, int n) {
     }
 }
 
+// Feature parameters come from the caller: the type must be depth or colour
+// and colour channels must name one of the 3 colour planes, or a response
+// would read outside the integral images.
+void checkFeature(int type, int c1, int c2) {
+    if (type != T_DEPTH && type != T_COLOR) throw Error(CF_ERR_ARG, "invalid feature type");
+    if (type == T_COLOR && (c1 < 0 || c1 > 2 || c2 < 0 || c2 > 2)) throw Error(CF_ERR_ARG, "invalid colour channel");
+}
+
+void checkFeaturesSoA(const int8_t* soa, int F) {
+    for (int f = 0; f < F; f++) {
+        checkFeature(soa[static_cast<size_t>(P_TYPE) * F + f], static_cast<uint8_t>(soa[static_cast<size_t>(P_C1) * F + f]),
+                static_cast<uint8_t>(soa[static_cast<size_t>(P_C2) * F + f]));
+    }
+}
+
 void uploadFeaturesSoA(cf_ctx* ctx, FeatTables& tabs, const int8_t* soa, const float* thrTF, int F, int T) {
     tabs.soa.resize(static_cast<size_t>(F) * kFeatParams);
     tabs.thrTF.resize(static_cast<size_t>(F) * T);
+    checkFeaturesSoA(soa, F);
     CF_HIP(hipMemcpyAsync(tabs.soa.get(), soa, static_cast<size_t>(F) * kFeatParams, hipMemcpyHostToDevice, ctx->stream));
     if (thrTF) {
         CF_HIP(hipMemcpyAsync(tabs.thrTF.get(), thrTF, static_cast<size_t>(F) * T * sizeof(float), hipMemcpyHostToDevice,
@@ -307,6 +323,7 @@ int cf_feature_responses(cf_ctx* ctx, cf_images* im, const cf_sample* samples, i
         require(im && samples && features && out, "null argument");
         require(n > 0 && F > 0, "sizes must be positive");
         checkSamples(im, samples, n);
+        checkFeaturesSoA(features, F);
         std::vector<FeatRec> fr(F);
         for (int f = 0; f < F; f++) {
             std::memset(&fr[f], 0, sizeof(FeatRec));
@@ -527,6 +544,8 @@ int cf_forest_create(cf_ctx* ctx, int numTrees, const int* numNodes, const cf_tr
                 if (!n.is_leaf) {
                     // children after their parent (guarantees traversal terminates), stored as a pair
                     require(n.left > i && n.left + 1 < numNodes[t] && n.right == n.left + 1, "invalid child indices");
+                    checkFeature(n.feature[P_TYPE], static_cast<uint8_t>(n.feature[P_C1]),
+                            static_cast<uint8_t>(n.feature[P_C2]));
                 }
             }
             fo->trees.push_back(std::move(tb));

@@ -159,6 +159,62 @@ __device__ __forceinline__ double featureResponse(const DevImage& im, const Feat
     return featureResponseR(im.col, im.dep, im.W, im.H, f, px, py, depthReciprocal(depthToFloat(depthMm)), flip);
 }
 
+// Gather-first form of the response, used where latency matters: the 8 corner
+// indices (box a: lower-right, lower-left, upper-right, upper-left, then box b)
+// are formed first so all corner loads can be in flight together.  Returns
+// false if either box leaves the image (the reference's NaN).  Computing box
+// b even when box a is already NaN changes nothing: the response is NaN then.
+__device__ __forceinline__ bool boxCorners(int px, int py, int ox, int oy, int rx, int ry, int W, int H, int* idx) {
+    const int w = rx > 1 ? rx : 1;
+    const int h = ry > 1 ? ry : 1;
+    const int x = px + ox, y = py + oy;
+    const int lx = x - w, hx = x + w, uy = y - h, ly = y + h;
+    idx[0] = ly * W + hx;
+    idx[1] = ly * W + lx;
+    idx[2] = uy * W + hx;
+    idx[3] = uy * W + lx;
+    return !(lx < 0 || hx >= W || uy < 0 || ly >= H);
+}
+
+__device__ __forceinline__ bool responseCorners(const Feat& f, int px, int py, int W, int H, double rinv, bool flip,
+        int* idx) {
+    const int o1x = flip ? -f.o1x : f.o1x;
+    const int o2x = flip ? -f.o2x : f.o2x;
+    const bool a = boxCorners(px, py, normalizeCoord(o1x, rinv), normalizeCoord(f.o1y, rinv),
+            normalizeCoord(f.r1x, rinv), normalizeCoord(f.r1y, rinv), W, H, idx);
+    const bool b = boxCorners(px, py, normalizeCoord(o2x, rinv), normalizeCoord(f.o2y, rinv),
+            normalizeCoord(f.r2x, rinv), normalizeCoord(f.r2y, rinv), W, H, idx + 4);
+    return a && b;
+}
+
+// averageRegionColor of both boxes and their difference from gathered corners.
+__device__ __forceinline__ double colorResponseFromCorners(const float* c) {
+    const double a = (static_cast<double>(c[0]) - static_cast<double>(c[2])) +
+            (static_cast<double>(c[3]) - static_cast<double>(c[1]));
+    const double b = (static_cast<double>(c[4]) - static_cast<double>(c[6])) +
+            (static_cast<double>(c[7]) - static_cast<double>(c[5]));
+    return a - b;
+}
+
+__device__ __forceinline__ double depthBoxFromCorners(const int2* d) {
+    const int numValid = static_cast<int>((static_cast<uint32_t>(d[0].y) - static_cast<uint32_t>(d[2].y)) +
+            (static_cast<uint32_t>(d[3].y) - static_cast<uint32_t>(d[1].y)));
+    if (numValid == 0) return cf_nan();
+    const int sum = static_cast<int>((static_cast<uint32_t>(d[0].x) - static_cast<uint32_t>(d[2].x)) +
+            (static_cast<uint32_t>(d[3].x) - static_cast<uint32_t>(d[1].x)));
+    return (static_cast<double>(sum) / 1000.0) / static_cast<double>(numValid);
+}
+
+// averageRegionDepth of both boxes and their difference (NaN if a box has no
+// valid pixel).
+__device__ __forceinline__ double depthResponseFromCorners(const int2* d) {
+    const double a = depthBoxFromCorners(d);
+    const double b = depthBoxFromCorners(d + 4);
+    if (__builtin_isnan(a)) return a;
+    if (__builtin_isnan(b)) return b;
+    return a - b;
+}
+
 // Smallest float >= r (NaN stays NaN).  For a float threshold t:
 //   r <= (double)t  <=>  roundUpToFloat(r) <= t
 // so the split test `r <= t` of SplitFunction::split (random_tree.h:128) can be

@@ -33,6 +33,7 @@ constexpr int kEvalThreads = 256;
 constexpr int kFeatTile = 16;
 constexpr int kSamplesPerThread = 4;   // samples held in registers per pass
 constexpr int kPass = kEvalThreads * kSamplesPerThread;
+constexpr int kXcds = 8;                  // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // One training pixel held in registers for the feature loop.
 struct RegSample {
@@ -55,14 +56,24 @@ __device__ __forceinline__ int binOf(float q, const float (&th)[TM], int T) {
 
 // All images of a set share W x H and are stored back to back, so a sample's
 // integrals are found from its image index.
+//
+// Work mapping: workgroups are dispatched round-robin over the 8 XCDs, each
+// with its own L2.  Workgroup L runs on XCD L % 8; the j-th workgroup of an
+// XCD takes feature tile j % tiles of chunk (j / tiles) * 8 + L % 8, so an XCD
+// sweeps all feature tiles of one chunk (one image region) before the next
+// and keeps that image's integrals cache-resident.
 template<int TM>
 __global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const float* __restrict__ colBase,
         const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
-        const Chunk* __restrict__ chunks, const FeatRec* __restrict__ feat, const float* __restrict__ thrFT, int F,
-        int T, int C, uint32_t* __restrict__ G) {
+        const Chunk* __restrict__ chunks, int numChunks, int tiles, const FeatRec* __restrict__ feat,
+        const float* __restrict__ thrFT, int F, int T, int C, uint32_t* __restrict__ G) {
     extern __shared__ uint32_t h[];
-    const Chunk ch = chunks[blockIdx.x];
-    const int f0 = blockIdx.y * kFeatTile;
+    const int xcd = blockIdx.x % kXcds;
+    const int j = blockIdx.x / kXcds;
+    const int chunkIdx = (j / tiles) * kXcds + xcd;
+    if (chunkIdx >= numChunks) return;
+    const Chunk ch = chunks[chunkIdx];
+    const int f0 = (j % tiles) * kFeatTile;
     const int nf = min(kFeatTile, F - f0);
     const int binsPerF = (T + 1) * C;
     const int nb = nf * binsPerF;
@@ -76,7 +87,12 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const float* __restr
 #pragma unroll
         for (int k = 0; k < kSamplesPerThread; k++) {
             const int i = base + k * kEvalThreads + threadIdx.x;
+            // lanes past the chunk's end gather (unused) corners at index 0
+            // of image 0, so every field must hold an in-range value
             rs[k].lab = -1;
+            rs[k].img = 0;
+            rs[k].xy = 0;
+            rs[k].rinv = 0.0;
             if (i < ch.count) {
                 const cf_sample s = samples[ch.begin + i];
                 const bool valid = s.depth_mm > 0;
@@ -93,28 +109,69 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_hist(const float* __restr
 #pragma unroll
             for (int t = 0; t < TM; t++) th[t] = (t < T) ? tf[t] : 0.0f;
             const int hoff = fi * binsPerF;
+            // flipped responses of flip=Both samples take the generic path below
+            if (fr.type == T_DEPTH) {
+                int2 cv[kSamplesPerThread][8];
+                bool ok[kSamplesPerThread];
+#pragma unroll
+                for (int k = 0; k < kSamplesPerThread; k++) {
+                    const int lab = rs[k].lab;
+                    int idx[8];
+                    ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                            responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
+                                    ((lab >> 8) & 3) == 1, idx);
+                    const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
+#pragma unroll
+                    for (int c = 0; c < 8; c++) cv[k][c] = dep[ok[k] ? idx[c] : 0];
+                }
+#pragma unroll
+                for (int k = 0; k < kSamplesPerThread; k++) {
+                    const int lab = rs[k].lab;
+                    if (lab < 0) continue;
+                    int b = T;
+                    if (ok[k]) b = binOf<TM>(roundUpToFloat(depthResponseFromCorners(cv[k])), th, T);
+                    atomicAdd(&h[hoff + b * C + (lab & 0xff)], 1u);
+                }
+            } else {
+                float cv[kSamplesPerThread][8];
+                bool ok[kSamplesPerThread];
+#pragma unroll
+                for (int k = 0; k < kSamplesPerThread; k++) {
+                    const int lab = rs[k].lab;
+                    int idx[8];
+                    ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                            responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
+                                    ((lab >> 8) & 3) == 1, idx);
+                    const float* colA = colBase + static_cast<size_t>(rs[k].img) * 3 * plane + fr.c1 * plane;
+                    const float* colB = colBase + static_cast<size_t>(rs[k].img) * 3 * plane + fr.c2 * plane;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) cv[k][c] = colA[ok[k] ? idx[c] : 0];
+#pragma unroll
+                    for (int c = 4; c < 8; c++) cv[k][c] = colB[ok[k] ? idx[c] : 0];
+                }
+#pragma unroll
+                for (int k = 0; k < kSamplesPerThread; k++) {
+                    const int lab = rs[k].lab;
+                    if (lab < 0) continue;
+                    int b = T;
+                    if (ok[k]) b = binOf<TM>(roundUpToFloat(colorResponseFromCorners(cv[k])), th, T);
+                    atomicAdd(&h[hoff + b * C + (lab & 0xff)], 1u);
+                }
+            }
+            // the second (flipped) response of samples with flip setting Both
 #pragma unroll
             for (int k = 0; k < kSamplesPerThread; k++) {
                 const int lab = rs[k].lab;
-                if (lab >= 0) {
-                    const int label = lab & 0xff;
-                    const int flipSetting = (lab >> 8) & 3;
-                    const bool valid = (lab >> 10) == 0;
+                if (lab < 0 || ((lab >> 8) & 3) != 2) continue;
+                int b = T;
+                if ((lab >> 10) == 0) {
                     const float* col = colBase + static_cast<size_t>(rs[k].img) * 3 * plane;
                     const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
-                    const int px = rs[k].xy & 0xffff, py = rs[k].xy >> 16;
-                    int b1 = T, b2 = T;
-                    if (valid) {
-                        const double r1 = featureResponseR(col, dep, W, H, fr, px, py, rs[k].rinv, flipSetting == 1);
-                        b1 = binOf<TM>(roundUpToFloat(r1), th, T);
-                        if (flipSetting == 2) {
-                            const double r2 = featureResponseR(col, dep, W, H, fr, px, py, rs[k].rinv, true);
-                            b2 = binOf<TM>(roundUpToFloat(r2), th, T);
-                        }
-                    }
-                    atomicAdd(&h[hoff + b1 * C + label], 1u);
-                    if (flipSetting == 2) atomicAdd(&h[hoff + b2 * C + label], 1u);
+                    const double r = featureResponseR(col, dep, W, H, fr, rs[k].xy & 0xffff, rs[k].xy >> 16,
+                            rs[k].rinv, true);
+                    b = binOf<TM>(roundUpToFloat(r), th, T);
                 }
+                atomicAdd(&h[hoff + b * C + (lab & 0xff)], 1u);
             }
         }
     }
@@ -386,25 +443,26 @@ void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* 
     const int tiles = (tabs.F + kFeatTile - 1) / kFeatTile;
     const size_t lds = static_cast<size_t>(kFeatTile) * (tabs.T + 1) * C * sizeof(uint32_t);
     if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
-    const dim3 grid(numChunks, tiles), block(kEvalThreads);
+    const int groups = (numChunks + kXcds - 1) / kXcds;
+    const dim3 grid(groups * kXcds * tiles), block(kEvalThreads);
     if (tabs.T <= 8) {
         hipLaunchKernelGGL(k_eval_hist<8>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
-                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+                chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
     } else if (tabs.T <= 16) {
         hipLaunchKernelGGL(k_eval_hist<16>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
-                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+                chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
     } else if (tabs.T <= 24) {
         hipLaunchKernelGGL(k_eval_hist<24>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
-                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+                chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
     } else if (tabs.T <= 32) {
         hipLaunchKernelGGL(k_eval_hist<32>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
-                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+                chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
     } else if (tabs.T <= 64) {
         hipLaunchKernelGGL(k_eval_hist<64>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
-                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+                chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
     } else {
         hipLaunchKernelGGL(k_eval_hist<kMaxThresholds>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
-                chunks, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
+                chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);
     }
 }
 

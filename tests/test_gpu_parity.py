@@ -95,6 +95,22 @@ def test_known_answer_complex_case(ctx):
             assert tuple(c[f, t, label]) == lr
 
 
+def test_invalid_features_rejected_on_host(ctx):
+    """A colour channel outside 0..2 or an unknown type is refused before any
+    kernel runs (it would address outside the integral images)."""
+    from test_oracle_pins import complex_case
+    color, depth, feats, thr, samples = complex_case()
+    im = _images_f32(ctx, color, depth)
+    for p, v in ((9, 3), (10, -1), (0, 2)):
+        bad = feats.copy()
+        bad[0, 0] = 1
+        bad[p, 0] = v
+        with pytest.raises(cf.CurfilError, match="invalid"):
+            cf.feature_responses(ctx, im, samples, bad)
+        with pytest.raises(cf.CurfilError, match="invalid"):
+            cf.node_counters(ctx, im, samples, bad, thr, 2)
+
+
 def test_known_answer_depth_simple(ctx):
     W, H = 16, 20
     depth = np.zeros((1, H, W), np.int32)
