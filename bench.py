@@ -51,7 +51,21 @@ def main():
     ap.add_argument("--frames", type=int, default=800)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only copy a device buffer of this many MiB 3 times (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration_mb:
+        import torch
+        n = args.copy_calibration_mb << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {n} bytes 3 times")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
               f"this build", file=sys.stderr)
@@ -106,16 +120,58 @@ def main():
     value = evals * len(step_times) / total_time
     nodes = [len(forest.tree(t)[0]) for t in range(trees)]
 
-    # roofline of the split-evaluation (histogram) kernel, from its HIP-event time
+    # roofline of the response kernel (pass 1 of the level evaluation: the integral gathers), from its
+    # HIP-event time on the library's stream
     eval_s = sum(p["eval_kernel_ms"] for p in profiles) / 1e3
     responses = sum(p["responses"] for p in profiles)
     achieved = responses * BYTES_PER_RESPONSE / eval_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_eval_hist", "launches": int(sum(p["eval_launches"] for p in profiles)),
+                "kernel": "k_eval_bins", "launches": int(sum(p["eval_launches"] for p in profiles)),
                 "avg_launch_ms": round(sum(p["eval_kernel_ms"] for p in profiles) /
                                        max(1, sum(p["eval_launches"] for p in profiles)), 4),
                 "bytes_per_response": BYTES_PER_RESPONSE}
+
+    plain_vs_batched = predict = None
+    if not args.no_extras:
+        plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(forest, rgb, depth, labels, cfg_kw, args.cpu_sample)
+
+    out = {
+        "metric": "feature-threshold evals/s (train)",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": len(step_times),
+        "warmup": args.warmup,
+        "ms_per_step": total_time / len(step_times) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain full-frame training: {N} frames {W}x{H}, {trees} trees depth 18, "
+                               f"4000 samples/image/tree, F=2000 x T=20, boxRadius 120, regionSize 20, "
+                               f"minSampleCount 50, 4 classes + void",
+                   "mode": "batched level-wise", "evaluations_per_step": evals, "nodes_per_tree": nodes,
+                   "root_samples_per_tree": roots[0]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "phase_ms": {k: round(prof[k], 1) for k in ("featgen_ms", "eval_kernel_ms", "hist_ms", "score_ms",
+                                                    "partition_ms")},
+        "plain_vs_batched": plain_vs_batched,
+        "predict": predict,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):
+    """Plain per-node vs batched training on one tree, and dense prediction."""
+    import numpy as np
+    import curfil_amd as cf
 
     # plain per-node path vs batched on the same single tree (tree 0, depth 12)
     cfg1 = cf.TrainingConfiguration(**dict(cfg_kw, samplesPerImage=4000, maxDepth=12))
@@ -147,41 +203,14 @@ def main():
     lat = np.array(lat)
     log(f"predict: {mpix:.1f} Mpix/s batched ({N / pred_s:.1f} frames/s); single-frame p50 "
         f"{np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(forest, rgb, depth, labels, cfg_kw, args.cpu_sample)
-
-    out = {
-        "metric": "feature-threshold evals/s (train)",
-        "value": value,
-        "unit": "evals/s",
-        "n_gpus": 1,
-        "steps": len(step_times),
-        "warmup": args.warmup,
-        "ms_per_step": total_time / len(step_times) * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"plain full-frame training: {N} frames {W}x{H}, {trees} trees depth 18, "
-                               f"4000 samples/image/tree, F=2000 x T=20, boxRadius 120, regionSize 20, "
-                               f"minSampleCount 50, 4 classes + void",
-                   "mode": "batched level-wise", "evaluations_per_step": evals, "nodes_per_tree": nodes,
-                   "root_samples_per_tree": roots[0]},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "phase_ms": {k: round(prof[k], 1) for k in ("featgen_ms", "eval_kernel_ms", "score_ms", "partition_ms")},
-        "plain_vs_batched": {"workload": "tree 0 only, depth 12", "plain_evals_per_s": plain_evals / plain_s,
-                             "batched_evals_per_s": plain_evals / batched1_s,
-                             "speedup": plain_s / batched1_s, "nodes": int(plain_nodes)},
-        "predict": {"frames": N, "trees": trees, "batch": batch, "mpix_per_s": mpix, "frames_per_s": N / pred_s,
-                    "single_frame_p50_ms": float(np.percentile(lat, 50)),
-                    "single_frame_p99_ms": float(np.percentile(lat, 99)),
-                    "note": "batched = labels to host; single frame = posteriors + labels to host"},
-    }
-    print(json.dumps(out), flush=True)
+    plain_vs_batched = {"workload": "tree 0 only, depth 12", "plain_evals_per_s": plain_evals / plain_s,
+                        "batched_evals_per_s": plain_evals / batched1_s,
+                        "speedup": plain_s / batched1_s, "nodes": int(plain_nodes)}
+    predict = {"frames": N, "trees": trees, "batch": batch, "mpix_per_s": mpix, "frames_per_s": N / pred_s,
+               "single_frame_p50_ms": float(np.percentile(lat, 50)),
+               "single_frame_p99_ms": float(np.percentile(lat, 99)),
+               "note": "batched = labels to host; single frame = posteriors + labels to host"}
+    return plain_vs_batched, predict
 
 
 def cpu_baseline(forest, rgb, depth, labels, cfg_kw, sample):

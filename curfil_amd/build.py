@@ -19,6 +19,7 @@ SOURCES = [
     "k_integral.hip",
     "k_features.hip",
     "k_eval.hip",
+    "k_bins.hip",
     "k_partition.hip",
     "k_classify.hip",
     "capi.cpp",

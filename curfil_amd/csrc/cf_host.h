@@ -24,6 +24,7 @@ struct TrainStats {
     // device time per phase (ms), filled when the context profiles
     double evalKernelMs = 0, evalLaunches = 0;
     double featGenMs = 0, scoreMs = 0, partitionMs = 0;
+    double histMs = 0;   // two-pass evaluation: node histograms from the bin rows
 };
 
 // One tree to grow: its id, random source state and root samples.

@@ -172,8 +172,34 @@ struct FeatSummary {
 // histogram bins G[slot][F][T+1][C] (interval histograms)
 void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const Chunk* chunks,
         int numChunks, const FeatTables& tabs, int C, uint32_t* G);
+
+// ---- two-pass level evaluation (k_bins.hip)
+constexpr int kBinGroup = 16;            // features per bin row (16 x 8-bit bins = one uint4 per sample)
+constexpr int kMaxBinThresholds = 255;   // bins 0..T must fit in 8 bits
+// a block of <= 1024 consecutive entries of the image-major permutation, all of one tree
+struct BinBlock {
+    int tree;
+    int begin;   // first position in the permutation
+    int count;
+    int pad;
+};
+void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder);
+void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
+        uint32_t* vals);
+void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels);
+size_t radixSortTempBytes(int n, int bits);
+void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
+        const uint32_t* valsIn, uint32_t* valsOut, int n, int bits);
+// bins[g][level index] for feature groups [g0, g0 + ng) of the evaluation order
+void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
+        const BinBlock* blocks, int numBlocks, int g0, int ng, const uint16_t* evalOrder, const FeatTables& tabs, int N,
+        uint4* bins);
+// G[slot][ng * 16][T + 1][C] from the bin rows of the chunks' samples
+void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
+        int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G);
 void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
-        const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot);
+        const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot, int fOffset = 0,
+        int Fb = -1, const uint16_t* evalOrder = nullptr);
 void launchScoresFromCounters(hipStream_t s, const uint32_t* counters, const uint32_t* hist, int F, int T, int C,
         double* scores);
 void launchReduce(hipStream_t s, const FeatSummary* summ, int numSlots, int F, NodeBest* best);

@@ -1,0 +1,313 @@
+// Two-pass split evaluation of a whole level (batched training):
+//
+//   k_eval_bins   pass 1, image-major: the level's samples ordered by
+//                 (tree, image) are cut into blocks of 1024; one workgroup per
+//                 (block, group of 16 features) computes every response's
+//                 interval among the feature's thresholds (the bin used by
+//                 the interval histograms) and stores the 16 bins of a sample
+//                 as one 16-byte row, bins[group][level index].
+//   k_hist_bins   pass 2, node-major: one workgroup per (node chunk, group)
+//                 streams the chunk's bin rows and labels into LDS interval
+//                 histograms and adds them to the node's global bins.
+//
+// Counting is order-independent, so the histograms equal those of the direct
+// path (k_eval_hist), whose single kernel gathers the integrals in node order:
+// at deep levels a node's samples come from every image and the gathers miss
+// in L2.  In pass 1 a workgroup's samples come from one or two images, the
+// workgroups of one XCD sweep one run of blocks (about one image per tree)
+// for all feature groups before the next, and features are visited in an
+// order grouped by the integral planes they read (depth, then colour channel
+// pairs), so an XCD's L2 holds the planes in use.
+#include <hipcub/hipcub.hpp>
+
+#include "cf_internal.h"
+
+namespace cf {
+
+namespace {
+
+constexpr int kBinThreads = 256;
+constexpr int kBinPerThread = 4;
+constexpr int kSuper = 4;     // sample blocks an XCD sweeps together
+constexpr int kXcdCount = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs
+
+struct BinSample {
+    double rinv;   // depthReciprocal(depth in metres)
+    int img;
+    int xy;        // x | y << 16
+    int lab;       // flip << 8 | (invalid depth) << 10;  < 0: empty slot
+};
+
+template<int TM>
+__device__ __forceinline__ int binOfT(float q, const float (&th)[TM], int T) {
+    int b = 0;
+#pragma unroll
+    for (int t = 0; t < TM; t++) {
+        if (t < T) b += (th[t] < q) ? 1 : 0;
+    }
+    return (q != q) ? T : b;
+}
+
+// Shifts a sample's 16-byte bin row right by one byte and inserts b as the
+// last byte: after 16 inserts the first inserted bin is byte 0.
+__device__ __forceinline__ void pushBin(uint32_t (&w)[4], uint32_t b) {
+    w[0] = (w[0] >> 8) | (w[1] << 24);
+    w[1] = (w[1] >> 8) | (w[2] << 24);
+    w[2] = (w[2] >> 8) | (w[3] << 24);
+    w[3] = (w[3] >> 8) | (b << 24);
+}
+
+}  // namespace
+
+template<int TM>
+__global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restrict__ colBase,
+        const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
+        const uint32_t* __restrict__ perm, const BinBlock* __restrict__ blocks, int numBlocks, int g0, int ng,
+        const uint16_t* __restrict__ evalOrder, const FeatRec* __restrict__ feat, const float* __restrict__ thrFT,
+        int F, int T, int N, uint4* __restrict__ bins) {
+    // blockIdx -> (XCD, j); XCD x takes super-blocks x, x + 8, ...; within a
+    // super-block the order is feature group major, sample block minor.
+    const int xcd = blockIdx.x % kXcdCount;
+    const int j = blockIdx.x / kXcdCount;
+    const int perSuper = ng * kSuper;
+    const int sup = (j / perSuper) * kXcdCount + xcd;
+    const int r = j % perSuper;
+    const int g = r / kSuper;
+    const int blk = sup * kSuper + r % kSuper;
+    if (blk >= numBlocks) return;
+    const BinBlock bb = blocks[blk];
+    const size_t plane = static_cast<size_t>(W) * H;
+
+    BinSample rs[kBinPerThread];
+    int outIdx[kBinPerThread];
+#pragma unroll
+    for (int k = 0; k < kBinPerThread; k++) {
+        const int i = k * kBinThreads + threadIdx.x;
+        // empty lanes gather (unused) corners at index 0 of image 0
+        rs[k].lab = -1;
+        rs[k].img = 0;
+        rs[k].xy = 0;
+        rs[k].rinv = 0.0;
+        outIdx[k] = -1;
+        if (i < bb.count) {
+            const int s = static_cast<int>(perm[bb.begin + i]);
+            const cf_sample x = samples[s];
+            const bool valid = x.depth_mm > 0;
+            outIdx[k] = s;
+            rs[k].rinv = valid ? depthReciprocal(depthToFloat(x.depth_mm)) : 0.0;
+            rs[k].img = static_cast<int>(x.image);
+            rs[k].xy = x.x | (x.y << 16);
+            rs[k].lab = (x.flip << 8) | ((valid ? 0 : 1) << 10);
+        }
+    }
+    uint32_t row[kBinPerThread][4];
+#pragma unroll
+    for (int k = 0; k < kBinPerThread; k++) row[k][0] = row[k][1] = row[k][2] = row[k][3] = 0u;
+
+    const int pos0 = (g0 + g) * kBinGroup;
+    const uint16_t* eo = evalOrder + static_cast<size_t>(bb.tree) * F;
+    for (int fi = 0; fi < kBinGroup; fi++) {
+        const int pos = pos0 + fi;
+        if (pos >= F) {   // uniform: the last group of the table may be partial
+#pragma unroll
+            for (int k = 0; k < kBinPerThread; k++) pushBin(row[k], 0u);
+            continue;
+        }
+        const int f = eo[pos];
+        const size_t tf = static_cast<size_t>(bb.tree) * F + f;
+        const Feat fr = loadFeat(feat + tf);
+        float th[TM];
+        const float* tp = thrFT + tf * T;
+#pragma unroll
+        for (int t = 0; t < TM; t++) th[t] = (t < T) ? tp[t] : 0.0f;
+        if (fr.type == T_DEPTH) {
+            int2 cv[kBinPerThread][8];
+            bool ok[kBinPerThread];
+#pragma unroll
+            for (int k = 0; k < kBinPerThread; k++) {
+                const int lab = rs[k].lab;
+                int idx[8];
+                ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                        responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
+                                ((lab >> 8) & 3) == 1, idx);
+                const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
+#pragma unroll
+                for (int c = 0; c < 8; c++) cv[k][c] = dep[ok[k] ? idx[c] : 0];
+            }
+#pragma unroll
+            for (int k = 0; k < kBinPerThread; k++) {
+                const int b = ok[k] ? binOfT<TM>(roundUpToFloat(depthResponseFromCorners(cv[k])), th, T) : T;
+                pushBin(row[k], static_cast<uint32_t>(b));
+            }
+        } else {
+            float cv[kBinPerThread][8];
+            bool ok[kBinPerThread];
+#pragma unroll
+            for (int k = 0; k < kBinPerThread; k++) {
+                const int lab = rs[k].lab;
+                int idx[8];
+                ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                        responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
+                                ((lab >> 8) & 3) == 1, idx);
+                const float* colA = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c1) * plane;
+                const float* colB = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c2) * plane;
+#pragma unroll
+                for (int c = 0; c < 4; c++) cv[k][c] = colA[ok[k] ? idx[c] : 0];
+#pragma unroll
+                for (int c = 4; c < 8; c++) cv[k][c] = colB[ok[k] ? idx[c] : 0];
+            }
+#pragma unroll
+            for (int k = 0; k < kBinPerThread; k++) {
+                const int b = ok[k] ? binOfT<TM>(roundUpToFloat(colorResponseFromCorners(cv[k])), th, T) : T;
+                pushBin(row[k], static_cast<uint32_t>(b));
+            }
+        }
+    }
+    uint4* out = bins + static_cast<size_t>(g) * N;
+#pragma unroll
+    for (int k = 0; k < kBinPerThread; k++) {
+        if (outIdx[k] >= 0) out[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
+    }
+}
+
+// Pass 2: interval histograms of (node chunk, feature group) from the bin rows.
+// G holds the batch's groups: G[slot][ng * 16][T + 1][C].
+__global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bins, const uint8_t* __restrict__ labels,
+        const Chunk* __restrict__ chunks, int g0, int ng, int F, int T, int C, int N, uint32_t* __restrict__ G) {
+    extern __shared__ uint32_t h[];
+    const int chunkIdx = blockIdx.x / ng;
+    const int g = blockIdx.x % ng;
+    const Chunk ch = chunks[chunkIdx];
+    const int nf = min(kBinGroup, F - (g0 + g) * kBinGroup);
+    const int binsPerF = (T + 1) * C;
+    const int nb = nf * binsPerF;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    const uint4* row = bins + static_cast<size_t>(g) * N + ch.begin;
+    const uint8_t* lab = labels + ch.begin;
+    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+        const uint4 w = row[i];
+        const int l = lab[i];
+        const uint32_t words[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int fi = 0; fi < kBinGroup; fi++) {
+            if (fi < nf) {
+                const int b = static_cast<int>((words[fi >> 2] >> (8 * (fi & 3))) & 0xffu);
+                atomicAdd(&h[fi * binsPerF + b * C + l], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t* out = G + (static_cast<size_t>(ch.slot) * ng + g) * kBinGroup * binsPerF;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        const uint32_t v = h[k];
+        if (v) atomicAdd(&out[k], v);
+    }
+}
+
+// Per tree: feature evaluation order grouped by the planes a feature reads
+// (depth first, then colour by (channel1, channel2)), stable in feature index.
+__global__ void __launch_bounds__(64) k_feature_eval_order(const FeatRec* __restrict__ feat, int F,
+        uint16_t* __restrict__ evalOrder) {
+    constexpr int kKeys = 10;
+    __shared__ int count[kKeys];
+    __shared__ int start[kKeys];
+    const FeatRec* ft = feat + static_cast<size_t>(blockIdx.x) * F;
+    uint16_t* eo = evalOrder + static_cast<size_t>(blockIdx.x) * F;
+    if (threadIdx.x < kKeys) count[threadIdx.x] = 0;
+    __syncthreads();
+    auto keyOf = [](const Feat& x) { return x.type == T_DEPTH ? 0 : 1 + x.c1 * 3 + x.c2; };
+    for (int f = threadIdx.x; f < F; f += blockDim.x) atomicAdd(&count[keyOf(loadFeat(ft + f))], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < kKeys; k++) {
+            start[k] = acc;
+            acc += count[k];
+        }
+        // stable placement (F is small: a few thousand features per table)
+        for (int f = 0; f < F; f++) eo[start[keyOf(loadFeat(ft + f))]++] = static_cast<uint16_t>(f);
+    }
+}
+
+// Radix-sort keys (tree * images + image) and values (level index) of one
+// tree's level segment.
+__global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict__ samples, int begin, int n,
+        uint32_t keyBase, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[begin + i] = keyBase + samples[begin + i].image;
+    vals[begin + i] = static_cast<uint32_t>(begin + i);
+}
+
+__global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restrict__ samples, int n,
+        uint8_t* __restrict__ labels) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) labels[i] = samples[i].label;
+}
+
+void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder) {
+    hipLaunchKernelGGL(k_feature_eval_order, dim3(trees), dim3(64), 0, s, tabs.feat.get(), tabs.F, evalOrder);
+}
+
+void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
+        uint32_t* vals) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals);
+}
+
+void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_level_labels, dim3((n + 255) / 256), dim3(256), 0, s, samples, n, labels);
+}
+
+size_t radixSortTempBytes(int n, int bits) {
+    size_t bytes = 0;
+    CF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const uint32_t*>(nullptr),
+            static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+            n, 0, bits));
+    return bytes;
+}
+
+void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
+        const uint32_t* valsIn, uint32_t* valsOut, int n, int bits) {
+    CF_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n, 0, bits, s));
+}
+
+void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
+        const BinBlock* blocks, int numBlocks, int g0, int ng, const uint16_t* evalOrder, const FeatTables& tabs, int N,
+        uint4* bins) {
+    if (numBlocks == 0 || ng == 0) return;
+    if (tabs.T > kMaxBinThresholds) throw Error(CF_ERR_ARG, "too many thresholds for 8-bit bins");
+    const int supers = (numBlocks + kSuper - 1) / kSuper;
+    const int perXcd = (supers + kXcdCount - 1) / kXcdCount;
+    const dim3 grid(static_cast<unsigned>(perXcd) * kXcdCount * ng * kSuper), block(kBinThreads);
+#define CF_EVAL_BINS(TMV)                                                                                     \
+    hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, 0, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
+            blocks, numBlocks, g0, ng, evalOrder, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, N, bins)
+    if (tabs.T <= 8) {
+        CF_EVAL_BINS(8);
+    } else if (tabs.T <= 16) {
+        CF_EVAL_BINS(16);
+    } else if (tabs.T <= 24) {
+        CF_EVAL_BINS(24);
+    } else if (tabs.T <= 32) {
+        CF_EVAL_BINS(32);
+    } else if (tabs.T <= 64) {
+        CF_EVAL_BINS(64);
+    } else {
+        CF_EVAL_BINS(kMaxBinThresholds);
+    }
+#undef CF_EVAL_BINS
+}
+
+void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
+        int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G) {
+    if (numChunks == 0 || ng == 0) return;
+    const size_t lds = static_cast<size_t>(kBinGroup) * (tabs.T + 1) * C * sizeof(uint32_t);
+    if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
+    hipLaunchKernelGGL(k_hist_bins, dim3(static_cast<unsigned>(numChunks) * ng), dim3(256), lds, s, bins, labels,
+            chunks, g0, ng, tabs.F, tabs.T, C, N, G);
+}
+
+}  // namespace cf

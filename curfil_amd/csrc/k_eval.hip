@@ -242,19 +242,24 @@ template<int CM>
 __global__ void __launch_bounds__(256) k_scores(const uint32_t* __restrict__ G, const uint32_t* __restrict__ nodeHist,
         const int* __restrict__ slotTree, int numSlots, const uint16_t* __restrict__ rank,
         const uint16_t* __restrict__ order, int F, int T, int C, FeatSummary* __restrict__ summ,
-        double* __restrict__ allScores, int onlySlot) {
+        double* __restrict__ allScores, int onlySlot, int fOffset, int Fb, const uint16_t* __restrict__ evalOrder) {
+    // G holds Fb features per slot starting at evaluation position fOffset;
+    // position p is feature evalOrder[tree][p] (p itself without an order).
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    int slot, f;
+    int slot, j;
     if (onlySlot >= 0) {
         slot = onlySlot;
-        f = gid;
-        if (f >= F) return;
+        j = gid;
+        if (j >= Fb) return;
     } else {
-        if (gid >= numSlots * F) return;
-        slot = gid / F;
-        f = gid % F;
+        if (gid >= numSlots * Fb) return;
+        slot = gid / Fb;
+        j = gid % Fb;
     }
+    const int pos = fOffset + j;
+    if (pos >= F) return;
     const int tree = slotTree[slot];
+    const int f = evalOrder ? static_cast<int>(evalOrder[static_cast<size_t>(tree) * F + pos]) : pos;
     uint32_t A[CM], run[CM];
 #pragma unroll
     for (int c = 0; c < CM; c++) {
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(256) k_scores(const uint32_t* __restrict__ G, 
         run[c] = 0;
     }
     const size_t tf = static_cast<size_t>(tree) * F + f;
-    const uint32_t* g = G + (static_cast<size_t>(slot) * F + f) * (T + 1) * C;
+    const uint32_t* g = G + (static_cast<size_t>(slot) * Fb + j) * (T + 1) * C;
     const uint16_t* rk = rank + tf * T;
     const uint16_t* od = order + tf * T;
     double best = -DBL_MAX, second = -DBL_MAX;
@@ -467,15 +472,19 @@ void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* 
 }
 
 void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
-        const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot) {
-    const int threads = onlySlot >= 0 ? tabs.F : numSlots * tabs.F;
+        const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot, int fOffset, int Fb,
+        const uint16_t* evalOrder) {
+    if (Fb < 0) Fb = tabs.F;
+    const int threads = onlySlot >= 0 ? Fb : numSlots * Fb;
+    if (threads == 0) return;
     const dim3 grid((threads + 255) / 256);
     if (C <= 8) {
         hipLaunchKernelGGL(k_scores<8>, grid, dim3(256), 0, s, G, nodeHist, slotTree, numSlots, tabs.rank.get(),
-                tabs.order.get(), tabs.F, tabs.T, C, summ, allScores, onlySlot);
+                tabs.order.get(), tabs.F, tabs.T, C, summ, allScores, onlySlot, fOffset, Fb, evalOrder);
     } else {
         hipLaunchKernelGGL(k_scores<kMaxClasses>, grid, dim3(256), 0, s, G, nodeHist, slotTree, numSlots,
-                tabs.rank.get(), tabs.order.get(), tabs.F, tabs.T, C, summ, allScores, onlySlot);
+                tabs.rank.get(), tabs.order.get(), tabs.F, tabs.T, C, summ, allScores, onlySlot, fOffset, Fb,
+                evalOrder);
     }
 }
 

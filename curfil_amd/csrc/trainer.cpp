@@ -15,6 +15,7 @@
 // launch sequence and one host sync per node) or for every node of every
 // tree of the level together (CF_MODE_BATCHED).
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -78,6 +79,13 @@ private:
 size_t evalBudgetBytes() {
     const char* e = std::getenv("CURFIL_EVAL_BUDGET_MB");
     size_t mb = e ? static_cast<size_t>(std::atoll(e)) : 2048;
+    if (mb < 16) mb = 16;
+    return mb << 20;
+}
+
+size_t binsBudgetBytes() {
+    const char* e = std::getenv("CURFIL_BINS_BUDGET_MB");
+    size_t mb = e ? static_cast<size_t>(std::atoll(e)) : 4096;
     if (mb < 16) mb = 16;
     return mb << 20;
 }
@@ -152,6 +160,17 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     }
     int cur = 0;
 
+    // Batched levels use the two-pass evaluation (image-major bins, then
+    // node histograms) unless a sample is evaluated in both orientations
+    // (one 8-bit bin per sample and feature), T exceeds the 8-bit bins, or
+    // CURFIL_EVAL_DIRECT is set.
+    bool twoPass = mode == CF_MODE_BATCHED && T <= kMaxBinThresholds && std::getenv("CURFIL_EVAL_DIRECT") == nullptr;
+    for (const auto& j : jobs) {
+        for (const cf_sample& x : j.roots) {
+            if (x.flip == 2) twoPass = false;
+        }
+    }
+
     FeatTables tabs;
     DevBuf<FeatGenJob> dJobs;
     DevBuf<Chunk> dChunks;
@@ -165,11 +184,19 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<int> dChunkLeft;
     DevBuf<uint32_t> dLeftHist;
     DevBuf<int2> dDst;
+    DevBuf<uint32_t> dKeys, dKeys2, dVals, dPerm;
+    DevBuf<uint8_t> dSortTemp, dLab8;
+    DevBuf<BinBlock> dBlocks;
+    DevBuf<uint16_t> dEvalOrder;
+    DevBuf<uint4> dBins;
+    const size_t binsBudget = binsBudgetBytes();
     PhaseTimer timer(s, ctx->profiling);
     const size_t perSlot = static_cast<size_t>(F) * (T + 1) * C;
     const size_t budget = evalBudgetBytes();
     size_t maxSlots = std::max<size_t>(1, budget / (perSlot * sizeof(uint32_t)));
 
+    // per-level evaluation log on stderr (with profiling on), like the reference's per-level INFO lines
+    const bool levelLog = ctx->profiling && std::getenv("CURFIL_LEVEL_LOG") != nullptr;
     for (int levelNr = 1; !active.empty(); levelNr++) {
         if (levelNr == cfg.max_depth) break;
         stats.levels++;
@@ -223,6 +250,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         }
         const int numNodes = static_cast<int>(nodes.size());
         std::vector<NodeBest> best(numNodes);
+        const double evalMsBefore = stats.evalKernelMs;
 
         // ---- evaluation
         auto evalSlots = [&](int s0, int s1) {
@@ -270,14 +298,132 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 stats.replays++;
             }
         };
+        // the literal sequential argmax of one node from its direct-path histograms
+        auto replayNode = [&](int i) {
+            const LevelNode& ln = nodes[i];
+            std::vector<Chunk> bc(chunks.begin() + ln.chunkBegin, chunks.begin() + ln.chunkEnd);
+            for (auto& c : bc) c.slot = 0;
+            dChunks.resize(bc.size());
+            dG.resize(perSlot);
+            dAllScores.resize(static_cast<size_t>(T) * F);
+            CF_HIP(hipMemcpyAsync(dChunks.get(), bc.data(), bc.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
+            CF_HIP(hipMemsetAsync(dG.get(), 0, perSlot * sizeof(uint32_t), s));
+            launchEvalHistograms(s, images->set(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()), tabs, C,
+                    dG.get());
+            launchScores(s, dG.get(), dNodeHist.get() + static_cast<size_t>(i) * C, dSlotTree.get() + i, 1, tabs, C,
+                    dSumm.get() + static_cast<size_t>(i) * F, dAllScores.get(), 0);
+            launchSequentialArgmax(s, dAllScores.get(), T, F, dBest.get() + i);
+            CF_HIP(hipMemcpyAsync(&best[i], dBest.get() + i, sizeof(NodeBest), hipMemcpyDeviceToHost, s));
+            CF_HIP(hipStreamSynchronize(s));
+            stats.replays++;
+        };
+        auto evalTwoPass = [&]() {
+            int N = 0;
+            for (int a = 0; a < numActive; a++) N = std::max(N, active[a].begin + active[a].n);
+            // image-major order of the level: (tree, image) keys, stable radix sort of level indices
+            dKeys.resize(N);
+            dKeys2.resize(N);
+            dVals.resize(N);
+            dPerm.resize(N);
+            dLab8.resize(N);
+            for (int a = 0; a < numActive; a++) {
+                launchLevelKeys(s, level[cur].get(), active[a].begin, active[a].n,
+                        static_cast<uint32_t>(a) * static_cast<uint32_t>(images->count), dKeys.get(), dVals.get());
+            }
+            const uint64_t keyRange = static_cast<uint64_t>(numActive) * static_cast<uint64_t>(images->count);
+            int bits = 1;
+            while ((1ull << bits) < keyRange) bits++;
+            const size_t sortBytes = radixSortTempBytes(N, bits);
+            dSortTemp.resize(std::max<size_t>(sortBytes, 1));
+            radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), N, bits);
+            launchLevelLabels(s, level[cur].get(), N, dLab8.get());
+            std::vector<BinBlock> blocks;
+            for (int a = 0; a < numActive; a++) {
+                for (int p = 0; p < active[a].n; p += 1024) {
+                    blocks.push_back(BinBlock{a, active[a].begin + p, std::min(1024, active[a].n - p), 0});
+                }
+            }
+            dBlocks.resize(blocks.size());
+            CF_HIP(hipMemcpyAsync(dBlocks.get(), blocks.data(), blocks.size() * sizeof(BinBlock), hipMemcpyHostToDevice, s));
+            dEvalOrder.resize(static_cast<size_t>(numActive) * F);
+            launchFeatureEvalOrder(s, tabs, numActive, dEvalOrder.get());
+            // node histograms and trees of the whole level
+            std::vector<uint32_t> nh(static_cast<size_t>(numNodes) * C);
+            std::vector<int> st(numNodes);
+            for (int i = 0; i < numNodes; i++) {
+                const LevelNode& ln = nodes[i];
+                std::memcpy(&nh[static_cast<size_t>(i) * C],
+                        &out[active[ln.active].job].hist[static_cast<size_t>(ln.node) * C], C * sizeof(uint32_t));
+                st[i] = ln.active;
+            }
+            dNodeHist.resize(nh.size());
+            dSlotTree.resize(numNodes);
+            dSumm.resize(static_cast<size_t>(numNodes) * F);
+            dBest.resize(numNodes);
+            CF_HIP(hipMemcpyAsync(dNodeHist.get(), nh.data(), nh.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            CF_HIP(hipMemcpyAsync(dSlotTree.get(), st.data(), numNodes * sizeof(int), hipMemcpyHostToDevice, s));
+            // feature groups in batches that fit the bin-row budget; node slots in batches that fit the histogram budget
+            const int groupsTotal = (F + kBinGroup - 1) / kBinGroup;
+            const size_t rowBytes = static_cast<size_t>(N) * sizeof(uint4);
+            const int ng = static_cast<int>(std::max<size_t>(1, std::min<size_t>(groupsTotal, binsBudget / rowBytes)));
+            dBins.resize(static_cast<size_t>(ng) * N);
+            for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
+                const int ngb = std::min(ng, groupsTotal - g0);
+                timer.begin();
+                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(),
+                        static_cast<int>(blocks.size()), g0, ngb, dEvalOrder.get(), tabs, N, dBins.get());
+                timer.end(stats.evalKernelMs);
+                stats.evalLaunches += 1;
+                const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
+                const int maxSlotsB = static_cast<int>(std::max<size_t>(1, budget / (perSlotB * sizeof(uint32_t))));
+                for (int s0 = 0; s0 < numNodes;) {
+                    const int s1 = std::min(numNodes, s0 + maxSlotsB);
+                    const int ns = s1 - s0;
+                    std::vector<Chunk> bc(chunks.begin() + nodes[s0].chunkBegin, chunks.begin() + nodes[s1 - 1].chunkEnd);
+                    for (auto& c : bc) c.slot -= s0;
+                    dChunks.resize(bc.size());
+                    dG.resize(static_cast<size_t>(ns) * perSlotB);
+                    CF_HIP(hipMemcpyAsync(dChunks.get(), bc.data(), bc.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
+                    CF_HIP(hipMemsetAsync(dG.get(), 0, static_cast<size_t>(ns) * perSlotB * sizeof(uint32_t), s));
+                    timer.begin();
+                    launchHistBins(s, dBins.get(), dLab8.get(), dChunks.get(), static_cast<int>(bc.size()), g0, ngb, tabs,
+                            C, N, dG.get());
+                    timer.end(stats.histMs);
+                    timer.begin();
+                    launchScores(s, dG.get(), dNodeHist.get() + static_cast<size_t>(s0) * C, dSlotTree.get() + s0, ns,
+                            tabs, C, dSumm.get() + static_cast<size_t>(s0) * F, nullptr, -1, g0 * kBinGroup,
+                            ngb * kBinGroup, dEvalOrder.get());
+                    timer.end(stats.scoreMs);
+                    s0 = s1;
+                }
+            }
+            timer.begin();
+            launchReduce(s, dSumm.get(), numNodes, F, dBest.get());
+            timer.end(stats.scoreMs);
+            CF_HIP(hipMemcpyAsync(best.data(), dBest.get(), numNodes * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
+            CF_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < numNodes; i++) {
+                if (best[i].flag) replayNode(i);
+            }
+        };
         if (mode == CF_MODE_PLAIN) {
             for (int i = 0; i < numNodes; i++) evalSlots(i, i + 1);
+        } else if (twoPass) {
+            evalTwoPass();
         } else {
             for (int s0 = 0; s0 < numNodes;) {
                 const int s1 = static_cast<int>(std::min<size_t>(numNodes, s0 + maxSlots));
                 evalSlots(s0, s1);
                 s0 = s1;
             }
+        }
+
+        if (levelLog) {
+            double levelSamples = 0;
+            for (const auto& ln : nodes) levelSamples += ln.count;
+            std::fprintf(stderr, "[curfil] level %d: trees %d nodes %d samples %.0f chunks %zu eval %.2f ms (%.3g responses/s)\n",
+                    levelNr, numActive, numNodes, levelSamples, chunks.size(), stats.evalKernelMs - evalMsBefore,
+                    levelSamples * F / ((stats.evalKernelMs - evalMsBefore) * 1e-3));
         }
 
         // ---- split records
