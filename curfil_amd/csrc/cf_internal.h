@@ -176,7 +176,7 @@ void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* 
 // ---- two-pass level evaluation (k_bins.hip)
 constexpr int kBinGroup = 16;            // features per bin row (16 x 8-bit bins = one uint4 per sample)
 constexpr int kMaxBinThresholds = 255;   // bins 0..T must fit in 8 bits
-// a block of <= 1024 consecutive entries of the image-major permutation, all of one tree
+// a block of <= 1024 consecutive entries of the image-major permutation, all of one (tree, image)
 struct BinBlock {
     int tree;
     int begin;   // first position in the permutation
@@ -190,10 +190,11 @@ void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* 
 size_t radixSortTempBytes(int n, int bits);
 void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
         const uint32_t* valsIn, uint32_t* valsOut, int n, int bits);
-// bins[g][level index] for feature groups [g0, g0 + ng) of the evaluation order
+void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts);
+// bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
-        const BinBlock* blocks, int numBlocks, int g0, int ng, const uint16_t* evalOrder, const FeatTables& tabs, int N,
-        uint4* bins);
+        const BinBlock* blocks, const int2* work, int numWork, int g0, const uint16_t* evalOrder, const FeatTables& tabs,
+        int N, uint4* bins);
 // G[slot][ng * 16][T + 1][C] from the bin rows of the chunks' samples
 void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
         int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G);

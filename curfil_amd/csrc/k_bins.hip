@@ -1,7 +1,8 @@
 // Two-pass split evaluation of a whole level (batched training):
 //
 //   k_eval_bins   pass 1, image-major: the level's samples ordered by
-//                 (tree, image) are cut into blocks of 1024; one workgroup per
+//                 (tree, image) are cut into blocks of <= 1024 within each
+//                 (tree, image) segment; one workgroup per
 //                 (block, group of 16 features) computes every response's
 //                 interval among the feature's thresholds (the bin used by
 //                 the interval histograms) and stores the 16 bins of a sample
@@ -13,9 +14,9 @@
 // Counting is order-independent, so the histograms equal those of the direct
 // path (k_eval_hist), whose single kernel gathers the integrals in node order:
 // at deep levels a node's samples come from every image and the gathers miss
-// in L2.  In pass 1 a workgroup's samples come from one or two images, the
-// workgroups of one XCD sweep one run of blocks (about one image per tree)
-// for all feature groups before the next, and features are visited in an
+// in L2.  In pass 1 a workgroup's samples come from one image, the
+// workgroups of one XCD sweep one (tree, image) segment for all feature
+// groups of a batch before the next, and features are visited in an
 // order grouped by the integral planes they read (depth, then colour channel
 // pairs), so an XCD's L2 holds the planes in use.
 #include <hipcub/hipcub.hpp>
@@ -28,8 +29,6 @@ namespace {
 
 constexpr int kBinThreads = 256;
 constexpr int kBinPerThread = 4;
-constexpr int kSuper = 4;     // sample blocks an XCD sweeps together
-constexpr int kXcdCount = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs
 
 struct BinSample {
     double rinv;   // depthReciprocal(depth in metres)
@@ -62,111 +61,114 @@ __device__ __forceinline__ void pushBin(uint32_t (&w)[4], uint32_t b) {
 template<int TM>
 __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restrict__ colBase,
         const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
-        const uint32_t* __restrict__ perm, const BinBlock* __restrict__ blocks, int numBlocks, int g0, int ng,
-        const uint16_t* __restrict__ evalOrder, const FeatRec* __restrict__ feat, const float* __restrict__ thrFT,
-        int F, int T, int N, uint4* __restrict__ bins) {
-    // blockIdx -> (XCD, j); XCD x takes super-blocks x, x + 8, ...; within a
-    // super-block the order is feature group major, sample block minor.
-    const int xcd = blockIdx.x % kXcdCount;
-    const int j = blockIdx.x / kXcdCount;
-    const int perSuper = ng * kSuper;
-    const int sup = (j / perSuper) * kXcdCount + xcd;
-    const int r = j % perSuper;
-    const int g = r / kSuper;
-    const int blk = sup * kSuper + r % kSuper;
-    if (blk >= numBlocks) return;
-    const BinBlock bb = blocks[blk];
+        const uint32_t* __restrict__ perm, const BinBlock* __restrict__ blocks, const int2* __restrict__ work,
+        int g0, const uint16_t* __restrict__ evalOrder, const FeatRec* __restrict__ feat,
+        const float* __restrict__ thrFT, int F, int T, int N, uint4* __restrict__ bins) {
+    // work[blockIdx] = (sample block, feature group within the batch); the
+    // host interleaves the XCDs' lists (workgroup w runs on XCD w % 8) so each
+    // XCD sweeps one (tree, image) segment for every group of the batch
+    // before the next segment.  Padding entries have a negative block.
+    const int2 wk = work[blockIdx.x];
+    if (wk.x < 0) return;
+    const int g = wk.y;
+    const BinBlock bb = blocks[wk.x];
     const size_t plane = static_cast<size_t>(W) * H;
-
-    BinSample rs[kBinPerThread];
-    int outIdx[kBinPerThread];
-#pragma unroll
-    for (int k = 0; k < kBinPerThread; k++) {
-        const int i = k * kBinThreads + threadIdx.x;
-        // empty lanes gather (unused) corners at index 0 of image 0
-        rs[k].lab = -1;
-        rs[k].img = 0;
-        rs[k].xy = 0;
-        rs[k].rinv = 0.0;
-        outIdx[k] = -1;
-        if (i < bb.count) {
-            const int s = static_cast<int>(perm[bb.begin + i]);
-            const cf_sample x = samples[s];
-            const bool valid = x.depth_mm > 0;
-            outIdx[k] = s;
-            rs[k].rinv = valid ? depthReciprocal(depthToFloat(x.depth_mm)) : 0.0;
-            rs[k].img = static_cast<int>(x.image);
-            rs[k].xy = x.x | (x.y << 16);
-            rs[k].lab = (x.flip << 8) | ((valid ? 0 : 1) << 10);
-        }
-    }
-    uint32_t row[kBinPerThread][4];
-#pragma unroll
-    for (int k = 0; k < kBinPerThread; k++) row[k][0] = row[k][1] = row[k][2] = row[k][3] = 0u;
 
     const int pos0 = (g0 + g) * kBinGroup;
     const uint16_t* eo = evalOrder + static_cast<size_t>(bb.tree) * F;
-    for (int fi = 0; fi < kBinGroup; fi++) {
-        const int pos = pos0 + fi;
-        if (pos >= F) {   // uniform: the last group of the table may be partial
-#pragma unroll
-            for (int k = 0; k < kBinPerThread; k++) pushBin(row[k], 0u);
-            continue;
-        }
-        const int f = eo[pos];
-        const size_t tf = static_cast<size_t>(bb.tree) * F + f;
-        const Feat fr = loadFeat(feat + tf);
-        float th[TM];
-        const float* tp = thrFT + tf * T;
-#pragma unroll
-        for (int t = 0; t < TM; t++) th[t] = (t < T) ? tp[t] : 0.0f;
-        if (fr.type == T_DEPTH) {
-            int2 cv[kBinPerThread][8];
-            bool ok[kBinPerThread];
-#pragma unroll
-            for (int k = 0; k < kBinPerThread; k++) {
-                const int lab = rs[k].lab;
-                int idx[8];
-                ok[k] = lab >= 0 && (lab >> 10) == 0 &&
-                        responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
-                                ((lab >> 8) & 3) == 1, idx);
-                const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
-#pragma unroll
-                for (int c = 0; c < 8; c++) cv[k][c] = dep[ok[k] ? idx[c] : 0];
-            }
-#pragma unroll
-            for (int k = 0; k < kBinPerThread; k++) {
-                const int b = ok[k] ? binOfT<TM>(roundUpToFloat(depthResponseFromCorners(cv[k])), th, T) : T;
-                pushBin(row[k], static_cast<uint32_t>(b));
-            }
-        } else {
-            float cv[kBinPerThread][8];
-            bool ok[kBinPerThread];
-#pragma unroll
-            for (int k = 0; k < kBinPerThread; k++) {
-                const int lab = rs[k].lab;
-                int idx[8];
-                ok[k] = lab >= 0 && (lab >> 10) == 0 &&
-                        responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
-                                ((lab >> 8) & 3) == 1, idx);
-                const float* colA = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c1) * plane;
-                const float* colB = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c2) * plane;
-#pragma unroll
-                for (int c = 0; c < 4; c++) cv[k][c] = colA[ok[k] ? idx[c] : 0];
-#pragma unroll
-                for (int c = 4; c < 8; c++) cv[k][c] = colB[ok[k] ? idx[c] : 0];
-            }
-#pragma unroll
-            for (int k = 0; k < kBinPerThread; k++) {
-                const int b = ok[k] ? binOfT<TM>(roundUpToFloat(colorResponseFromCorners(cv[k])), th, T) : T;
-                pushBin(row[k], static_cast<uint32_t>(b));
-            }
-        }
-    }
     uint4* out = bins + static_cast<size_t>(g) * N;
+    for (int base = 0; base < bb.count; base += kBinThreads * kBinPerThread) {
+        BinSample rs[kBinPerThread];
+        int outIdx[kBinPerThread];
 #pragma unroll
-    for (int k = 0; k < kBinPerThread; k++) {
-        if (outIdx[k] >= 0) out[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
+        for (int k = 0; k < kBinPerThread; k++) {
+            const int i = base + k * kBinThreads + threadIdx.x;
+            // empty lanes keep in-range fields (they issue no gathers)
+            rs[k].lab = -1;
+            rs[k].img = 0;
+            rs[k].xy = 0;
+            rs[k].rinv = 0.0;
+            outIdx[k] = -1;
+            if (i < bb.count) {
+                const int s = static_cast<int>(perm[bb.begin + i]);
+                const cf_sample x = samples[s];
+                const bool valid = x.depth_mm > 0;
+                outIdx[k] = s;
+                rs[k].rinv = valid ? depthReciprocal(depthToFloat(x.depth_mm)) : 0.0;
+                rs[k].img = static_cast<int>(x.image);
+                rs[k].xy = x.x | (x.y << 16);
+                rs[k].lab = (x.flip << 8) | ((valid ? 0 : 1) << 10);
+            }
+        }
+        uint32_t row[kBinPerThread][4];
+#pragma unroll
+        for (int k = 0; k < kBinPerThread; k++) row[k][0] = row[k][1] = row[k][2] = row[k][3] = 0u;
+
+        for (int fi = 0; fi < kBinGroup; fi++) {
+            const int pos = pos0 + fi;
+            if (pos >= F) {   // uniform: the last group of the table may be partial
+#pragma unroll
+                for (int k = 0; k < kBinPerThread; k++) pushBin(row[k], 0u);
+                continue;
+            }
+            const int f = eo[pos];
+            const size_t tf = static_cast<size_t>(bb.tree) * F + f;
+            const Feat fr = loadFeat(feat + tf);
+            float th[TM];
+            const float* tp = thrFT + tf * T;
+#pragma unroll
+            for (int t = 0; t < TM; t++) th[t] = (t < T) ? tp[t] : 0.0f;
+            if (fr.type == T_DEPTH) {
+                int2 cv[kBinPerThread][8];
+                bool ok[kBinPerThread];
+#pragma unroll
+                for (int k = 0; k < kBinPerThread; k++) {
+                    const int lab = rs[k].lab;
+                    int idx[8];
+                    ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                            responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
+                                    ((lab >> 8) & 3) == 1, idx);
+                    const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
+                    if (ok[k]) {   // lanes whose box leaves the image issue no gathers
+#pragma unroll
+                        for (int c = 0; c < 8; c++) cv[k][c] = dep[idx[c]];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kBinPerThread; k++) {
+                    const int b = ok[k] ? binOfT<TM>(roundUpToFloat(depthResponseFromCorners(cv[k])), th, T) : T;
+                    pushBin(row[k], static_cast<uint32_t>(b));
+                }
+            } else {
+                float cv[kBinPerThread][8];
+                bool ok[kBinPerThread];
+#pragma unroll
+                for (int k = 0; k < kBinPerThread; k++) {
+                    const int lab = rs[k].lab;
+                    int idx[8];
+                    ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                            responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
+                                    ((lab >> 8) & 3) == 1, idx);
+                    const float* colA = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c1) * plane;
+                    const float* colB = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c2) * plane;
+                    if (ok[k]) {
+#pragma unroll
+                        for (int c = 0; c < 4; c++) cv[k][c] = colA[idx[c]];
+#pragma unroll
+                        for (int c = 4; c < 8; c++) cv[k][c] = colB[idx[c]];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kBinPerThread; k++) {
+                    const int b = ok[k] ? binOfT<TM>(roundUpToFloat(colorResponseFromCorners(cv[k])), th, T) : T;
+                    pushBin(row[k], static_cast<uint32_t>(b));
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kBinPerThread; k++) {
+            if (outIdx[k] >= 0) out[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
+        }
     }
 }
 
@@ -192,7 +194,8 @@ __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bin
 #pragma unroll
         for (int fi = 0; fi < kBinGroup; fi++) {
             if (fi < nf) {
-                const int b = static_cast<int>((words[fi >> 2] >> (8 * (fi & 3))) & 0xffu);
+                // bins are 0..T; the clamp keeps the LDS index inside the histogram whatever a row holds
+                const int b = min(static_cast<int>((words[fi >> 2] >> (8 * (fi & 3))) & 0xffu), T);
                 atomicAdd(&h[fi * binsPerF + b * C + l], 1u);
             }
         }
@@ -240,6 +243,13 @@ __global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict_
     vals[begin + i] = static_cast<uint32_t>(begin + i);
 }
 
+// starts[key] = first position of key in the sorted keys (entries of absent keys are left alone)
+__global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__ keys, int n, int* __restrict__ starts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0 || keys[i] != keys[i - 1]) starts[keys[i]] = i;
+}
+
 __global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restrict__ samples, int n,
         uint8_t* __restrict__ labels) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -254,6 +264,11 @@ void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, 
         uint32_t* vals) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals);
+}
+
+void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_key_starts, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, starts);
 }
 
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels) {
@@ -275,16 +290,14 @@ void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t*
 }
 
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
-        const BinBlock* blocks, int numBlocks, int g0, int ng, const uint16_t* evalOrder, const FeatTables& tabs, int N,
-        uint4* bins) {
-    if (numBlocks == 0 || ng == 0) return;
+        const BinBlock* blocks, const int2* work, int numWork, int g0, const uint16_t* evalOrder, const FeatTables& tabs,
+        int N, uint4* bins) {
+    if (numWork == 0) return;
     if (tabs.T > kMaxBinThresholds) throw Error(CF_ERR_ARG, "too many thresholds for 8-bit bins");
-    const int supers = (numBlocks + kSuper - 1) / kSuper;
-    const int perXcd = (supers + kXcdCount - 1) / kXcdCount;
-    const dim3 grid(static_cast<unsigned>(perXcd) * kXcdCount * ng * kSuper), block(kBinThreads);
+    const dim3 grid(static_cast<unsigned>(numWork)), block(kBinThreads);
 #define CF_EVAL_BINS(TMV)                                                                                     \
     hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, 0, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
-            blocks, numBlocks, g0, ng, evalOrder, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, N, bins)
+            blocks, work, g0, evalOrder, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, N, bins)
     if (tabs.T <= 8) {
         CF_EVAL_BINS(8);
     } else if (tabs.T <= 16) {

@@ -14,6 +14,7 @@
 // either node by node (CF_MODE_PLAIN, the reference GPU path's shape: one
 // launch sequence and one host sync per node) or for every node of every
 // tree of the level together (CF_MODE_BATCHED).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -187,6 +188,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<uint32_t> dKeys, dKeys2, dVals, dPerm;
     DevBuf<uint8_t> dSortTemp, dLab8;
     DevBuf<BinBlock> dBlocks;
+    DevBuf<int2> dWork;
+    DevBuf<int> dStarts;
     DevBuf<uint16_t> dEvalOrder;
     DevBuf<uint4> dBins;
     const size_t binsBudget = binsBudgetBytes();
@@ -337,14 +340,52 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             dSortTemp.resize(std::max<size_t>(sortBytes, 1));
             radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), N, bits);
             launchLevelLabels(s, level[cur].get(), N, dLab8.get());
+            // (tree, image) segments of the sorted order -> blocks of <= 1024 samples within a segment
+            const int numKeys = static_cast<int>(keyRange);
+            dStarts.resize(numKeys);
+            CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, numKeys * sizeof(int), s));
+            launchKeyStarts(s, dKeys2.get(), N, dStarts.get());
+            std::vector<int> starts(numKeys);
+            CF_HIP(hipMemcpyAsync(starts.data(), dStarts.get(), numKeys * sizeof(int), hipMemcpyDeviceToHost, s));
+            CF_HIP(hipStreamSynchronize(s));
             std::vector<BinBlock> blocks;
-            for (int a = 0; a < numActive; a++) {
-                for (int p = 0; p < active[a].n; p += 1024) {
-                    blocks.push_back(BinBlock{a, active[a].begin + p, std::min(1024, active[a].n - p), 0});
+            std::vector<std::pair<int, int>> segBlocks;   // per segment: first block, block count
+            {
+                int k = 0;
+                while (k < numKeys && starts[k] < 0) k++;
+                while (k < numKeys) {
+                    int k2 = k + 1;
+                    while (k2 < numKeys && starts[k2] < 0) k2++;
+                    const int b = starts[k], e = k2 < numKeys ? starts[k2] : N;
+                    const int first = static_cast<int>(blocks.size());
+                    for (int p = b; p < e; p += 1024) {
+                        blocks.push_back(BinBlock{k / images->count, p, std::min(1024, e - p), 0});
+                    }
+                    segBlocks.emplace_back(first, static_cast<int>(blocks.size()) - first);
+                    k = k2;
                 }
             }
             dBlocks.resize(blocks.size());
             CF_HIP(hipMemcpyAsync(dBlocks.get(), blocks.data(), blocks.size() * sizeof(BinBlock), hipMemcpyHostToDevice, s));
+            // work list of a batch of ngb groups: segment i goes to XCD i % 8; each XCD takes its segments in
+            // order, all groups of a segment (group major, block minor) before the next; workgroup w runs
+            // on XCD w % 8, so the lists are interleaved
+            constexpr int kXcds = 8;
+            auto buildWork = [&](int ngb, std::vector<int2>& work) {
+                std::vector<size_t> len(kXcds, 0);
+                for (size_t i = 0; i < segBlocks.size(); i++) len[i % kXcds] += static_cast<size_t>(segBlocks[i].second) * ngb;
+                const size_t maxLen = *std::max_element(len.begin(), len.end());
+                work.assign(maxLen * kXcds, make_int2(-1, 0));
+                std::vector<size_t> pos(kXcds, 0);
+                for (size_t i = 0; i < segBlocks.size(); i++) {
+                    const int x = static_cast<int>(i % kXcds);
+                    for (int g = 0; g < ngb; g++) {
+                        for (int b = 0; b < segBlocks[i].second; b++) {
+                            work[pos[x]++ * kXcds + x] = make_int2(segBlocks[i].first + b, g);
+                        }
+                    }
+                }
+            };
             dEvalOrder.resize(static_cast<size_t>(numActive) * F);
             launchFeatureEvalOrder(s, tabs, numActive, dEvalOrder.get());
             // node histograms and trees of the whole level
@@ -367,11 +408,19 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             const size_t rowBytes = static_cast<size_t>(N) * sizeof(uint4);
             const int ng = static_cast<int>(std::max<size_t>(1, std::min<size_t>(groupsTotal, binsBudget / rowBytes)));
             dBins.resize(static_cast<size_t>(ng) * N);
+            std::vector<int2> work;
+            int workGroups = -1;
             for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
                 const int ngb = std::min(ng, groupsTotal - g0);
+                if (ngb != workGroups) {
+                    buildWork(ngb, work);
+                    workGroups = ngb;
+                    dWork.resize(work.size());
+                    CF_HIP(hipMemcpyAsync(dWork.get(), work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+                }
                 timer.begin();
-                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(),
-                        static_cast<int>(blocks.size()), g0, ngb, dEvalOrder.get(), tabs, N, dBins.get());
+                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(), dWork.get(),
+                        static_cast<int>(work.size()), g0, dEvalOrder.get(), tabs, N, dBins.get());
                 timer.end(stats.evalKernelMs);
                 stats.evalLaunches += 1;
                 const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
