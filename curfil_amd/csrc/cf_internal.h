@@ -193,8 +193,13 @@ void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t*
 void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts);
 // bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
-        const BinBlock* blocks, const int2* work, int numWork, int g0, const uint16_t* evalOrder, const FeatTables& tabs,
-        int N, uint4* bins);
+        const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
+        const FeatTables& tabs, int N, uint4* bins);
+// padded threshold width of the bin kernel for T thresholds
+int binThresholdWidth(int T);
+// features and +inf-padded thresholds in evaluation order: gFeat[trees * F], gThr[trees * F * binThresholdWidth(T)]
+void launchBinTables(hipStream_t s, const FeatTables& tabs, int trees, const uint16_t* evalOrder, FeatRec* gFeat,
+        float* gThr);
 // G[slot][ng * 16][T + 1][C] from the bin rows of the chunks' samples
 void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
         int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G);

@@ -37,13 +37,12 @@ struct BinSample {
     int lab;       // flip << 8 | (invalid depth) << 10;  < 0: empty slot
 };
 
+// #{t : th[t] < q} over a row padded with +inf (never below q), T for NaN.
 template<int TM>
 __device__ __forceinline__ int binOfT(float q, const float (&th)[TM], int T) {
     int b = 0;
 #pragma unroll
-    for (int t = 0; t < TM; t++) {
-        if (t < T) b += (th[t] < q) ? 1 : 0;
-    }
+    for (int t = 0; t < TM; t++) b += (th[t] < q) ? 1 : 0;
     return (q != q) ? T : b;
 }
 
@@ -62,8 +61,8 @@ template<int TM>
 __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restrict__ colBase,
         const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
         const uint32_t* __restrict__ perm, const BinBlock* __restrict__ blocks, const int2* __restrict__ work,
-        int g0, const uint16_t* __restrict__ evalOrder, const FeatRec* __restrict__ feat,
-        const float* __restrict__ thrFT, int F, int T, int N, uint4* __restrict__ bins) {
+        int g0, const FeatRec* __restrict__ gFeat, const float* __restrict__ gThr, int F, int T, int N,
+        uint4* __restrict__ bins) {
     // work[blockIdx] = (sample block, feature group within the batch); the
     // host interleaves the XCDs' lists (workgroup w runs on XCD w % 8) so each
     // XCD sweeps one (tree, image) segment for every group of the batch
@@ -75,7 +74,6 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
     const size_t plane = static_cast<size_t>(W) * H;
 
     const int pos0 = (g0 + g) * kBinGroup;
-    const uint16_t* eo = evalOrder + static_cast<size_t>(bb.tree) * F;
     uint4* out = bins + static_cast<size_t>(g) * N;
     for (int base = 0; base < bb.count; base += kBinThreads * kBinPerThread) {
         BinSample rs[kBinPerThread];
@@ -111,13 +109,12 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                 for (int k = 0; k < kBinPerThread; k++) pushBin(row[k], 0u);
                 continue;
             }
-            const int f = eo[pos];
-            const size_t tf = static_cast<size_t>(bb.tree) * F + f;
-            const Feat fr = loadFeat(feat + tf);
+            const size_t tp = static_cast<size_t>(bb.tree) * F + pos;
+            const Feat fr = loadFeat(gFeat + tp);
             float th[TM];
-            const float* tp = thrFT + tf * T;
+            const float* thp = gThr + tp * TM;
 #pragma unroll
-            for (int t = 0; t < TM; t++) th[t] = (t < T) ? tp[t] : 0.0f;
+            for (int t = 0; t < TM; t++) th[t] = thp[t];
             if (fr.type == T_DEPTH) {
                 int2 cv[kBinPerThread][8];
                 bool ok[kBinPerThread];
@@ -169,6 +166,21 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
         for (int k = 0; k < kBinPerThread; k++) {
             if (outIdx[k] >= 0) out[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
         }
+    }
+}
+
+// Feature records and thresholds in evaluation order, thresholds padded to
+// TM per feature with +inf: gFeat[tree][pos], gThr[tree][pos][TM].
+__global__ void __launch_bounds__(256) k_bin_tables(const FeatRec* __restrict__ feat, const float* __restrict__ thrFT,
+        const uint16_t* __restrict__ evalOrder, int total, int F, int T, int TM, FeatRec* __restrict__ gFeat,
+        float* __restrict__ gThr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int tree = i / F;
+    const size_t tf = static_cast<size_t>(tree) * F + evalOrder[i];
+    gFeat[i] = feat[tf];
+    for (int t = 0; t < TM; t++) {
+        gThr[static_cast<size_t>(i) * TM + t] = t < T ? thrFT[tf * T + t] : __builtin_inff();
     }
 }
 
@@ -256,6 +268,22 @@ __global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restric
     if (i < n) labels[i] = samples[i].label;
 }
 
+int binThresholdWidth(int T) {
+    static const int widths[] = {8, 12, 16, 20, 24, 32, 64, kMaxBinThresholds};
+    for (int w : widths) {
+        if (T <= w) return w;
+    }
+    throw Error(CF_ERR_ARG, "too many thresholds for 8-bit bins");
+}
+
+void launchBinTables(hipStream_t s, const FeatTables& tabs, int trees, const uint16_t* evalOrder, FeatRec* gFeat,
+        float* gThr) {
+    const int total = trees * tabs.F;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_bin_tables, dim3((total + 255) / 256), dim3(256), 0, s, tabs.feat.get(), tabs.thrFT.get(),
+            evalOrder, total, tabs.F, tabs.T, binThresholdWidth(tabs.T), gFeat, gThr);
+}
+
 void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder) {
     hipLaunchKernelGGL(k_feature_eval_order, dim3(trees), dim3(64), 0, s, tabs.feat.get(), tabs.F, evalOrder);
 }
@@ -290,26 +318,23 @@ void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t*
 }
 
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
-        const BinBlock* blocks, const int2* work, int numWork, int g0, const uint16_t* evalOrder, const FeatTables& tabs,
-        int N, uint4* bins) {
+        const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
+        const FeatTables& tabs, int N, uint4* bins) {
     if (numWork == 0) return;
-    if (tabs.T > kMaxBinThresholds) throw Error(CF_ERR_ARG, "too many thresholds for 8-bit bins");
+    const int TM = binThresholdWidth(tabs.T);
     const dim3 grid(static_cast<unsigned>(numWork)), block(kBinThreads);
 #define CF_EVAL_BINS(TMV)                                                                                     \
     hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, 0, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
-            blocks, work, g0, evalOrder, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, N, bins)
-    if (tabs.T <= 8) {
-        CF_EVAL_BINS(8);
-    } else if (tabs.T <= 16) {
-        CF_EVAL_BINS(16);
-    } else if (tabs.T <= 24) {
-        CF_EVAL_BINS(24);
-    } else if (tabs.T <= 32) {
-        CF_EVAL_BINS(32);
-    } else if (tabs.T <= 64) {
-        CF_EVAL_BINS(64);
-    } else {
-        CF_EVAL_BINS(kMaxBinThresholds);
+            blocks, work, g0, gFeat, gThr, tabs.F, tabs.T, N, bins)
+    switch (TM) {
+        case 8: CF_EVAL_BINS(8); break;
+        case 12: CF_EVAL_BINS(12); break;
+        case 16: CF_EVAL_BINS(16); break;
+        case 20: CF_EVAL_BINS(20); break;
+        case 24: CF_EVAL_BINS(24); break;
+        case 32: CF_EVAL_BINS(32); break;
+        case 64: CF_EVAL_BINS(64); break;
+        default: CF_EVAL_BINS(kMaxBinThresholds); break;
     }
 #undef CF_EVAL_BINS
 }

@@ -189,6 +189,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<uint8_t> dSortTemp, dLab8;
     DevBuf<BinBlock> dBlocks;
     DevBuf<int2> dWork;
+    DevBuf<FeatRec> dGFeat;
+    DevBuf<float> dGThr;
     DevBuf<int> dStarts;
     DevBuf<uint16_t> dEvalOrder;
     DevBuf<uint4> dBins;
@@ -388,6 +390,9 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             };
             dEvalOrder.resize(static_cast<size_t>(numActive) * F);
             launchFeatureEvalOrder(s, tabs, numActive, dEvalOrder.get());
+            dGFeat.resize(static_cast<size_t>(numActive) * F);
+            dGThr.resize(static_cast<size_t>(numActive) * F * binThresholdWidth(T));
+            launchBinTables(s, tabs, numActive, dEvalOrder.get(), dGFeat.get(), dGThr.get());
             // node histograms and trees of the whole level
             std::vector<uint32_t> nh(static_cast<size_t>(numNodes) * C);
             std::vector<int> st(numNodes);
@@ -420,7 +425,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 }
                 timer.begin();
                 launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(), dWork.get(),
-                        static_cast<int>(work.size()), g0, dEvalOrder.get(), tabs, N, dBins.get());
+                        static_cast<int>(work.size()), g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get());
                 timer.end(stats.evalKernelMs);
                 stats.evalLaunches += 1;
                 const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
