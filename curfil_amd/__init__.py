@@ -70,6 +70,7 @@ def _load():
         "cf_feature_responses": [P, P, P, I, P, I, P],
         "cf_node_counters": [P, P, P, I, P, P, I, I, I, P],
         "cf_scores": [P, P, P, I, I, I, P],
+        "cf_argmax_scores": [P, P, I, I, P, P, P],
         "cf_node_best_split": [P, P, P, I, P, P, I, I, I, P, P, P, P],
         "cf_forest_train": [P, P, P, I, I, I, ctypes.POINTER(P)],
         "cf_forest_sample": [P, P, P, I, I, ctypes.POINTER(P)], "cf_forest_grow": [P, P, P, P, I],
@@ -235,6 +236,16 @@ def scores(ctx, counters, histogram):
     out = np.zeros((T, F), np.float64)
     ctx.check(lib.cf_scores(ctx.h, _ptr(counters), _ptr(histogram), F, T, C, _ptr(out)))
     return out
+
+
+def argmax_scores(ctx, scores):
+    """The reference's threshold-major scan with the 1e-13 tie rule over scores [T][F]:
+    (best feature, best threshold, best score)."""
+    scores = np.ascontiguousarray(scores, np.float64)
+    T, F = scores.shape
+    bf, bt, bs = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    ctx.check(lib.cf_argmax_scores(ctx.h, _ptr(scores), F, T, ctypes.byref(bf), ctypes.byref(bt), ctypes.byref(bs)))
+    return bf.value, bt.value, bs.value
 
 
 def node_best_split(ctx, images, samples, features, thresholds, num_classes, histogram):

@@ -414,6 +414,24 @@ int cf_scores(cf_ctx* ctx, const uint32_t* counters, const uint32_t* histogram, 
     });
 }
 
+int cf_argmax_scores(cf_ctx* ctx, const double* scores, int F, int T, int* bestFeature, int* bestThreshold,
+        double* bestScore) {
+    return guarded(ctx, [&] {
+        require(scores && bestFeature && bestThreshold && bestScore, "null argument");
+        require(F > 0 && T > 0, "sizes must be positive");
+        DevBuf<double> dS(static_cast<size_t>(F) * T);
+        DevBuf<NodeBest> dB(1);
+        CF_HIP(hipMemcpyAsync(dS.get(), scores, dS.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        launchSequentialArgmax(ctx->stream, dS.get(), T, F, dB.get());
+        NodeBest nb;
+        CF_HIP(hipMemcpyAsync(&nb, dB.get(), sizeof(NodeBest), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        *bestFeature = nb.feat;
+        *bestThreshold = nb.thresh;
+        *bestScore = nb.score;
+    });
+}
+
 int cf_forest_sample(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int numTrees, int numLabels,
         cf_forest** out) {
     return guarded(ctx, [&] {

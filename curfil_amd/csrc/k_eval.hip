@@ -382,34 +382,55 @@ __global__ void __launch_bounds__(64) k_reduce(const FeatSummary* __restrict__ s
 }
 
 // The literal scan of random_tree_image.cpp:445-457 for one node.
-__global__ void k_seq_argmax(const double* __restrict__ scores, int T, int F, NodeBest* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double bestScore = -__builtin_inf();
-    int bestT = 0, bestF = 0;
-    for (int t = 0; t < T; t++) {
-        for (int f = 0; f < F; f++) {
-            const double sc = scores[static_cast<size_t>(t) * F + f];
-            bool better;
-            if (__builtin_isnan(bestScore)) {
-                better = true;
-            } else {
-                const double diff = fabs(sc - bestScore);
-                better = !(diff < 1e-13) && (sc > bestScore);
-            }
-            if (better) {
-                bestF = f;
-                bestT = t;
-                bestScore = sc;
+// detail::isScoreBetter (random_tree.cpp:36-52) as used by the scan.
+__device__ __forceinline__ bool scoreBetter(double best, double score) {
+    if (__builtin_isnan(best)) return true;
+    return !(fabs(score - best) < 1e-13) && (score > best);
+}
+
+// The scan of random_tree_image.cpp:445-457 for one node, one wave: scores
+// are read 64 at a time in scan order; a ballot marks the lanes whose score
+// beats the current best, and only those are resolved in order with the
+// reference's predicate.  A score that does not beat the best cannot beat any
+// later (larger) best: fl(score - best) does not grow as best grows.  The
+// result is the sequential scan's.
+__global__ void __launch_bounds__(64) k_seq_argmax(const double* __restrict__ scores, int T, int F,
+        NodeBest* __restrict__ out) {
+    constexpr int kWin = 8;   // windows of 64 scores read per round
+    const int lane = threadIdx.x;
+    const long long n = static_cast<long long>(T) * F;
+    double best = -__builtin_inf();
+    long long pos = 0;
+    for (long long base = 0; base < n; base += 64 * kWin) {
+        double sc[kWin];
+#pragma unroll
+        for (int u = 0; u < kWin; u++) {
+            const long long i = base + u * 64 + lane;
+            sc[u] = i < n ? scores[i] : -__builtin_inf();
+        }
+#pragma unroll
+        for (int u = 0; u < kWin; u++) {
+            unsigned long long m = __ballot(scoreBetter(best, sc[u]));
+            while (m) {
+                const int l = __ffsll(static_cast<long long>(m)) - 1;
+                const double c = __shfl(sc[u], l, 64);
+                if (scoreBetter(best, c)) {
+                    best = c;
+                    pos = base + u * 64 + l;
+                }
+                m &= m - 1;
             }
         }
     }
-    NodeBest nb;
-    nb.score = bestScore;
-    nb.feat = bestF;
-    nb.thresh = bestT;
-    nb.flag = 0;
-    nb.pad = 0;
-    *out = nb;
+    if (lane == 0) {
+        NodeBest nb;
+        nb.score = best;
+        nb.feat = static_cast<int>(pos % F);
+        nb.thresh = static_cast<int>(pos / F);
+        nb.flag = 0;
+        nb.pad = 0;
+        *out = nb;
+    }
 }
 
 // counters[F][T][C][2] of one node (slot 0) from its bins.

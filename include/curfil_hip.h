@@ -140,6 +140,12 @@ int cf_node_counters(cf_ctx* ctx, cf_images* images, const cf_sample* samples, i
 int cf_scores(cf_ctx* ctx, const uint32_t* counters, const uint32_t* histogram, int num_features,
         int num_thresholds, int num_classes, double* scores);
 
+/* the reference's host argmax over one node's scores [T][F]: threshold-major
+ * sequential scan with detail::isScoreBetter's 1e-13 rule
+ * (random_tree_image.cpp:445-457, random_tree.cpp:36-52), on the device. */
+int cf_argmax_scores(cf_ctx* ctx, const double* scores, int num_features, int num_thresholds, int* best_feature,
+        int* best_threshold, double* best_score);
+
 /* best split of one node (plain path): counters, scores and the reference's
  * sequential argmax with its 1e-13 tie rule (random_tree_image.cpp:445-457,
  * random_tree.cpp:36-52), all on the device. */

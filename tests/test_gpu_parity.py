@@ -202,6 +202,31 @@ def test_node_best_split_matches_oracle(ctx, rand_data):
 
 
 # ------------------------------------------------------------------ feature generation
+def _adversarial_score_matrices():
+    rng = np.random.default_rng(11)
+    yield "random", rng.random((20, 2000))
+    yield "increasing", np.linspace(0.0, 1.0, 20 * 2000).reshape(20, 2000)   # every score is a new best
+    yield "decreasing", np.linspace(1.0, 0.0, 10 * 300).reshape(10, 300)
+    m = rng.random((16, 700)) * 0.5
+    m[3, 5] = 0.9
+    m[7, 600] = 0.9 + 5e-14     # within 1e-13 of the best: must not replace it
+    m[9, 10] = 0.9 + 2e-13      # beyond the band: replaces
+    m[12, 1] = 0.9 + 2e-13      # equal to the best: must not replace it
+    yield "band", m
+    steps = np.full((8, 129), 0.25)
+    steps.flat[::7] += np.arange(steps.flat[::7].size) * 6e-14   # a staircase of sub-band increments
+    yield "staircase", steps
+    yield "constant", np.full((5, 64), 0.5)
+    yield "single", np.array([[0.3]])
+
+
+def test_argmax_scores_matches_sequential_scan(ctx):
+    for name, m in _adversarial_score_matrices():
+        got = cf.argmax_scores(ctx, m)
+        want = orc.argmax(m)
+        assert got[0] == want[0] and got[1] == want[1] and got[2] == want[2], (name, got, want)
+
+
 def test_golden_feature_hash_on_device(ctx):
     color, depth, samples = golden_hash_dataset()
     im = _images_f32(ctx, color, depth)
