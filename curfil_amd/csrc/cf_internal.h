@@ -181,7 +181,7 @@ struct BinBlock {
     int tree;
     int begin;   // first position in the permutation
     int count;
-    int pad;
+    int image;   // the image of every sample in the block
 };
 void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder);
 void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,

@@ -32,7 +32,6 @@ constexpr int kBinPerThread = 4;
 
 struct BinSample {
     double rinv;   // depthReciprocal(depth in metres)
-    int img;
     int xy;        // x | y << 16
     int lab;       // flip << 8 | (invalid depth) << 10;  < 0: empty slot
 };
@@ -44,6 +43,14 @@ __device__ __forceinline__ int binOfT(float q, const float (&th)[TM], int T) {
 #pragma unroll
     for (int t = 0; t < TM; t++) b += (th[t] < q) ? 1 : 0;
     return (q != q) ? T : b;
+}
+
+// Element i of an image plane through a 32-bit byte offset (a plane is far
+// below 4 GiB), so the load can take the uniform base in scalar registers.
+template<typename V>
+__device__ __forceinline__ V planeAt(const V* base, int i) {
+    const uint32_t off = static_cast<uint32_t>(i) * static_cast<uint32_t>(sizeof(V));
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + off);
 }
 
 // Shifts a sample's 16-byte bin row right by one byte and inserts b as the
@@ -72,6 +79,9 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
     const int g = wk.y;
     const BinBlock bb = blocks[wk.x];
     const size_t plane = static_cast<size_t>(W) * H;
+    // all samples of a block belong to image bb.image: the plane bases are uniform
+    const int2* dep = depBase + static_cast<size_t>(bb.image) * plane;
+    const float* col = colBase + static_cast<size_t>(bb.image) * 3 * plane;
 
     const int pos0 = (g0 + g) * kBinGroup;
     uint4* out = bins + static_cast<size_t>(g) * N;
@@ -83,7 +93,6 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
             const int i = base + k * kBinThreads + threadIdx.x;
             // empty lanes keep in-range fields (they issue no gathers)
             rs[k].lab = -1;
-            rs[k].img = 0;
             rs[k].xy = 0;
             rs[k].rinv = 0.0;
             outIdx[k] = -1;
@@ -93,7 +102,6 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                 const bool valid = x.depth_mm > 0;
                 outIdx[k] = s;
                 rs[k].rinv = valid ? depthReciprocal(depthToFloat(x.depth_mm)) : 0.0;
-                rs[k].img = static_cast<int>(x.image);
                 rs[k].xy = x.x | (x.y << 16);
                 rs[k].lab = (x.flip << 8) | ((valid ? 0 : 1) << 10);
             }
@@ -125,10 +133,9 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                     ok[k] = lab >= 0 && (lab >> 10) == 0 &&
                             responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
                                     ((lab >> 8) & 3) == 1, idx);
-                    const int2* dep = depBase + static_cast<size_t>(rs[k].img) * plane;
                     if (ok[k]) {   // lanes whose box leaves the image issue no gathers
 #pragma unroll
-                        for (int c = 0; c < 8; c++) cv[k][c] = dep[idx[c]];
+                        for (int c = 0; c < 8; c++) cv[k][c] = planeAt(dep, idx[c]);
                     }
                 }
 #pragma unroll
@@ -146,13 +153,13 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                     ok[k] = lab >= 0 && (lab >> 10) == 0 &&
                             responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
                                     ((lab >> 8) & 3) == 1, idx);
-                    const float* colA = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c1) * plane;
-                    const float* colB = colBase + (static_cast<size_t>(rs[k].img) * 3 + fr.c2) * plane;
+                    const float* colA = col + static_cast<size_t>(fr.c1) * plane;
+                    const float* colB = col + static_cast<size_t>(fr.c2) * plane;
                     if (ok[k]) {
 #pragma unroll
-                        for (int c = 0; c < 4; c++) cv[k][c] = colA[idx[c]];
+                        for (int c = 0; c < 4; c++) cv[k][c] = planeAt(colA, idx[c]);
 #pragma unroll
-                        for (int c = 4; c < 8; c++) cv[k][c] = colB[idx[c]];
+                        for (int c = 4; c < 8; c++) cv[k][c] = planeAt(colB, idx[c]);
                     }
                 }
 #pragma unroll

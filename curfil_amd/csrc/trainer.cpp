@@ -361,7 +361,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     const int b = starts[k], e = k2 < numKeys ? starts[k2] : N;
                     const int first = static_cast<int>(blocks.size());
                     for (int p = b; p < e; p += 1024) {
-                        blocks.push_back(BinBlock{k / images->count, p, std::min(1024, e - p), 0});
+                        blocks.push_back(BinBlock{k / images->count, p, std::min(1024, e - p), k % images->count});
                     }
                     segBlocks.emplace_back(first, static_cast<int>(blocks.size()) - first);
                     k = k2;
