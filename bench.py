@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=800)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -126,7 +126,7 @@ def main():
     responses = sum(p["responses"] for p in profiles)
     achieved = responses * BYTES_PER_RESPONSE / eval_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(N, W, H),
                 "kernel": "k_eval_bins", "launches": int(sum(p["eval_launches"] for p in profiles)),
                 "avg_launch_ms": round(sum(p["eval_kernel_ms"] for p in profiles) /
                                        max(1, sum(p["eval_launches"] for p in profiles)), 4),
@@ -166,6 +166,21 @@ def main():
         "predict": predict,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(N, W, H):
+    """HBM bytes per k_eval_bins launch from the committed rocprofv3 counter
+    passes (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a copy kernel of known size), for this workload only."""
+    path = os.path.join(ROOT, "profiles", "k_eval_bins_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload", {}).get("frames") != N or t["workload"].get("width") != W or t["workload"].get("height") != H:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):
