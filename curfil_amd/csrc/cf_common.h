@@ -187,6 +187,14 @@ __device__ __forceinline__ bool responseCorners(const Feat& f, int px, int py, i
     return a && b;
 }
 
+// Element i of an image plane through a 32-bit byte offset (a plane is far
+// below 4 GiB), so the load can take the uniform base in scalar registers.
+template<typename V>
+__device__ __forceinline__ V planeAt(const V* base, int i) {
+    const uint32_t off = static_cast<uint32_t>(i) * static_cast<uint32_t>(sizeof(V));
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + off);
+}
+
 // averageRegionColor of both boxes and their difference from gathered corners.
 __device__ __forceinline__ double colorResponseFromCorners(const float* c) {
     const double a = (static_cast<double>(c[0]) - static_cast<double>(c[2])) +

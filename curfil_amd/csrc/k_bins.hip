@@ -45,14 +45,6 @@ __device__ __forceinline__ int binOfT(float q, const float (&th)[TM], int T) {
     return (q != q) ? T : b;
 }
 
-// Element i of an image plane through a 32-bit byte offset (a plane is far
-// below 4 GiB), so the load can take the uniform base in scalar registers.
-template<typename V>
-__device__ __forceinline__ V planeAt(const V* base, int i) {
-    const uint32_t off = static_cast<uint32_t>(i) * static_cast<uint32_t>(sizeof(V));
-    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + off);
-}
-
 // Shifts a sample's 16-byte bin row right by one byte and inserts b as the
 // last byte: after 16 inserts the first inserted bin is byte 0.
 __device__ __forceinline__ void pushBin(uint32_t (&w)[4], uint32_t b) {
@@ -192,7 +184,11 @@ __global__ void __launch_bounds__(256) k_bin_tables(const FeatRec* __restrict__ 
 }
 
 // Pass 2: interval histograms of (node chunk, feature group) from the bin rows.
-// G holds the batch's groups: G[slot][ng * 16][T + 1][C].
+// G holds the batch's groups: G[slot][ng * 16][T + 1][C].  Lane l counts
+// feature l % 16 of sample l / 16, so one LDS atomic instruction covers 4
+// samples x 16 features: the 16 features' histograms lie in different banks
+// and at most 4 lanes can hit one address (all lanes counting one feature, as
+// for samples of an almost pure node, would collide on few bins).
 __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bins, const uint8_t* __restrict__ labels,
         const Chunk* __restrict__ chunks, int g0, int ng, int F, int T, int C, int N, uint32_t* __restrict__ G) {
     extern __shared__ uint32_t h[];
@@ -202,26 +198,36 @@ __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bin
     const int nf = min(kBinGroup, F - (g0 + g) * kBinGroup);
     const int binsPerF = (T + 1) * C;
     const int nb = nf * binsPerF;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) h[k] = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) h[k] = 0;
     __syncthreads();
-    const uint4* row = bins + static_cast<size_t>(g) * N + ch.begin;
+    const int fi = threadIdx.x & (kBinGroup - 1);
+    const int sub = threadIdx.x >> 4;   // 16 samples per pass of the workgroup
+    const uint8_t* row = reinterpret_cast<const uint8_t*>(bins + static_cast<size_t>(g) * N + ch.begin);
     const uint8_t* lab = labels + ch.begin;
-    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
-        const uint4 w = row[i];
-        const int l = lab[i];
-        const uint32_t words[4] = {w.x, w.y, w.z, w.w};
+    uint32_t* hf = h + fi * binsPerF;
+    if (fi < nf) {
+        constexpr int kUnroll = 8;
+        for (int base = 0; base < ch.count; base += 16 * kUnroll) {
+            int b[kUnroll], l[kUnroll];
 #pragma unroll
-        for (int fi = 0; fi < kBinGroup; fi++) {
-            if (fi < nf) {
+            for (int u = 0; u < kUnroll; u++) {
+                const int i = base + u * 16 + sub;
+                l[u] = -1;
+                if (i < ch.count) {
+                    b[u] = row[static_cast<size_t>(i) * kBinGroup + fi];
+                    l[u] = lab[i];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
                 // bins are 0..T; the clamp keeps the LDS index inside the histogram whatever a row holds
-                const int b = min(static_cast<int>((words[fi >> 2] >> (8 * (fi & 3))) & 0xffu), T);
-                atomicAdd(&h[fi * binsPerF + b * C + l], 1u);
+                if (l[u] >= 0) atomicAdd(&hf[min(b[u], T) * C + l[u]], 1u);
             }
         }
     }
     __syncthreads();
     uint32_t* out = G + (static_cast<size_t>(ch.slot) * ng + g) * kBinGroup * binsPerF;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nb; k += 256) {
         const uint32_t v = h[k];
         if (v) atomicAdd(&out[k], v);
     }

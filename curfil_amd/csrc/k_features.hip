@@ -12,7 +12,7 @@
 // order, and how many draws a candidate consumes depends on which responses
 // are NaN; k_fg_walk keeps that order exactly: one workgroup per (tree, level)
 // owns the sample-index generator (MT19937 state twisted in parallel in LDS),
-// evaluates the next 256 queued draws for the current candidate in parallel
+// evaluates the next 8 candidates at the next 256 queued draws in parallel
 // and lets one lane walk them with the reference's retry rules.
 #include "cf_internal.h"
 #include "cf_rng.h"
@@ -99,13 +99,28 @@ __device__ __forceinline__ int featureSortKey(const FeatRec& f) {
     return k;
 }
 
+constexpr int kWalkCand = 8;    // candidates evaluated per round
+constexpr int kWalkDraws = 256; // queued draws per round (one per thread)
+
+// What the walk did with candidate j of a round: window positions
+// [begin, end) were consumed with st = 0 (still open at the window's end),
+// 1 (accepted into output slot `slot`) or 2 (rejected); t0 thresholds had
+// been found before the round.
+struct WalkStep {
+    int st, begin, end, t0, slot;
+};
+
 struct WalkShared {
     uint32_t mt[624];
     int dq[kDrawQueue];
-    float resp[256];
-    float thr[kMaxThresholds];
+    float resp[kWalkCand][kWalkDraws];
+    unsigned long long finite[kWalkCand][kWalkDraws / 64];   // bit: response is not NaN
+    FeatRec cand[kWalkCand];
+    WalkStep step[kWalkCand];
     int waveCounts[4];
-    int status, consumed;
+    // walk state after a round: next draw, next candidate, features generated,
+    // thresholds found and tries left for a candidate that ran past the round
+    int head, k, generated, t, tries, err;
 };
 
 // One MT19937 twist of the LDS state, in three dependency-free phases.
@@ -162,6 +177,15 @@ __device__ int refillDraws(WalkShared& sh, int tail, const UniformIntParams& up)
     return tail;
 }
 
+// Per round: the responses (as float) of the next kWalkCand candidates at the
+// next kWalkDraws queued draws, one draw per thread; then lane 0 walks the
+// candidates in order through those draws with the reference's rules
+// (random_tree_image.cpp:648-700: a NaN response costs one of 10 tries, a
+// finite one is a threshold and resets the tries; T thresholds accept the
+// candidate, 10 NaN in a row reject it).  A candidate's first draw is the one
+// after its predecessor's last, so every candidate of the round is evaluated
+// at every draw of the window; a candidate still open at the end of the
+// window carries its state into the next round, which starts with it.
 __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ jobs, const DevImage* __restrict__ imgs,
         const FeatRec* __restrict__ cand, int numCand, int F, int T, FeatRec* __restrict__ outFeat,
         float* __restrict__ outThr, int* __restrict__ outKey, int* __restrict__ err) {
@@ -177,61 +201,171 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
             v = mtInitStep(v, i);
             sh.mt[i] = v;
         }
+        sh.head = 0;
+        sh.k = 0;
+        sh.generated = 0;
+        sh.t = 0;
+        sh.tries = 10;
+        sh.err = 0;
     }
     __syncthreads();
-    int tail = 0, head = 0, generated = 0;
-    for (int k = 0; generated < F; k++) {
-        // `--maxLoops <= 0` with maxLoops = 3F, checked right after sampling candidate k
-        if (k + 1 >= 3 * F) {
-            if (tid == 0) atomicExch(err, 1);
-            break;
-        }
-        const FeatRec fr = myCand[k];
-        const Feat ff = loadFeat(myCand + k);
-        int pos = head;
-        int t = 0, tries = 10;  // meaningful in lane 0 only
-        int status = 0;
-        for (;;) {
-            while (tail - pos < 256) tail = refillDraws(sh, tail, up);
-            {
-                const int d = sh.dq[(pos + tid) % kDrawQueue];
-                const cf_sample s = job.samples[d];
-                const double r = featureResponse(imgs[s.image], ff, s.x, s.y, s.depth_mm, false);
-                sh.resp[tid] = static_cast<float>(r);
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int consumed = 256, st = 0;
-                for (int i = 0; i < 256; i++) {
-                    const float v = sh.resp[i];
-                    if (!__builtin_isnan(v)) {
-                        sh.thr[t++] = v;
-                        tries = 10;
-                        if (t == T) { st = 1; consumed = i + 1; break; }
-                    } else if (--tries == 0) {
-                        st = 2;
-                        consumed = i + 1;
-                        break;
+    int tail = 0;
+    for (;;) {
+        const int head = sh.head, k0 = sh.k;
+        const bool done = sh.generated >= F || sh.err != 0;
+        __syncthreads();
+        if (done) break;
+        while (tail - head < kWalkDraws) tail = refillDraws(sh, tail, up);
+        {
+            // all candidates' corner gathers are issued before any response is formed
+            const int d = sh.dq[(head + tid) % kDrawQueue];
+            const cf_sample smp = job.samples[d];
+            const DevImage im = imgs[smp.image];
+            const bool valid = smp.depth_mm > 0;
+            const double rinv = valid ? depthReciprocal(depthToFloat(smp.depth_mm)) : 0.0;
+            const size_t plane = static_cast<size_t>(im.W) * im.H;
+            bool ok[kWalkCand];
+            bool isDepth[kWalkCand];
+            int2 cv[kWalkCand][8];   // depth corners, or colour corners' bits in .x
+#pragma unroll
+            for (int j = 0; j < kWalkCand; j++) {
+                ok[j] = false;
+                isDepth[j] = false;
+                if (k0 + j < numCand) {
+                    const Feat ff = loadFeat(myCand + k0 + j);
+                    int idx[8];
+                    isDepth[j] = ff.type == T_DEPTH;
+                    ok[j] = valid && responseCorners(ff, smp.x, smp.y, im.W, im.H, rinv, false, idx);
+                    if (ok[j]) {
+                        if (isDepth[j]) {
+#pragma unroll
+                            for (int c = 0; c < 8; c++) cv[j][c] = planeAt(im.dep, idx[c]);
+                        } else {
+                            const float* colA = im.col + static_cast<size_t>(ff.c1) * plane;
+                            const float* colB = im.col + static_cast<size_t>(ff.c2) * plane;
+#pragma unroll
+                            for (int c = 0; c < 4; c++) cv[j][c].x = __float_as_int(planeAt(colA, idx[c]));
+#pragma unroll
+                            for (int c = 4; c < 8; c++) cv[j][c].x = __float_as_int(planeAt(colB, idx[c]));
+                        }
                     }
                 }
-                sh.status = st;
-                sh.consumed = consumed;
             }
-            __syncthreads();
-            status = sh.status;
-            pos += sh.consumed;
-            __syncthreads();
-            if (status != 0) break;
+#pragma unroll
+            for (int j = 0; j < kWalkCand; j++) {
+                float v = __builtin_nanf("");
+                if (ok[j]) {
+                    if (isDepth[j]) {
+                        v = static_cast<float>(depthResponseFromCorners(cv[j]));
+                    } else {
+                        float c8[8];
+#pragma unroll
+                        for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[j][c].x);
+                        v = static_cast<float>(colorResponseFromCorners(c8));
+                    }
+                }
+                sh.resp[j][tid] = v;
+                const unsigned long long fm = __ballot(!__builtin_isnan(v));
+                if ((tid & 63) == 0) sh.finite[j][tid >> 6] = fm;
+            }
+            if (tid < kWalkCand && k0 + tid < numCand) sh.cand[tid] = myCand[k0 + tid];
         }
-        head = pos;
-        if (status == 1) {
-            const int slot = job.outOffset + generated;
-            for (int i = tid; i < T; i += 256) outThr[static_cast<size_t>(slot) * T + i] = sh.thr[i];
-            if (tid == 0) {
-                outFeat[slot] = fr;
-                outKey[slot] = featureSortKey(fr);
+        __syncthreads();
+        if (tid < 64) {
+            // wave 0 walks the finite-response bits with wave-uniform values (scalar
+            // unit): a NaN run costs one try per draw, a finite response is the next
+            // threshold and resets the tries
+            int p = 0;
+            int k = k0;
+            int gen = __builtin_amdgcn_readfirstlane(sh.generated);
+            int t = __builtin_amdgcn_readfirstlane(sh.t);
+            int tries = __builtin_amdgcn_readfirstlane(sh.tries);
+            if (tid < kWalkCand) sh.step[tid].st = 2;   // not walked: nothing to copy
+            while (k < k0 + kWalkCand && gen < F) {
+                // `--maxLoops <= 0` with maxLoops = 3F, checked right after sampling candidate k
+                if (k + 1 >= 3 * F) {
+                    if (tid == 0) {
+                        sh.err = 1;
+                        atomicExch(err, 1);
+                    }
+                    break;
+                }
+                const int j = k - k0;
+                const int begin = p, t0 = t, slot = job.outOffset + gen;
+                int st = 0;
+                while (st == 0 && p < kWalkDraws) {
+                    const unsigned long long word = sh.finite[j][p >> 6];
+                    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(word));
+                    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(word >> 32));
+                    unsigned long long m = ((static_cast<unsigned long long>(hi) << 32) | lo) >> (p & 63);
+                    int rem = 64 - (p & 63);
+                    while (rem > 0) {
+                        const int z = m ? min(__builtin_ctzll(m), rem) : rem;   // NaN draws before the next finite
+                        if (z >= tries) {
+                            p += tries;
+                            st = 2;
+                            break;
+                        }
+                        tries -= z;
+                        p += z;
+                        rem -= z;
+                        if (rem == 0) break;
+                        m >>= z;
+                        t++;               // the finite draw at p
+                        tries = 10;
+                        p++;
+                        rem--;
+                        m >>= 1;
+                        if (t == T) {
+                            st = 1;
+                            break;
+                        }
+                    }
+                }
+                if (tid == 0) {
+                    WalkStep ws;
+                    ws.st = st;
+                    ws.begin = begin;
+                    ws.end = p;
+                    ws.t0 = t0;
+                    ws.slot = slot;
+                    sh.step[j] = ws;
+                    if (st == 1) {
+                        const FeatRec fr = sh.cand[j];
+                        outFeat[slot] = fr;
+                        outKey[slot] = featureSortKey(fr);
+                    }
+                }
+                if (st == 0) break;   // the window ran out inside candidate k
+                if (st == 1) gen++;
+                k++;
+                t = 0;
+                tries = 10;
             }
-            generated++;
+            if (tid == 0) {
+                sh.head = head + p;
+                sh.k = k;
+                sh.generated = gen;
+                sh.t = t;
+                sh.tries = tries;
+            }
+        }
+        __syncthreads();
+        // thresholds of the accepted and still open candidates: the finite
+        // draws of [begin, end) in order, after the t0 found earlier
+#pragma unroll
+        for (int j = 0; j < kWalkCand; j++) {
+            const WalkStep ws = sh.step[j];
+            if (ws.st == 2 || tid < ws.begin || tid >= ws.end) continue;
+            if (!((sh.finite[j][tid >> 6] >> (tid & 63)) & 1ull)) continue;
+            int before = 0;   // finite draws in [begin, tid)
+            for (int w = ws.begin >> 6; w <= (tid >> 6); w++) {
+                unsigned long long m = sh.finite[j][w];
+                if (w == (ws.begin >> 6)) m &= ~0ull << (ws.begin & 63);
+                if (w == (tid >> 6)) m &= (1ull << (tid & 63)) - 1ull;
+                before += __popcll(m);
+            }
+            outThr[static_cast<size_t>(ws.slot) * T + ws.t0 + before] = sh.resp[j][tid];
         }
         __syncthreads();
     }
