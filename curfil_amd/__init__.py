@@ -68,6 +68,7 @@ def _load():
         "cf_images_set_labels": [P, I, P], "cf_images_get_integrals": [P, I, P, P, P],
         "cf_generate_features": [P, P, P, I, I, I, I, I, I, I, P, P],
         "cf_feature_responses": [P, P, P, I, P, I, P],
+        "cf_feature_responses_paired": [P, P, P, I, P, P],
         "cf_node_counters": [P, P, P, I, P, P, I, I, I, P],
         "cf_scores": [P, P, P, I, I, I, P],
         "cf_argmax_scores": [P, P, I, I, P, P, P],
@@ -84,6 +85,10 @@ def _load():
         "cf_forest_get_posteriors": [P, I, P],
         "cf_predict": [P, P, P, I, P, P], "cf_predict_batch": [P, P, P, P, I, P, P],
         "cf_synth_rgbd": [U32, U32, I, I, I, I, D, P, P, P],
+        "cf_server_create": [P, P, D, I, ctypes.POINTER(P)],
+        "cf_server_submit": [P, P, P, I, I, ctypes.POINTER(ctypes.c_uint64)],
+        "cf_server_wait": [P, ctypes.c_uint64, P, P, ctypes.POINTER(D)],
+        "cf_server_stats": [P, P, I], "cf_server_destroy": [P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -91,6 +96,8 @@ def _load():
         fn.restype = ctypes.c_int
     lib.cf_last_error.argtypes = [P]
     lib.cf_last_error.restype = ctypes.c_char_p
+    lib.cf_server_error.argtypes = [P]
+    lib.cf_server_error.restype = ctypes.c_char_p
     return lib
 
 
@@ -129,7 +136,9 @@ class Context:
 
     def close(self):
         if self.h:
-            for child in list(self._children):
+            # servers first: they use forests and images of this context
+            children = sorted(self._children, key=lambda c: 0 if isinstance(c, Server) else 1)
+            for child in children:
                 child.close()
             lib.cf_ctx_destroy(self.h)
             self.h = None
@@ -442,6 +451,120 @@ def synth_rgbd(seed, width, height, sites, classes, zero_frac, class_seed=4711):
     if st != CF_OK:
         raise CurfilError(st, "cf_synth_rgbd failed")
     return rgb, depth, labels
+
+
+class Server:
+    """Serving loop over a resident forest (cf_server_*): submit frames as they arrive, wait for results."""
+
+    def __init__(self, ctx, forest, budget_mb=8192.0, max_batch=64):
+        self.ctx, self.forest = ctx, forest
+        self.C = forest.num_classes
+        h = ctypes.c_void_p()
+        ctx.check(lib.cf_server_create(ctx.h, forest.h, float(budget_mb), max_batch, ctypes.byref(h)))
+        self.h = h
+        self.sizes = {}
+        ctx._adopt(self)
+
+    def submit(self, rgb, depth):
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        depth = np.ascontiguousarray(depth, np.uint16)
+        H, W = depth.shape
+        rid = ctypes.c_uint64()
+        st = lib.cf_server_submit(self.h, _ptr(rgb), _ptr(depth), W, H, ctypes.byref(rid))
+        if st != CF_OK:
+            raise CurfilError(st, "cf_server_submit failed" + (": " + lib.cf_server_error(self.h).decode()))
+        self.sizes[rid.value] = (H, W)
+        return rid.value
+
+    def wait(self, rid, probabilities=True, labels=True):
+        H, W = self.sizes.pop(rid)
+        probs = np.zeros((self.C, H, W), np.float32) if probabilities else None
+        lab = np.zeros((H, W), np.uint8) if labels else None
+        lat = ctypes.c_double()
+        st = lib.cf_server_wait(self.h, rid, _ptr(probs), _ptr(lab), ctypes.byref(lat))
+        if st != CF_OK:
+            raise CurfilError(st, "cf_server_wait failed: " + lib.cf_server_error(self.h).decode())
+        return lab, probs, lat.value
+
+    def stats(self):
+        out = np.zeros(5, np.float64)
+        lib.cf_server_stats(self.h, _ptr(out), 5)
+        return dict(zip(("frames", "pixels", "launches", "device_mb", "queued"), out.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def feature_responses_paired(ctx, images, samples, features):
+    """Response of sample i to feature i (features [11][n] int8) -> float64 [n]."""
+    samples = np.ascontiguousarray(samples, SAMPLE_DTYPE)
+    features = np.ascontiguousarray(features, np.int8)
+    out = np.zeros(len(samples), np.float64)
+    ctx.check(lib.cf_feature_responses_paired(ctx.h, images.h, _ptr(samples), len(samples), _ptr(features),
+                                              _ptr(out)))
+    return out
+
+
+def synth_forest(ctx, images, depth_mm, trees=10, depth=20, classes=5, seed=4711, box_radius=120, region_size=20,
+                 retries=10):
+    """A forest of complete trees with random splits (benchmark workload; no reference counterpart).
+
+    Per split: type U{0,1}, offsets U[-box_radius, box_radius]^2, regions U[1, region_size]^2, colour
+    channels U{0,1,2}; the threshold is the (float) response at a random pixel of a random frame of
+    `images` (redrawn up to `retries` times while NaN, else 0).  Leaves: Dirichlet(1) posteriors over
+    `classes` (normalised Exp(1) draws) stored as counts scaled to 2^20, so normalizeHistograms(0,
+    False) reproduces them.  Trees are complete, in breadth-first order (children of i at 2i+1, 2i+2).
+    `depth_mm`: the raw depth frames [n][H][W] the images were built from (pixel validity)."""
+    rng = np.random.default_rng(seed)
+    n_frames, H, W = depth_mm.shape
+    n_nodes = (1 << depth) - 1
+    n_split = (1 << (depth - 1)) - 1
+    idx = np.arange(n_nodes, dtype=np.int64)
+    out = []
+    for _ in range(trees):
+        nodes = np.zeros(n_nodes, NODE_DTYPE)
+        nodes["id"] = idx
+        nodes["level"] = np.floor(np.log2(idx + 1)).astype(np.int32) + 1
+        nodes["parent"] = np.where(idx > 0, (idx - 1) // 2, -1)
+        split = idx < n_split
+        nodes["left"] = np.where(split, 2 * idx + 1, -1)
+        nodes["right"] = np.where(split, 2 * idx + 2, -1)
+        nodes["is_leaf"] = ~split
+        f = np.zeros((11, n_split), np.int8)
+        f[0] = rng.integers(0, 2, n_split)
+        for r in (1, 2, 3, 4):
+            f[r] = rng.integers(-box_radius, box_radius + 1, n_split)
+        for r in (5, 6, 7, 8):
+            f[r] = rng.integers(1, region_size + 1, n_split)
+        col = f[0] == COLOR
+        f[9] = np.where(col, rng.integers(0, 3, n_split), 0)
+        f[10] = np.where(col, rng.integers(0, 3, n_split), 0)
+        thr = np.zeros(n_split, np.float32)
+        todo = np.arange(n_split)
+        for _ in range(retries):
+            if len(todo) == 0:
+                break
+            s = np.zeros(len(todo), SAMPLE_DTYPE)
+            s["image"] = rng.integers(0, n_frames, len(todo))
+            s["x"] = rng.integers(0, W, len(todo))
+            s["y"] = rng.integers(0, H, len(todo))
+            s["depth_mm"] = depth_mm[s["image"], s["y"], s["x"]]
+            r = feature_responses_paired(ctx, images, s, np.ascontiguousarray(f[:, todo]))
+            ok = ~np.isnan(r)
+            thr[todo[ok]] = r[ok].astype(np.float32)
+            todo = todo[~ok]
+        nodes["feature"][:n_split] = f.T
+        nodes["threshold"][:n_split] = thr
+        hist = np.ones((n_nodes, classes), np.uint32)
+        e = rng.exponential(1.0, (n_nodes - n_split, classes))
+        hist[n_split:] = np.maximum(np.rint(e / e.sum(axis=1, keepdims=True) * (1 << 20)), 0).astype(np.uint32)
+        out.append((nodes, hist))
+    return RandomForestImage.from_trees(ctx, out, classes)
 
 
 def synth_dataset(n, width, height, sites, classes, zero_frac, seed0=4711, threads=8):

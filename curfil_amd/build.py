@@ -26,6 +26,7 @@ SOURCES = [
     "trainer.cpp",
     "sampling.cpp",
     "synth.cpp",
+    "server.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -90,11 +90,17 @@ void singleNodeBins(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n,
             dG.get());
 }
 
+}  // namespace
+
+namespace cf {
+
 void uploadForest(cf_forest* fo) {
     if (fo->uploaded) return;
     if (fo->post.size() != fo->trees.size()) throw Error(CF_ERR_RUNTIME, "histograms not normalized");
+    // one 32-byte record per node: feature, threshold and the left child (right = left + 1), or
+    // -(leaf index)-1 for a leaf, so a traversal step is one record load
     std::vector<SplitRec> nodes;
-    std::vector<int32_t> child, roots;
+    std::vector<int32_t> roots;
     std::vector<double> post;
     for (size_t t = 0; t < fo->trees.size(); t++) {
         const TreeBuild& tb = fo->trees[t];
@@ -105,31 +111,29 @@ void uploadForest(cf_forest* fo) {
             SplitRec sr{};
             for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = n.feature[p];
             sr.threshold = n.threshold;
-            nodes.push_back(sr);
             if (n.is_leaf) {
                 const int leaf = static_cast<int>(post.size() / fo->C);
-                child.push_back(-leaf - 1);
+                sr.flags = -leaf - 1;
                 post.insert(post.end(), fo->post[t].begin() + i * fo->C, fo->post[t].begin() + (i + 1) * fo->C);
             } else {
                 if (n.right != n.left + 1) throw Error(CF_ERR_ARG, "children must be stored as adjacent pairs");
-                child.push_back(base + n.left);
+                sr.flags = base + n.left;
             }
+            nodes.push_back(sr);
         }
     }
     fo->dNodes.resize(nodes.size());
-    fo->dChild.resize(child.size());
     fo->dPost.resize(std::max<size_t>(post.size(), 1));
     fo->dRoots.resize(roots.size());
     hipStream_t s = fo->ctx->stream;
     CF_HIP(hipMemcpyAsync(fo->dNodes.get(), nodes.data(), nodes.size() * sizeof(SplitRec), hipMemcpyHostToDevice, s));
-    CF_HIP(hipMemcpyAsync(fo->dChild.get(), child.data(), child.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     CF_HIP(hipMemcpyAsync(fo->dPost.get(), post.data(), post.size() * sizeof(double), hipMemcpyHostToDevice, s));
     CF_HIP(hipMemcpyAsync(fo->dRoots.get(), roots.data(), roots.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     CF_HIP(hipStreamSynchronize(s));
     fo->uploaded = true;
 }
 
-}  // namespace
+}  // namespace cf
 
 cf::ImageSet cf_images::set() const { return cf::ImageSet{col.get(), dep.get(), W, H}; }
 
@@ -322,6 +326,7 @@ int cf_feature_responses(cf_ctx* ctx, cf_images* im, const cf_sample* samples, i
     return guarded(ctx, [&] {
         require(im && samples && features && out, "null argument");
         require(n > 0 && F > 0, "sizes must be positive");
+        require(static_cast<size_t>(n) * F < (1u << 31), "n x F too large for one call");
         checkSamples(im, samples, n);
         checkFeaturesSoA(features, F);
         std::vector<FeatRec> fr(F);
@@ -336,6 +341,29 @@ int cf_feature_responses(cf_ctx* ctx, cf_images* im, const cf_sample* samples, i
         CF_HIP(hipMemcpyAsync(dF.get(), fr.data(), F * sizeof(FeatRec), hipMemcpyHostToDevice, ctx->stream));
         launchResponses(ctx->stream, im->desc.get(), dS.get(), n, dF.get(), F, dO.get());
         CF_HIP(hipMemcpyAsync(out, dO.get(), static_cast<size_t>(n) * F * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_feature_responses_paired(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n, const int8_t* features,
+        double* out) {
+    return guarded(ctx, [&] {
+        require(im && samples && features && out, "null argument");
+        require(n > 0, "sizes must be positive");
+        checkSamples(im, samples, n);
+        checkFeaturesSoA(features, n);
+        std::vector<FeatRec> fr(n);
+        for (int f = 0; f < n; f++) {
+            std::memset(&fr[f], 0, sizeof(FeatRec));
+            for (int p = 0; p < kFeatParams; p++) fr[f].p[p] = features[static_cast<size_t>(p) * n + f];
+        }
+        DevBuf<cf_sample> dS(n);
+        DevBuf<FeatRec> dF(n);
+        DevBuf<double> dO(n);
+        CF_HIP(hipMemcpyAsync(dS.get(), samples, static_cast<size_t>(n) * sizeof(cf_sample), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dF.get(), fr.data(), static_cast<size_t>(n) * sizeof(FeatRec), hipMemcpyHostToDevice, ctx->stream));
+        launchResponsesPaired(ctx->stream, im->desc.get(), dS.get(), n, dF.get(), dO.get());
+        CF_HIP(hipMemcpyAsync(out, dO.get(), static_cast<size_t>(n) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         CF_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -667,16 +695,23 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         for (int i = 0; i < count; i++) require(indices[i] >= 0 && indices[i] < im->count, "frame index out of range");
         uploadForest(fo);
         const size_t plane = static_cast<size_t>(im->W) * im->H;
-        DevBuf<int> dIdx(count);
-        DevBuf<float> dP(probs ? static_cast<size_t>(count) * fo->C * plane : 1);
-        DevBuf<uint8_t> dL(labels ? static_cast<size_t>(count) * plane : 1);
-        CF_HIP(hipMemcpyAsync(dIdx.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        DevForest df{fo->dNodes.get(), fo->dChild.get(), fo->dPost.get(), fo->dRoots.get(),
+        // device staging lives in the forest and only grows: no allocation per call at a steady size
+        fo->dFrames.resize(count);
+        if (probs) fo->dProbs.resize(static_cast<size_t>(count) * fo->C * plane);
+        if (labels) fo->dLabels.resize(static_cast<size_t>(count) * plane);
+        CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(),
                 static_cast<int>(fo->trees.size()), fo->C};
-        launchClassify(ctx->stream, im->desc.get(), dIdx.get(), count, im->W, im->H, df, probs ? dP.get() : nullptr,
-                labels ? dL.get() : nullptr);
-        if (probs) CF_HIP(hipMemcpyAsync(probs, dP.get(), dP.bytes(), hipMemcpyDeviceToHost, ctx->stream));
-        if (labels) CF_HIP(hipMemcpyAsync(labels, dL.get(), dL.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df,
+                probs ? fo->dProbs.get() : nullptr, labels ? fo->dLabels.get() : nullptr);
+        if (probs) {
+            CF_HIP(hipMemcpyAsync(probs, fo->dProbs.get(), static_cast<size_t>(count) * fo->C * plane * sizeof(float),
+                    hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (labels) {
+            CF_HIP(hipMemcpyAsync(labels, fo->dLabels.get(), static_cast<size_t>(count) * plane, hipMemcpyDeviceToHost,
+                    ctx->stream));
+        }
         CF_HIP(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -243,7 +243,7 @@ __device__ __forceinline__ float roundUpToFloat(double r) {
 struct SplitRec {
     FeatRec feat;
     float threshold;
-    int32_t flags;
+    int32_t flags;       // packed forests: left child (global index), or -(leaf index)-1
     int32_t pad0, pad1;
 };
 

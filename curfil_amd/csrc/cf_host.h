@@ -61,8 +61,16 @@ struct cf_forest {
     std::vector<std::vector<double>> post;  // normalized histograms [nodes][C] per tree
     bool uploaded = false;
     cf::DevBuf<cf::SplitRec> dNodes;
-    cf::DevBuf<int32_t> dChild;
     cf::DevBuf<double> dPost;
     cf::DevBuf<int32_t> dRoots;
+    // prediction staging kept between calls (grown on demand)
+    cf::DevBuf<int> dFrames;
+    cf::DevBuf<float> dProbs;
+    cf::DevBuf<uint8_t> dLabels;
     cf::TrainStats stats;
 };
+
+namespace cf {
+// packs a normalized forest into device node records (once; later calls return at once)
+void uploadForest(cf_forest* fo);
+}  // namespace cf

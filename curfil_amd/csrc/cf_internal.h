@@ -213,6 +213,8 @@ void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, N
 void launchCountersFromBins(hipStream_t s, const uint32_t* G, const FeatTables& tabs, int C, uint32_t* counters);
 void launchResponses(hipStream_t s, const DevImage* imgs, const cf_sample* samples, int n, const FeatRec* feats,
         int F, double* out);
+void launchResponsesPaired(hipStream_t s, const DevImage* imgs, const cf_sample* samples, int n, const FeatRec* feats,
+        double* out);
 
 // Partition
 void launchPartitionDirections(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
@@ -222,8 +224,7 @@ void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk
 
 // Prediction
 struct DevForest {
-    const SplitRec* nodes;      // all trees concatenated
-    const int32_t* child;       // left child (global index) or -(leaf index)-1
+    const SplitRec* nodes;      // all trees concatenated; flags: left child (global index) or -(leaf index)-1
     const double* post;         // [leaves][C]
     const int32_t* treeRoot;    // root node index of each tree
     int numTrees;
@@ -231,5 +232,12 @@ struct DevForest {
 };
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
         const DevForest& forest, float* probs, uint8_t* labels);
+// one frame of a serving batch: its integrals and its outputs ([C][H][W] posteriors, [H][W] labels)
+struct FrameJob {
+    DevImage im;
+    float* probs;
+    uint8_t* labels;
+};
+void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int W, int H, const DevForest& forest);
 
 }  // namespace cf

@@ -12,16 +12,48 @@
 
 namespace cf {
 
+constexpr int kLockstep = 4;   // trees traversed together by one thread
+
+// One node record: the feature (16 bytes) and {threshold, child, -, -}.
+struct NodeRec {
+    uint4 feat;
+    uint4 tail;
+};
+
+__device__ __forceinline__ Feat unpackFeat(const uint4& w) {
+    Feat f;
+    f.type = sbyte(w.x, 0);
+    f.o1x = sbyte(w.x, 1);
+    f.o1y = sbyte(w.x, 2);
+    f.o2x = sbyte(w.x, 3);
+    f.o2y = sbyte(w.y, 0);
+    f.r1x = sbyte(w.y, 1);
+    f.r1y = sbyte(w.y, 2);
+    f.r2x = sbyte(w.y, 3);
+    f.r2y = sbyte(w.z, 0);
+    f.c1 = static_cast<int>((w.z >> 8) & 0xffu);
+    f.c2 = static_cast<int>((w.z >> 16) & 0xffu);
+    return f;
+}
+
+__device__ __forceinline__ NodeRec loadNode(const SplitRec* nodes, int i) {
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
+    return NodeRec{p[0], p[1]};
+}
+
+// One thread per pixel; the frames of a batch are the grid's y dimension.
+// Trees are walked kLockstep at a time: each step loads the current node of
+// every unfinished tree, issues all their corner gathers, then forms the
+// responses, so the dependent load chains of the trees overlap.  Leaf
+// posteriors are then added in tree order, as the reference does.
 template<int CM>
-__global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ imgs, const int* __restrict__ frameIdx,
-        int W, int H, DevForest forest, float* __restrict__ probs, uint8_t* __restrict__ labels) {
-    const int frame = blockIdx.y;
-    const int px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= W * H) return;
+__device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const DevForest& forest,
+        float* __restrict__ pout, uint8_t* __restrict__ lout) {
+    const int W = im.W, H = im.H;
     const int x = px % W, y = px / W;
-    const DevImage im = imgs[frameIdx[frame]];
     const int C = forest.C;
-    // PixelInstance depth
+    const size_t plane = static_cast<size_t>(W) * H;
+    // PixelInstance depth (random_tree_image.h:140-165)
     auto at = [&](int xx, int yy) -> int2 { return im.dep[yy * W + xx]; };
     const int2 c00 = at(x, y);
     const int2 cl = x > 0 ? at(x - 1, y) : make_int2(0, 0);
@@ -34,23 +66,86 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
         depth = static_cast<int32_t>(static_cast<uint32_t>(c00.x) -
                 (static_cast<uint32_t>(cl.x) + static_cast<uint32_t>(ca.x) - static_cast<uint32_t>(cal.x)));
     }
+    const bool depthOk = depth > 0;
+    const double rinv = depthOk ? depthReciprocal(depthToFloat(depth)) : 0.0;
     float acc[CM];
 #pragma unroll
     for (int c = 0; c < CM; c++) acc[c] = 0.0f;
-    for (int t = 0; t < forest.numTrees; t++) {
-        int node = forest.treeRoot[t];
-        int ch = forest.child[node];
-        while (ch >= 0) {
-            const Feat f = loadFeat(&forest.nodes[node].feat);
-            const float threshold = forest.nodes[node].threshold;
-            const double r = featureResponse(im, f, x, y, depth, false);
-            node = (r <= static_cast<double>(threshold)) ? ch : ch + 1;
-            ch = forest.child[node];
-        }
-        const double* post = forest.post + static_cast<size_t>(-ch - 1) * C;
+    for (int t0 = 0; t0 < forest.numTrees; t0 += kLockstep) {
+        NodeRec rec[kLockstep];
+        int leaf[kLockstep];
+        bool open[kLockstep];
 #pragma unroll
-        for (int c = 0; c < CM; c++) {
-            if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + post[c]);
+        for (int k = 0; k < kLockstep; k++) {
+            open[k] = t0 + k < forest.numTrees;
+            leaf[k] = 0;
+            if (open[k]) rec[k] = loadNode(forest.nodes, forest.treeRoot[t0 + k]);
+        }
+        for (;;) {
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < kLockstep; k++) {
+                if (open[k] && static_cast<int>(rec[k].tail.y) < 0) {
+                    leaf[k] = -static_cast<int>(rec[k].tail.y) - 1;
+                    open[k] = false;
+                }
+                any |= open[k];
+            }
+            if (!any) break;
+            bool ok[kLockstep], isDepth[kLockstep];
+            int2 cv[kLockstep][8];   // depth corners, or colour corners' bits in .x
+#pragma unroll
+            for (int k = 0; k < kLockstep; k++) {
+                ok[k] = false;
+                isDepth[k] = false;
+                if (open[k]) {
+                    const Feat f = unpackFeat(rec[k].feat);
+                    int idx[8];
+                    isDepth[k] = f.type == T_DEPTH;
+                    ok[k] = depthOk && responseCorners(f, x, y, W, H, rinv, false, idx);
+                    if (ok[k]) {
+                        if (isDepth[k]) {
+#pragma unroll
+                            for (int c = 0; c < 8; c++) cv[k][c] = planeAt(im.dep, idx[c]);
+                        } else {
+                            const float* colA = im.col + static_cast<size_t>(f.c1) * plane;
+                            const float* colB = im.col + static_cast<size_t>(f.c2) * plane;
+#pragma unroll
+                            for (int c = 0; c < 4; c++) cv[k][c].x = __float_as_int(planeAt(colA, idx[c]));
+#pragma unroll
+                            for (int c = 4; c < 8; c++) cv[k][c].x = __float_as_int(planeAt(colB, idx[c]));
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kLockstep; k++) {
+                if (!open[k]) continue;
+                double r = cf_nan();
+                if (ok[k]) {
+                    if (isDepth[k]) {
+                        r = depthResponseFromCorners(cv[k]);
+                    } else {
+                        float c8[8];
+#pragma unroll
+                        for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[k][c].x);
+                        r = colorResponseFromCorners(c8);
+                    }
+                }
+                // SplitFunction::split: left iff r <= threshold; NaN goes right
+                const int child = static_cast<int>(rec[k].tail.y);
+                const float thr = __uint_as_float(rec[k].tail.x);
+                rec[k] = loadNode(forest.nodes, (r <= static_cast<double>(thr)) ? child : child + 1);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kLockstep; k++) {
+            if (t0 + k >= forest.numTrees) break;
+            const double* post = forest.post + static_cast<size_t>(leaf[k]) * C;
+#pragma unroll
+            for (int c = 0; c < CM; c++) {
+                if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + post[c]);
+            }
         }
     }
     double sum = 0.0;
@@ -60,8 +155,6 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
     }
     float bestProb = -1.0f;
     int label = 0;
-    const size_t plane = static_cast<size_t>(W) * H;
-    float* pout = probs ? probs + static_cast<size_t>(frame) * C * plane : nullptr;
 #pragma unroll
     for (int c = 0; c < CM; c++) {
         if (c < C) {
@@ -73,7 +166,30 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
             }
         }
     }
-    if (labels) labels[static_cast<size_t>(frame) * plane + px] = static_cast<uint8_t>(label);
+    if (lout) lout[px] = static_cast<uint8_t>(label);
+}
+
+// frames of one image set: frame f of the batch is image frameIdx[f]; outputs stacked per frame
+template<int CM>
+__global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ imgs, const int* __restrict__ frameIdx,
+        int W, int H, DevForest forest, float* __restrict__ probs, uint8_t* __restrict__ labels) {
+    const int frame = blockIdx.y;
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= W * H) return;
+    const size_t plane = static_cast<size_t>(W) * H;
+    classifyPixel<CM>(imgs[frameIdx[frame]], px, forest,
+            probs ? probs + static_cast<size_t>(frame) * forest.C * plane : nullptr,
+            labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
+}
+
+// independent frames of one size, each with its own images and outputs
+template<int CM>
+__global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restrict__ jobs, int W, int H,
+        DevForest forest) {
+    const FrameJob job = jobs[blockIdx.y];
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= W * H) return;
+    classifyPixel<CM>(job.im, px, forest, job.probs, job.labels);
 }
 
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
@@ -83,6 +199,18 @@ void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, in
         hipLaunchKernelGGL(k_classify<8>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
     } else if (forest.C <= kMaxClasses) {
         hipLaunchKernelGGL(k_classify<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
+    } else {
+        throw Error(CF_ERR_ARG, "too many classes (max 32)");
+    }
+}
+
+void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int W, int H, const DevForest& forest) {
+    if (numJobs == 0) return;
+    const dim3 grid((W * H + 255) / 256, numJobs);
+    if (forest.C <= 8) {
+        hipLaunchKernelGGL(k_classify_jobs<8>, grid, dim3(256), 0, s, jobs, W, H, forest);
+    } else if (forest.C <= kMaxClasses) {
+        hipLaunchKernelGGL(k_classify_jobs<kMaxClasses>, grid, dim3(256), 0, s, jobs, W, H, forest);
     } else {
         throw Error(CF_ERR_ARG, "too many classes (max 32)");
     }

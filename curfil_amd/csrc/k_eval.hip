@@ -463,6 +463,15 @@ __global__ void __launch_bounds__(256) k_responses(const DevImage* __restrict__ 
     out[i] = featureResponse(imgs[s.image], loadFeat(feats + f), s.x, s.y, s.depth_mm, s.flip == 1);
 }
 
+// response of sample i to feature i
+__global__ void __launch_bounds__(256) k_responses_paired(const DevImage* __restrict__ imgs,
+        const cf_sample* __restrict__ samples, int n, const FeatRec* __restrict__ feats, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cf_sample s = samples[i];
+    out[i] = featureResponse(imgs[s.image], loadFeat(feats + i), s.x, s.y, s.depth_mm, s.flip == 1);
+}
+
 void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const Chunk* chunks,
         int numChunks, const FeatTables& tabs, int C, uint32_t* G) {
     if (numChunks == 0) return;
@@ -529,6 +538,12 @@ void launchResponses(hipStream_t s, const DevImage* imgs, const cf_sample* sampl
     const int total = n * F;
     if (total == 0) return;
     hipLaunchKernelGGL(k_responses, dim3((total + 255) / 256), dim3(256), 0, s, imgs, samples, n, feats, F, out);
+}
+
+void launchResponsesPaired(hipStream_t s, const DevImage* imgs, const cf_sample* samples, int n, const FeatRec* feats,
+        double* out) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_responses_paired, dim3((n + 255) / 256), dim3(256), 0, s, imgs, samples, n, feats, out);
 }
 
 }  // namespace cf

@@ -1,0 +1,446 @@
+// Serving loop for dense prediction: RGB-D frames of any resolution arrive
+// one by one (cf_server_submit), are admitted in arrival order while device
+// memory of the budget is free, have their integral images built on an
+// upload stream, are classified in batches of equal resolution on a compute
+// stream, and return their posteriors and labels through a download stream.
+// One worker thread keeps the three stages busy: frames that arrive while a
+// batch runs are admitted and integrated meanwhile and go into the next
+// batch (continuous batching), and a frame's device slot is reused as soon
+// as its results are on the host.  The forest stays resident.
+//
+// RandomForestImage::predict (random_forest_image.cpp:258-352) per frame: the
+// same kernel body as cf_predict, so a frame's results do not depend on what
+// else is in flight.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "cf_host.h"
+
+namespace cf {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// pinned host buffers, reused by size
+class PinnedPool {
+public:
+    ~PinnedPool() {
+        for (auto& kv : free_) {
+            for (void* p : kv.second) (void)hipHostFree(p);
+        }
+    }
+    void* take(size_t bytes) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto& v = free_[bytes];
+            if (!v.empty()) {
+                void* p = v.back();
+                v.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        CF_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        return p;
+    }
+    void give(void* p, size_t bytes) {
+        std::lock_guard<std::mutex> g(mu_);
+        free_[bytes].push_back(p);
+    }
+
+private:
+    std::mutex mu_;
+    std::map<size_t, std::vector<void*>> free_;
+};
+
+// device memory of one frame of W x H: raw input, integral scratch, integrals, outputs
+struct Slot {
+    int W = 0, H = 0;
+    size_t bytes = 0;
+    DevBuf<uint8_t> in;      // rgb [H][W][3], then depth u16 [H][W] at inDepthOffset
+    DevBuf<uint32_t> tmp;    // column prefixes of the 5 channels
+    DevBuf<float> col;       // [3][H][W]
+    DevBuf<int2> dep;        // [H][W]
+    DevBuf<float> probs;     // [C][H][W]
+    DevBuf<uint8_t> labels;  // [H][W]
+};
+
+size_t inputBytes(int W, int H) {
+    const size_t px = static_cast<size_t>(W) * H;
+    return ((px * 3 + 15) / 16) * 16 + px * 2;
+}
+size_t depthOffset(int W, int H) { return ((static_cast<size_t>(W) * H * 3 + 15) / 16) * 16; }
+
+// device bytes of one slot: input, integral scratch, integrals, posteriors and labels
+size_t slotBytes(int W, int H, int C) {
+    const size_t px = static_cast<size_t>(W) * H;
+    return inputBytes(W, H) + px * 5 * sizeof(uint32_t) + px * 3 * sizeof(float) + px * sizeof(int2) +
+            px * C * sizeof(float) + px;
+}
+
+struct Request {
+    uint64_t id = 0;
+    int W = 0, H = 0;
+    void* hostIn = nullptr;      // pinned input
+    float* hostProbs = nullptr;  // pinned outputs
+    uint8_t* hostLabels = nullptr;
+    Slot* slot = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    bool finished = false;
+    Clock::time_point submitted, completed;
+};
+
+constexpr int kJobRing = 64;   // job tables in flight
+
+}  // namespace
+}  // namespace cf
+
+using namespace cf;
+
+struct cf_server {
+    cf_ctx* ctx = nullptr;
+    cf_forest* fo = nullptr;
+    size_t budget = 0;
+    int maxBatch = 64;
+    hipStream_t up = nullptr, comp = nullptr, down = nullptr;
+    PinnedPool pinned;
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    std::deque<Request*> queued;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+    std::map<std::pair<int, int>, std::vector<Slot*>> freeSlots;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<hipEvent_t> freeEvents;
+    size_t devUsed = 0;
+    uint64_t nextId = 1;
+    bool stop = false;
+    std::string error;
+    // job tables: device ring, pinned host staging, and an event per entry marking its kernel done
+    DevBuf<FrameJob> dJobs;
+    FrameJob* hJobs = nullptr;
+    hipEvent_t jobDone[kJobRing] = {};
+    int jobNext = 0;
+    // statistics
+    uint64_t framesDone = 0, launches = 0;
+    double pixelsDone = 0;
+    std::thread worker;
+
+    hipEvent_t takeEvent() {
+        if (!freeEvents.empty()) {
+            hipEvent_t e = freeEvents.back();
+            freeEvents.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+
+    // a free slot of W x H, or a new one if the budget allows (caller holds mu)
+    Slot* takeSlot(int W, int H) {
+        auto& fl = freeSlots[{W, H}];
+        if (!fl.empty()) {
+            Slot* s = fl.back();
+            fl.pop_back();
+            return s;
+        }
+        const size_t px = static_cast<size_t>(W) * H;
+        const size_t bytes = slotBytes(W, H, fo->C);
+        if (devUsed + bytes > budget) {
+            // give back idle slots of other sizes before refusing
+            for (auto& kv : freeSlots) {
+                while (!kv.second.empty() && devUsed + bytes > budget) {
+                    Slot* s = kv.second.back();
+                    kv.second.pop_back();
+                    devUsed -= s->bytes;
+                    for (auto it = slots.begin(); it != slots.end(); ++it) {
+                        if (it->get() == s) {
+                            slots.erase(it);
+                            break;
+                        }
+                    }
+                }
+            }
+            if (devUsed + bytes > budget) return nullptr;
+        }
+        auto s = std::make_unique<Slot>();
+        s->W = W;
+        s->H = H;
+        s->bytes = bytes;
+        s->in.resize(inputBytes(W, H));
+        s->tmp.resize(px * 5);
+        s->col.resize(px * 3);
+        s->dep.resize(px);
+        s->probs.resize(px * fo->C);
+        s->labels.resize(px);
+        devUsed += bytes;
+        slots.push_back(std::move(s));
+        return slots.back().get();
+    }
+
+    void run();
+};
+
+void cf_server::run() {
+    try {
+        CF_HIP(hipSetDevice(ctx->device));
+        std::vector<Request*> ready;     // integrating or integrated, not yet classified
+        std::vector<Request*> inflight;  // classified, results on their way to the host
+        const DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()),
+                fo->C};
+        for (;;) {
+            std::vector<Request*> admit;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                if (queued.empty() && ready.empty() && inflight.empty()) {
+                    cvWork.wait(lk, [&] { return stop || !queued.empty(); });
+                } else if (ready.empty() && !inflight.empty()) {
+                    cvWork.wait_for(lk, std::chrono::microseconds(20));
+                }
+                if (stop && queued.empty() && ready.empty() && inflight.empty()) break;
+                while (!queued.empty()) {
+                    Request* r = queued.front();
+                    Slot* s = takeSlot(r->W, r->H);
+                    if (!s) break;   // arrival order: later frames wait for this one
+                    queued.pop_front();
+                    r->slot = s;
+                    r->ready = takeEvent();
+                    r->done = takeEvent();
+                    admit.push_back(r);
+                }
+            }
+            // upload and integrate the admitted frames
+            for (Request* r : admit) {
+                Slot* s = r->slot;
+                CF_HIP(hipMemcpyAsync(s->in.get(), r->hostIn, inputBytes(r->W, r->H), hipMemcpyHostToDevice, up));
+                launchIntegralU8(up, s->in.get(), reinterpret_cast<const uint16_t*>(s->in.get() + depthOffset(r->W, r->H)),
+                        r->W, r->H, s->tmp.get(), s->col.get(), s->dep.get());
+                CF_HIP(hipEventRecord(r->ready, up));
+                ready.push_back(r);
+            }
+            // classify everything admitted, one launch per resolution (at most maxBatch frames each)
+            if (!ready.empty()) {
+                std::map<std::pair<int, int>, std::vector<Request*>> groups;
+                for (Request* r : ready) groups[{r->W, r->H}].push_back(r);
+                ready.clear();
+                for (auto& kv : groups) {
+                    auto& v = kv.second;
+                    for (size_t b = 0; b < v.size(); b += maxBatch) {
+                        const int n = static_cast<int>(std::min<size_t>(maxBatch, v.size() - b));
+                        const int e = jobNext;
+                        jobNext = (jobNext + 1) % kJobRing;
+                        CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
+                        FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
+                        for (int i = 0; i < n; i++) {
+                            Request* r = v[b + i];
+                            Slot* s = r->slot;
+                            hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
+                            hj[i].probs = s->probs.get();
+                            hj[i].labels = s->labels.get();
+                            CF_HIP(hipStreamWaitEvent(comp, r->ready, 0));
+                        }
+                        FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
+                        CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, comp));
+                        launchClassifyJobs(comp, dj, n, kv.first.first, kv.first.second, df);
+                        CF_HIP(hipEventRecord(jobDone[e], comp));
+                        launches++;
+                        CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
+                        for (int i = 0; i < n; i++) {
+                            Request* r = v[b + i];
+                            Slot* s = r->slot;
+                            const size_t px = static_cast<size_t>(r->W) * r->H;
+                            CF_HIP(hipMemcpyAsync(r->hostProbs, s->probs.get(), px * fo->C * sizeof(float),
+                                    hipMemcpyDeviceToHost, down));
+                            CF_HIP(hipMemcpyAsync(r->hostLabels, s->labels.get(), px, hipMemcpyDeviceToHost, down));
+                            CF_HIP(hipEventRecord(r->done, down));
+                            inflight.push_back(r);
+                        }
+                    }
+                }
+            }
+            // finished frames: results are in pinned host memory; their slots are free again
+            std::vector<Request*> still;
+            bool any = false;
+            for (Request* r : inflight) {
+                const hipError_t q = hipEventQuery(r->done);
+                if (q == hipErrorNotReady) {
+                    still.push_back(r);
+                    continue;
+                }
+                CF_HIP(q);
+                std::lock_guard<std::mutex> g(mu);
+                r->completed = Clock::now();
+                r->finished = true;
+                freeSlots[{r->W, r->H}].push_back(r->slot);
+                r->slot = nullptr;
+                freeEvents.push_back(r->ready);
+                freeEvents.push_back(r->done);
+                r->ready = r->done = nullptr;
+                framesDone++;
+                pixelsDone += static_cast<double>(r->W) * r->H;
+                any = true;
+            }
+            inflight.swap(still);
+            if (any) cvDone.notify_all();
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> g(mu);
+        error = e.what();
+        stop = true;
+        cvDone.notify_all();
+    }
+}
+
+extern "C" {
+
+int cf_server_create(cf_ctx* ctx, cf_forest* fo, double budget_mb, int max_batch, cf_server** out) {
+    if (!ctx) return CF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    try {
+        if (!fo || !out) throw Error(CF_ERR_ARG, "null argument");
+        if (!(budget_mb > 0) || max_batch <= 0) throw Error(CF_ERR_ARG, "budget and batch size must be positive");
+        if (fo->post.size() != fo->trees.size()) throw Error(CF_ERR_RUNTIME, "histograms not normalized");
+        CF_HIP(hipSetDevice(ctx->device));
+        auto sv = std::make_unique<cf_server>();
+        sv->ctx = ctx;
+        sv->fo = fo;
+        sv->budget = static_cast<size_t>(budget_mb * 1024.0 * 1024.0);
+        sv->maxBatch = max_batch;
+        uploadForest(fo);
+        CF_HIP(hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking));
+        CF_HIP(hipStreamCreateWithFlags(&sv->comp, hipStreamNonBlocking));
+        CF_HIP(hipStreamCreateWithFlags(&sv->down, hipStreamNonBlocking));
+        sv->dJobs.resize(static_cast<size_t>(kJobRing) * max_batch);
+        CF_HIP(hipHostMalloc(reinterpret_cast<void**>(&sv->hJobs), sizeof(FrameJob) * kJobRing * max_batch,
+                hipHostMallocDefault));
+        for (int i = 0; i < kJobRing; i++) {
+            CF_HIP(hipEventCreateWithFlags(&sv->jobDone[i], hipEventDisableTiming));
+            CF_HIP(hipEventRecord(sv->jobDone[i], sv->comp));
+        }
+        cf_server* raw = sv.get();
+        sv->worker = std::thread([raw] { raw->run(); });
+        *out = sv.release();
+        return CF_OK;
+    } catch (const Error& e) {
+        ctx->error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->error = e.what();
+        return CF_ERR_RUNTIME;
+    }
+}
+
+int cf_server_submit(cf_server* sv, const uint8_t* rgb, const uint16_t* depth, int width, int height,
+        uint64_t* id) {
+    if (!sv || !rgb || !depth || !id || width <= 0 || height <= 0 || width > 32767 || height > 32767) {
+        return CF_ERR_ARG;
+    }
+    // a frame that does not fit the device budget on its own could never be admitted
+    if (slotBytes(width, height, sv->fo->C) > sv->budget) return CF_ERR_NOMEM;
+    try {
+        auto r = std::make_unique<Request>();
+        r->W = width;
+        r->H = height;
+        const size_t px = static_cast<size_t>(width) * height;
+        r->hostIn = sv->pinned.take(inputBytes(width, height));
+        std::memcpy(r->hostIn, rgb, px * 3);
+        std::memcpy(static_cast<uint8_t*>(r->hostIn) + depthOffset(width, height), depth, px * 2);
+        r->hostProbs = static_cast<float*>(sv->pinned.take(px * sv->fo->C * sizeof(float)));
+        r->hostLabels = static_cast<uint8_t*>(sv->pinned.take(px));
+        std::lock_guard<std::mutex> g(sv->mu);
+        if (sv->stop) return CF_ERR_RUNTIME;
+        r->submitted = Clock::now();
+        r->id = sv->nextId++;
+        *id = r->id;
+        sv->queued.push_back(r.get());
+        sv->requests[r->id] = std::move(r);
+        sv->cvWork.notify_one();
+        return CF_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return CF_ERR_RUNTIME;
+    }
+}
+
+int cf_server_wait(cf_server* sv, uint64_t id, float* probs, uint8_t* labels, double* latency_ms) {
+    if (!sv) return CF_ERR_ARG;
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->requests.find(id);
+        if (it == sv->requests.end()) return CF_ERR_ARG;
+        Request* q = it->second.get();
+        sv->cvDone.wait(lk, [&] { return q->finished || !sv->error.empty(); });
+        if (!q->finished) return CF_ERR_RUNTIME;
+        r = std::move(it->second);
+        sv->requests.erase(it);
+    }
+    const size_t px = static_cast<size_t>(r->W) * r->H;
+    if (probs) std::memcpy(probs, r->hostProbs, px * sv->fo->C * sizeof(float));
+    if (labels) std::memcpy(labels, r->hostLabels, px);
+    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->completed - r->submitted).count();
+    sv->pinned.give(r->hostIn, inputBytes(r->W, r->H));
+    sv->pinned.give(r->hostProbs, px * sv->fo->C * sizeof(float));
+    sv->pinned.give(r->hostLabels, px);
+    return CF_OK;
+}
+
+int cf_server_stats(cf_server* sv, double* out, int n) {
+    if (!sv || !out) return CF_ERR_ARG;
+    std::lock_guard<std::mutex> g(sv->mu);
+    const double v[] = {static_cast<double>(sv->framesDone), sv->pixelsDone, static_cast<double>(sv->launches),
+            static_cast<double>(sv->devUsed) / (1024.0 * 1024.0), static_cast<double>(sv->queued.size())};
+    const int k = static_cast<int>(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < k; i++) out[i] = v[i];
+    return k;
+}
+
+const char* cf_server_error(cf_server* sv) {
+    if (!sv) return "null server";
+    std::lock_guard<std::mutex> g(sv->mu);
+    return sv->error.c_str();
+}
+
+int cf_server_destroy(cf_server* sv) {
+    if (!sv) return CF_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+        sv->cvWork.notify_all();
+    }
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->ctx->device);
+    (void)hipStreamSynchronize(sv->up);
+    (void)hipStreamSynchronize(sv->comp);
+    (void)hipStreamSynchronize(sv->down);
+    for (auto& kv : sv->requests) {
+        Request* r = kv.second.get();
+        const size_t px = static_cast<size_t>(r->W) * r->H;
+        sv->pinned.give(r->hostIn, inputBytes(r->W, r->H));
+        sv->pinned.give(r->hostProbs, px * sv->fo->C * sizeof(float));
+        sv->pinned.give(r->hostLabels, px);
+        if (r->ready) (void)hipEventDestroy(r->ready);
+        if (r->done) (void)hipEventDestroy(r->done);
+    }
+    for (hipEvent_t e : sv->freeEvents) (void)hipEventDestroy(e);
+    for (int i = 0; i < kJobRing; i++) {
+        if (sv->jobDone[i]) (void)hipEventDestroy(sv->jobDone[i]);
+    }
+    if (sv->hJobs) (void)hipHostFree(sv->hJobs);
+    (void)hipStreamDestroy(sv->up);
+    (void)hipStreamDestroy(sv->comp);
+    (void)hipStreamDestroy(sv->down);
+    delete sv;
+    return CF_OK;
+}
+
+}  // extern "C"

@@ -127,6 +127,12 @@ int cf_generate_features(cf_ctx* ctx, cf_images* images, const cf_sample* sample
 int cf_feature_responses(cf_ctx* ctx, cf_images* images, const cf_sample* samples, int n,
         const int8_t* features, int num_features, double* out);
 
+/* response of sample i to feature i (features [11][n] int8) -> out[n]:
+ * ImageFeatureFunction::calculateFeatureResponse for one instance
+ * (random_tree_image.h:485-623). */
+int cf_feature_responses_paired(cf_ctx* ctx, cf_images* images, const cf_sample* samples, int n,
+        const int8_t* features, double* out);
+
 /* per-node left/right class counters, layout [F][T][C][2] (counters(feat,
  * thresh, label, lr), random_tree_image_gpu.cu:1069-1085); replaces
  * calculateFeatureResponsesAndHistograms (random_tree_image_gpu.cu:1958-2135). */
@@ -209,6 +215,31 @@ int cf_predict(cf_ctx* ctx, cf_forest* forest, cf_images* images, int index, flo
 /* several frames of one image set in one launch sequence; outputs stacked per frame */
 int cf_predict_batch(cf_ctx* ctx, cf_forest* forest, cf_images* images, const int* indices, int count,
         float* probs, uint8_t* labels);
+
+/* ---- serving loop ------------------------------------------------------------ */
+/* RandomForestImage::predict (random_forest_image.cpp:258-352) as a service for
+ * frames that arrive one at a time, at any resolution: a worker thread admits
+ * submitted frames in arrival order while device memory of `budget_mb` is
+ * free (integral images, staging and outputs of each frame count), builds
+ * their integral images, classifies frames of equal size together (at most
+ * `max_batch` per launch) and copies results back, with the three stages on
+ * separate streams.  Frames submitted while others run join the next batch.
+ * A frame's results equal cf_predict's for the same frame.  The forest must
+ * stay unchanged while a server uses it.  (The reference classifies one frame
+ * per call; this loop has no single counterpart there.) */
+typedef struct cf_server cf_server;
+int cf_server_create(cf_ctx* ctx, cf_forest* forest, double budget_mb, int max_batch, cf_server** out);
+/* copies the frame (rgb [H][W][3] u8, depth [H][W] u16 mm); CF_ERR_NOMEM if one
+ * frame of this size exceeds the budget */
+int cf_server_submit(cf_server* server, const uint8_t* rgb, const uint16_t* depth_mm, int width, int height,
+        uint64_t* id);
+/* blocks until frame `id` is done; probs [C][H][W] and labels [H][W] may be
+ * NULL; latency_ms = completion (results in host memory) - submission */
+int cf_server_wait(cf_server* server, uint64_t id, float* probs, uint8_t* labels, double* latency_ms);
+/* out[0..4]: frames done, pixels done, classify launches, device MB in use, frames queued */
+int cf_server_stats(cf_server* server, double* out, int n);
+const char* cf_server_error(cf_server* server);
+int cf_server_destroy(cf_server* server);
 
 /* ---- synthetic workload (benchmark / tests; no counterpart in the reference) -- */
 /* One labelled RGB-D frame: Voronoi labels (site k -> label k mod (classes+1),

@@ -318,3 +318,61 @@ def test_three_tree_forest_and_prediction_match_oracle(ctx, config0):
         assert np.array_equal(bl[k], sl) and _same_float_bits(np.nan_to_num(bp[k]), np.nan_to_num(sp))
     acc = np.mean(forest.predict(im, 0)[0][labels[0] > 0] == labels[0][labels[0] > 0])
     print(f"training-image pixel accuracy (no void): {acc:.3f}")
+
+
+def test_random_complete_forest_prediction_matches_oracle(ctx, config0):
+    """Complete trees with random splits (the dense-prediction benchmark's forest, smaller): normalised
+    posteriors, per-pixel float posteriors and labels equal the oracle's CPU prediction bit for bit."""
+    rgb, depth, labels, im, ds = config0
+    forest = cf.synth_forest(ctx, im, depth.astype(np.int32), trees=4, depth=10, classes=5, seed=99)
+    forest.normalizeHistograms(0.0, False)
+    ones = np.ones(5, np.uint32)
+    trees, posts = [], []
+    for t in range(4):
+        nodes, hist = forest.tree(t)
+        assert len(nodes) == 1023 and nodes["is_leaf"].sum() == 512
+        post = np.array([orc.normalize_histogram(h, ones, 0.0, False) for h in hist])
+        assert np.array_equal(forest.posteriors(t), post)
+        trees.append(nodes)
+        posts.append(post)
+    for img in (0, 3):
+        gl, gp = forest.predict(im, img)
+        ol, op = ds.predict(img, trees, posts, 5)
+        assert np.array_equal(gl, ol)
+        assert _same_float_bits(gp, op)
+    bl, bp = forest.predict_batch(im, [3, 0])
+    sl, sp = forest.predict(im, 3)
+    assert np.array_equal(bl[0], sl) and _same_float_bits(bp[0], sp)
+
+
+def test_server_results_equal_single_frame_prediction(ctx, config0):
+    """Frames of three sizes submitted interleaved to the serving loop (continuous batching, and a budget
+    that makes frames wait for slots) give the same posteriors and labels, bit for bit, as predicting each
+    frame alone."""
+    rgb0, depth0, labels0, im0, ds = config0
+    forest = cf.synth_forest(ctx, im0, depth0.astype(np.int32), trees=3, depth=9, classes=5, seed=7)
+    forest.normalizeHistograms(0.0, False)
+    sizes = [(96, 64), (160, 120), (200, 150)]
+    frames, refs = [], []
+    for k, (W, H) in enumerate(sizes):
+        rgb, depth, _ = cf.synth_dataset(4, W, H, sites=8, classes=5, zero_frac=0.05, seed0=100 * k + 1)
+        im = _images_u8(ctx, rgb, depth)
+        for i in range(4):
+            frames.append((rgb[i], depth[i]))
+            refs.append(forest.predict(im, i))
+    order = [0, 4, 8, 1, 5, 9, 2, 6, 10, 3, 7, 11, 8, 0]
+    big = (200 * 150) * (5 + 20 + 12 + 8 + 5 * 4 + 1) / (1 << 20)
+    for budget in (256.0, 2.5 * big):
+        server = cf.Server(ctx, forest, budget_mb=budget, max_batch=4)
+        ids = [server.submit(*frames[j]) for j in order]
+        for rid, j in reversed(list(zip(ids, order))):
+            lab, probs, lat = server.wait(rid)
+            rl, rp = refs[j]
+            assert np.array_equal(lab, rl) and _same_float_bits(probs, rp), (budget, j)
+            assert lat > 0
+        st = server.stats()
+        assert st["frames"] == len(order)
+        server.close()
+    with pytest.raises(cf.CurfilError):
+        cf.Server(ctx, forest, budget_mb=0.01).submit(*frames[0])
+
