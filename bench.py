@@ -53,6 +53,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window per serving rate")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy a device buffer of this many MiB 3 times (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -132,9 +133,11 @@ def main():
                                        max(1, sum(p["eval_launches"] for p in profiles)), 4),
                 "bytes_per_response": BYTES_PER_RESPONSE}
 
-    plain_vs_batched = predict = None
+    plain_vs_batched = predict = predict10 = serving = None
     if not args.no_extras:
         plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
+        predict10, forest10 = dense_prediction_leg(ctx)
+        serving = serving_leg(ctx, forest10, args.serve_seconds)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -164,6 +167,8 @@ def main():
                                                     "partition_ms")},
         "plain_vs_batched": plain_vs_batched,
         "predict": predict,
+        "predict_forest10": predict10,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
@@ -226,6 +231,149 @@ def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):
                "single_frame_p99_ms": float(np.percentile(lat, 99)),
                "note": "batched = labels to host; single frame = posteriors + labels to host"}
     return plain_vs_batched, predict
+
+
+def dense_prediction_leg(ctx):
+    """Batched dense prediction: 10 complete trees of depth 20 with random splits (thresholds = responses
+    at random pixels, Dirichlet(1) leaves over 5 classes, seed 4711) on 64 synthetic 640x480 frames
+    (12 Voronoi sites, 5 classes, 5% missing depth)."""
+    import numpy as np
+    import curfil_amd as cf
+    W, H, n = 640, 480, 64
+    rgb, depth, _ = cf.synth_dataset(n, W, H, sites=12, classes=5, zero_frac=0.05, seed0=5711)
+    im = cf.Images(ctx, n, W, H)
+    for i in range(n):
+        im.set_rgbd_u8(i, rgb[i], depth[i])
+    t0 = time.time()
+    forest = cf.synth_forest(ctx, im, depth.astype(np.int32), trees=10, depth=20, classes=5, seed=4711)
+    forest.normalizeHistograms(0.0, False)
+    forest.predict_batch(im, list(range(n)), probabilities=True)   # forest upload + warm
+    log(f"forest of 10 trees x depth 20 built in {time.time() - t0:.1f}s")
+    out = {"workload": "64 frames 640x480, 10 complete trees of depth 20 (random splits), 5 classes"}
+    for probs in (False, True):
+        ts = []
+        for _ in range(5):
+            t1 = time.perf_counter()
+            forest.predict_batch(im, list(range(n)), probabilities=probs)
+            ts.append(time.perf_counter() - t1)
+        ms = float(np.median(ts)) * 1e3
+        key = "with_posteriors" if probs else "labels_only"
+        out[key] = {"batch_ms": ms, "frames_per_s": n / ms * 1e3, "mpix_per_s": n * W * H / ms / 1e3}
+    dp = cf.DeviceBuffer(ctx, n * 5 * H * W * 4)
+    dl = cf.DeviceBuffer(ctx, n * H * W)
+    ts = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        forest.predict_batch_device(im, list(range(n)), dp.ptr, dl.ptr)
+        ts.append(time.perf_counter() - t1)
+    ms = float(np.median(ts)) * 1e3
+    out["device_outputs"] = {"batch_ms": ms, "frames_per_s": n / ms * 1e3, "mpix_per_s": n * W * H / ms / 1e3}
+    dp.close()
+    dl.close()
+    lat = []
+    for i in range(100):
+        t1 = time.perf_counter()
+        forest.predict(im, i % n, probabilities=True)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    out["single_frame_p50_ms"] = float(np.percentile(lat, 50))
+    out["single_frame_p99_ms"] = float(np.percentile(lat, 99))
+    out["note"] = ("labels_only / with_posteriors: outputs copied to host memory; device_outputs: posteriors and "
+                   "labels written to device memory (caller-owned), no host copy")
+    log(f"dense prediction (10 trees depth 20): {out['device_outputs']['frames_per_s']:.0f} frames/s on device, "
+        f"{out['labels_only']['frames_per_s']:.0f} frames/s labels to host, "
+        f"{out['with_posteriors']['frames_per_s']:.0f} frames/s with posteriors; single frame p50 "
+        f"{out['single_frame_p50_ms']:.2f} ms p99 {out['single_frame_p99_ms']:.2f} ms")
+    return out, forest
+
+
+def serving_leg(ctx, forest, seconds):
+    """Poisson arrivals of RGB-D frames (320x240 40%, 640x480 50%, 1280x960 10%) at 1000-4000 frames/s
+    into the serving loop (8 GB budget), against the plain path (one frame at a time, in arrival order).
+    Latency = results in host memory - arrival."""
+    import threading
+    import numpy as np
+    import curfil_amd as cf
+    sizes = [(320, 240), (640, 480), (1280, 960)]
+    mix = np.array([0.4, 0.5, 0.1])
+    frames = []
+    for k, (W, H) in enumerate(sizes):
+        rgb, depth, _ = cf.synth_dataset(8, W, H, sites=12, classes=5, zero_frac=0.05, seed0=7711 + 100 * k)
+        frames.append((rgb, depth))
+    singles = []
+    for k, (W, H) in enumerate(sizes):
+        im = cf.Images(ctx, 1, W, H)
+        singles.append(im)
+    sys.setswitchinterval(0.0002)
+    results = {"workload": "Poisson arrivals, frames 320x240 (40%) / 640x480 (50%) / 1280x960 (10%), "
+                           "forest of 10 trees x depth 20, posteriors + labels to host", "rates": []}
+    rng = np.random.default_rng(4711)
+    server = cf.Server(ctx, forest, budget_mb=8192.0, max_batch=64)
+    # warm-up: the slots and pinned buffers a burst needs are allocated before timing
+    warm = [server.submit(frames[k][0][i % 8], frames[k][1][i % 8]) for k in range(3) for i in range(40)]
+    for rid in warm:
+        server.wait(rid, probabilities=False, labels=False)
+    for lam in (1000, 2000, 3000, 4000):
+        n = int(lam * seconds)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+        kinds = rng.choice(3, n, p=mix)
+        picks = rng.integers(0, 8, n)
+        row = {"rate_fps": lam, "frames": n}
+        # serving loop
+        launches0 = server.stats()["launches"]
+        ids = [None] * n
+        lat = np.zeros(n)
+
+        def submit_all():
+            t0 = time.perf_counter()
+            for i in range(n):
+                while time.perf_counter() - t0 < arrivals[i]:
+                    pass
+                rgb, depth = frames[kinds[i]]
+                ids[i] = server.submit(rgb[picks[i]], depth[picks[i]])
+        th = threading.Thread(target=submit_all)
+        t_start = time.perf_counter()
+        th.start()
+        parts = np.zeros((n, 3))
+        for i in range(n):
+            while ids[i] is None:
+                time.sleep(0.0001)
+            _, _, lat[i] = server.wait(ids[i], probabilities=False, labels=False)
+            parts[i] = server.last_breakdown
+        th.join()
+        slow = lat >= np.percentile(lat, 99)
+        wall = time.perf_counter() - t_start
+        px = float(sum(sizes[k][0] * sizes[k][1] for k in kinds))
+        row["serving"] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                          "frames_per_s": n / wall, "mpix_per_s": px / wall / 1e6,
+                          "launches": server.stats()["launches"] - launches0,
+                          "p99_tail_parts_ms": {"queued": float(parts[slow, 0].mean()),
+                                                "to_launch": float(parts[slow, 1].mean()),
+                                                "device_and_copy": float(parts[slow, 2].mean())},
+                          "median_parts_ms": {"queued": float(np.median(parts[:, 0])),
+                                              "to_launch": float(np.median(parts[:, 1])),
+                                              "device_and_copy": float(np.median(parts[:, 2]))}}
+        # plain path: one frame at a time in arrival order (upload + integrate + predict + results to host)
+        lat2 = np.zeros(n)
+        t0 = time.perf_counter()
+        for i in range(n):
+            now = time.perf_counter() - t0
+            if now < arrivals[i]:
+                while time.perf_counter() - t0 < arrivals[i]:
+                    pass
+            k = kinds[i]
+            rgb, depth = frames[k]
+            singles[k].set_rgbd_u8(0, rgb[picks[i]], depth[picks[i]])
+            forest.predict(singles[k], 0, probabilities=True)
+            lat2[i] = (time.perf_counter() - t0 - arrivals[i]) * 1e3
+        wall2 = time.perf_counter() - t0
+        row["plain"] = {"p50_ms": float(np.percentile(lat2, 50)), "p99_ms": float(np.percentile(lat2, 99)),
+                        "frames_per_s": n / wall2, "mpix_per_s": px / wall2 / 1e6}
+        log(f"serving {lam} fps offered: {row['serving']['frames_per_s']:.0f} fps, p50 {row['serving']['p50_ms']:.2f} "
+            f"p99 {row['serving']['p99_ms']:.2f} ms; plain {row['plain']['frames_per_s']:.0f} fps, "
+            f"p50 {row['plain']['p50_ms']:.1f} p99 {row['plain']['p99_ms']:.1f} ms")
+        results["rates"].append(row)
+    server.close()
+    return results
 
 
 def cpu_baseline(forest, rgb, depth, labels, cfg_kw, sample):

@@ -84,10 +84,13 @@ def _load():
         "cf_forest_train_stats": [P, P, P], "cf_forest_normalize": [P, D, I],
         "cf_forest_get_posteriors": [P, I, P],
         "cf_predict": [P, P, P, I, P, P], "cf_predict_batch": [P, P, P, P, I, P, P],
+        "cf_predict_batch_device": [P, P, P, P, I, P, P],
+        "cf_device_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_device_free": [P, P],
+        "cf_device_to_host": [P, P, P, ctypes.c_size_t],
         "cf_synth_rgbd": [U32, U32, I, I, I, I, D, P, P, P],
         "cf_server_create": [P, P, D, I, ctypes.POINTER(P)],
         "cf_server_submit": [P, P, P, I, I, ctypes.POINTER(ctypes.c_uint64)],
-        "cf_server_wait": [P, ctypes.c_uint64, P, P, ctypes.POINTER(D)],
+        "cf_server_wait": [P, ctypes.c_uint64, P, P, P],
         "cf_server_stats": [P, P, I], "cf_server_destroy": [P],
     }
     for name, args in sig.items():
@@ -429,6 +432,14 @@ class RandomForestImage:
                                             _ptr(labels)))
         return labels, probs
 
+    def predict_batch_device(self, images, indices, probs_ptr, labels_ptr):
+        """Outputs into caller-owned device memory (integer addresses, e.g. torch tensors' data_ptr();
+        0 skips an output): probs [n][C][H][W] float32, labels [n][H][W] uint8."""
+        idx = np.ascontiguousarray(indices, np.int32)
+        self.ctx.check(lib.cf_predict_batch_device(self.ctx.h, self.h, images.h, _ptr(idx), len(idx),
+                                                   ctypes.c_void_p(probs_ptr or None),
+                                                   ctypes.c_void_p(labels_ptr or None)))
+
     def close(self):
         if self.h:
             lib.cf_forest_destroy(self.h)
@@ -451,6 +462,31 @@ def synth_rgbd(seed, width, height, sites, classes, zero_frac, class_seed=4711):
     if st != CF_OK:
         raise CurfilError(st, "cf_synth_rgbd failed")
     return rgb, depth, labels
+
+
+class DeviceBuffer:
+    """Device memory owned by the caller (cf_device_alloc); `ptr` is its address."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, nbytes
+        p = ctypes.c_void_p()
+        ctx.check(lib.cf_device_alloc(ctx.h, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        ctx._adopt(self)
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx.check(lib.cf_device_to_host(self.ctx.h, _ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib.cf_device_free(self.ctx.h, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
 
 
 class Server:
@@ -480,11 +516,12 @@ class Server:
         H, W = self.sizes.pop(rid)
         probs = np.zeros((self.C, H, W), np.float32) if probabilities else None
         lab = np.zeros((H, W), np.uint8) if labels else None
-        lat = ctypes.c_double()
-        st = lib.cf_server_wait(self.h, rid, _ptr(probs), _ptr(lab), ctypes.byref(lat))
+        lat = np.zeros(4, np.float64)
+        st = lib.cf_server_wait(self.h, rid, _ptr(probs), _ptr(lab), _ptr(lat))
         if st != CF_OK:
             raise CurfilError(st, "cf_server_wait failed: " + lib.cf_server_error(self.h).decode())
-        return lab, probs, lat.value
+        self.last_breakdown = lat[1:].copy()
+        return lab, probs, float(lat[0])
 
     def stats(self):
         out = np.zeros(5, np.float64)

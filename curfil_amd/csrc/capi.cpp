@@ -716,6 +716,44 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
     });
 }
 
+int cf_predict_batch_device(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indices, int count, float* d_probs,
+        uint8_t* d_labels) {
+    return guarded(ctx, [&] {
+        require(fo && im && indices, "null argument");
+        require(count > 0, "no frames");
+        for (int i = 0; i < count; i++) require(indices[i] >= 0 && indices[i] < im->count, "frame index out of range");
+        uploadForest(fo);
+        fo->dFrames.resize(count);
+        CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()), fo->C};
+        launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df, d_probs, d_labels);
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_device_alloc(cf_ctx* ctx, size_t bytes, void** ptr) {
+    return guarded(ctx, [&] {
+        require(ptr != nullptr && bytes > 0, "invalid allocation");
+        CF_HIP(hipSetDevice(ctx->device));
+        CF_HIP(hipMalloc(ptr, bytes));
+    });
+}
+
+int cf_device_free(cf_ctx* ctx, void* ptr) {
+    return guarded(ctx, [&] {
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        if (ptr) CF_HIP(hipFree(ptr));
+    });
+}
+
+int cf_device_to_host(cf_ctx* ctx, void* host, const void* device, size_t bytes) {
+    return guarded(ctx, [&] {
+        require(host && device, "null argument");
+        CF_HIP(hipMemcpyAsync(host, device, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int cf_predict(cf_ctx* ctx, cf_forest* fo, cf_images* im, int index, float* probs, uint8_t* labels) {
     return cf_predict_batch(ctx, fo, im, &index, 1, probs, labels);
 }

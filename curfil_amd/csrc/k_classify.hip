@@ -12,7 +12,7 @@
 
 namespace cf {
 
-constexpr int kLockstep = 4;   // trees traversed together by one thread
+constexpr int kLockstep = 1;   // trees traversed together by one thread (1 measured fastest: 2 and 4 raise register use)
 
 // One node record: the feature (16 bytes) and {threshold, child, -, -}.
 struct NodeRec {

@@ -52,6 +52,7 @@ public:
         return p;
     }
     void give(void* p, size_t bytes) {
+        if (!p) return;
         std::lock_guard<std::mutex> g(mu_);
         free_[bytes].push_back(p);
     }
@@ -95,10 +96,13 @@ struct Request {
     Slot* slot = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
     bool finished = false;
-    Clock::time_point submitted, completed;
+    Clock::time_point submitted, admitted, launched, completed;
 };
 
 constexpr int kJobRing = 64;   // job tables in flight
+// classify work (pixels) allowed in unfinished launches: below it ready frames launch at once
+// (latency), above it they gather into larger batches (throughput)
+constexpr double kMaxPendingPixels = 6.0e6;
 
 }  // namespace
 }  // namespace cf
@@ -110,6 +114,7 @@ struct cf_server {
     cf_forest* fo = nullptr;
     size_t budget = 0;
     int maxBatch = 64;
+    // three streams (with the context's own, four: the hardware queues a process gets by default)
     hipStream_t up = nullptr, comp = nullptr, down = nullptr;
     PinnedPool pinned;
     std::mutex mu;
@@ -128,6 +133,7 @@ struct cf_server {
     FrameJob* hJobs = nullptr;
     hipEvent_t jobDone[kJobRing] = {};
     int jobNext = 0;
+    std::deque<std::pair<int, double>> launchedPx;   // unfinished launches: job entry, pixels
     // statistics
     uint64_t framesDone = 0, launches = 0;
     double pixelsDone = 0;
@@ -202,7 +208,7 @@ void cf_server::run() {
                 std::unique_lock<std::mutex> lk(mu);
                 if (queued.empty() && ready.empty() && inflight.empty()) {
                     cvWork.wait(lk, [&] { return stop || !queued.empty(); });
-                } else if (ready.empty() && !inflight.empty()) {
+                } else if (!inflight.empty()) {
                     cvWork.wait_for(lk, std::chrono::microseconds(20));
                 }
                 if (stop && queued.empty() && ready.empty() && inflight.empty()) break;
@@ -212,10 +218,17 @@ void cf_server::run() {
                     if (!s) break;   // arrival order: later frames wait for this one
                     queued.pop_front();
                     r->slot = s;
+                    r->admitted = Clock::now();
                     r->ready = takeEvent();
                     r->done = takeEvent();
                     admit.push_back(r);
                 }
+            }
+            // pinned result buffers are taken at admission, so the device budget also bounds them
+            for (Request* r : admit) {
+                const size_t px = static_cast<size_t>(r->W) * r->H;
+                r->hostProbs = static_cast<float*>(pinned.take(px * fo->C * sizeof(float)));
+                r->hostLabels = static_cast<uint8_t*>(pinned.take(px));
             }
             // upload and integrate the admitted frames
             for (Request* r : admit) {
@@ -226,8 +239,18 @@ void cf_server::run() {
                 CF_HIP(hipEventRecord(r->ready, up));
                 ready.push_back(r);
             }
-            // classify everything admitted, one launch per resolution (at most maxBatch frames each)
-            if (!ready.empty()) {
+            // classify what is ready, one launch per resolution (at most maxBatch frames each), while the
+            // unfinished launches hold less than kMaxPendingPixels of work: under load, ready frames
+            // gather into larger batches instead of queueing many small launches
+            double pendingPx = 0;
+            while (!launchedPx.empty()) {
+                const hipError_t q = hipEventQuery(jobDone[launchedPx.front().first]);
+                if (q == hipErrorNotReady) break;
+                CF_HIP(q);
+                launchedPx.pop_front();
+            }
+            for (const auto& lp : launchedPx) pendingPx += lp.second;
+            if (!ready.empty() && pendingPx < kMaxPendingPixels) {
                 std::map<std::pair<int, int>, std::vector<Request*>> groups;
                 for (Request* r : ready) groups[{r->W, r->H}].push_back(r);
                 ready.clear();
@@ -238,6 +261,7 @@ void cf_server::run() {
                         const int e = jobNext;
                         jobNext = (jobNext + 1) % kJobRing;
                         CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
+                        hipStream_t cs = comp;
                         FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
                         for (int i = 0; i < n; i++) {
                             Request* r = v[b + i];
@@ -245,12 +269,13 @@ void cf_server::run() {
                             hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
                             hj[i].probs = s->probs.get();
                             hj[i].labels = s->labels.get();
-                            CF_HIP(hipStreamWaitEvent(comp, r->ready, 0));
+                            CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
                         }
                         FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
-                        CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, comp));
-                        launchClassifyJobs(comp, dj, n, kv.first.first, kv.first.second, df);
-                        CF_HIP(hipEventRecord(jobDone[e], comp));
+                        CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
+                        launchClassifyJobs(cs, dj, n, kv.first.first, kv.first.second, df);
+                        CF_HIP(hipEventRecord(jobDone[e], cs));
+                        launchedPx.emplace_back(e, static_cast<double>(n) * kv.first.first * kv.first.second);
                         launches++;
                         CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
                         for (int i = 0; i < n; i++) {
@@ -261,6 +286,7 @@ void cf_server::run() {
                                     hipMemcpyDeviceToHost, down));
                             CF_HIP(hipMemcpyAsync(r->hostLabels, s->labels.get(), px, hipMemcpyDeviceToHost, down));
                             CF_HIP(hipEventRecord(r->done, down));
+                            r->launched = Clock::now();
                             inflight.push_back(r);
                         }
                     }
@@ -353,8 +379,6 @@ int cf_server_submit(cf_server* sv, const uint8_t* rgb, const uint16_t* depth, i
         r->hostIn = sv->pinned.take(inputBytes(width, height));
         std::memcpy(r->hostIn, rgb, px * 3);
         std::memcpy(static_cast<uint8_t*>(r->hostIn) + depthOffset(width, height), depth, px * 2);
-        r->hostProbs = static_cast<float*>(sv->pinned.take(px * sv->fo->C * sizeof(float)));
-        r->hostLabels = static_cast<uint8_t*>(sv->pinned.take(px));
         std::lock_guard<std::mutex> g(sv->mu);
         if (sv->stop) return CF_ERR_RUNTIME;
         r->submitted = Clock::now();
@@ -387,7 +411,13 @@ int cf_server_wait(cf_server* sv, uint64_t id, float* probs, uint8_t* labels, do
     const size_t px = static_cast<size_t>(r->W) * r->H;
     if (probs) std::memcpy(probs, r->hostProbs, px * sv->fo->C * sizeof(float));
     if (labels) std::memcpy(labels, r->hostLabels, px);
-    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->completed - r->submitted).count();
+    if (latency_ms) {
+        using ms = std::chrono::duration<double, std::milli>;
+        latency_ms[0] = ms(r->completed - r->submitted).count();
+        latency_ms[1] = ms(r->admitted - r->submitted).count();
+        latency_ms[2] = ms(r->launched - r->admitted).count();
+        latency_ms[3] = ms(r->completed - r->launched).count();
+    }
     sv->pinned.give(r->hostIn, inputBytes(r->W, r->H));
     sv->pinned.give(r->hostProbs, px * sv->fo->C * sizeof(float));
     sv->pinned.give(r->hostLabels, px);

@@ -216,6 +216,19 @@ int cf_predict(cf_ctx* ctx, cf_forest* forest, cf_images* images, int index, flo
 int cf_predict_batch(cf_ctx* ctx, cf_forest* forest, cf_images* images, const int* indices, int count,
         float* probs, uint8_t* labels);
 
+/* the same with outputs in device memory owned by the caller (d_probs
+ * [count][C][H][W] float, d_labels [count][H][W] uint8; either may be NULL):
+ * for pipelines whose next stage runs on the GPU.  Returns when the
+ * outputs are written. */
+int cf_predict_batch_device(cf_ctx* ctx, cf_forest* forest, cf_images* images, const int* indices, int count,
+        float* d_probs, uint8_t* d_labels);
+
+/* device memory for callers without an allocator of their own (for example
+ * for cf_predict_batch_device's outputs); no counterpart in the reference */
+int cf_device_alloc(cf_ctx* ctx, size_t bytes, void** ptr);
+int cf_device_free(cf_ctx* ctx, void* ptr);
+int cf_device_to_host(cf_ctx* ctx, void* host, const void* device, size_t bytes);
+
 /* ---- serving loop ------------------------------------------------------------ */
 /* RandomForestImage::predict (random_forest_image.cpp:258-352) as a service for
  * frames that arrive one at a time, at any resolution: a worker thread admits
@@ -234,7 +247,9 @@ int cf_server_create(cf_ctx* ctx, cf_forest* forest, double budget_mb, int max_b
 int cf_server_submit(cf_server* server, const uint8_t* rgb, const uint16_t* depth_mm, int width, int height,
         uint64_t* id);
 /* blocks until frame `id` is done; probs [C][H][W] and labels [H][W] may be
- * NULL; latency_ms = completion (results in host memory) - submission */
+ * NULL; latency_ms[0..3] (may be NULL): completion (results in host memory)
+ * - submission, and its parts: waiting for admission, admission to the
+ * batch's launch, launch to completion */
 int cf_server_wait(cf_server* server, uint64_t id, float* probs, uint8_t* labels, double* latency_ms);
 /* out[0..4]: frames done, pixels done, classify launches, device MB in use, frames queued */
 int cf_server_stats(cf_server* server, double* out, int n);

@@ -376,3 +376,18 @@ def test_server_results_equal_single_frame_prediction(ctx, config0):
     with pytest.raises(cf.CurfilError):
         cf.Server(ctx, forest, budget_mb=0.01).submit(*frames[0])
 
+
+def test_predict_batch_device_outputs_equal_host_outputs(ctx, config0):
+    rgb, depth, labels, im, ds = config0
+    forest = cf.synth_forest(ctx, im, depth.astype(np.int32), trees=3, depth=8, classes=5, seed=5)
+    forest.normalizeHistograms(0.0, False)
+    idx = [2, 0, 3]
+    hl, hp = forest.predict_batch(im, idx)
+    dp = cf.DeviceBuffer(ctx, 3 * 5 * 120 * 160 * 4)
+    dl = cf.DeviceBuffer(ctx, 3 * 120 * 160)
+    forest.predict_batch_device(im, idx, dp.ptr, dl.ptr)
+    assert np.array_equal(dl.to_host(np.uint8, (3, 120, 160)), hl)
+    assert _same_float_bits(dp.to_host(np.float32, (3, 5, 120, 160)), hp)
+    dp.close()
+    dl.close()
+
