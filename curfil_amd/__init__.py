@@ -83,6 +83,8 @@ def _load():
         "cf_forest_get_root_samples": [P, I, P, I], "cf_forest_get_prior": [P, P],
         "cf_forest_train_stats": [P, P, P], "cf_forest_normalize": [P, D, I],
         "cf_forest_get_posteriors": [P, I, P],
+        "cf_forest_export_tree": [P, I, ctypes.c_char_p, P, P, ctypes.c_char_p],
+        "cf_forest_import": [P, ctypes.POINTER(ctypes.c_char_p), I, P, P, ctypes.POINTER(P)],
         "cf_predict": [P, P, P, I, P, P], "cf_predict_batch": [P, P, P, P, I, P, P],
         "cf_predict_batch_device": [P, P, P, P, I, P, P],
         "cf_device_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_device_free": [P, P],
@@ -109,6 +111,14 @@ lib = _load()
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(ctx, st):
+    """Raise on a failed call; calls made without a context report through cf_last_error(NULL)."""
+    if ctx is not None:
+        ctx.check(st)
+    elif st != CF_OK:
+        raise CurfilError(st, lib.cf_last_error(None).decode())
 
 
 class Context:
@@ -302,6 +312,14 @@ class TrainingConfiguration:
         self.ignored_labels, self.useDepthImages = list(ignored_labels), useDepthImages
         self.horizontalFlipping = horizontalFlipping
 
+    @classmethod
+    def from_c(cls, c):
+        return cls(c.random_seed, c.samples_per_image, c.feature_count, c.min_sample_count, c.max_depth,
+                   c.box_radius, c.region_size, c.thresholds, numThreads=c.num_threads, maxImages=c.max_images,
+                   subsamplingType="pixelUniform" if c.subsampling_pixel_uniform else "classUniform",
+                   ignored_labels=[c.ignored_labels[i] for i in range(c.num_ignored)],
+                   useDepthImages=bool(c.use_depth_images), horizontalFlipping=bool(c.horizontal_flipping))
+
     def to_c(self):
         c = _TrainConfig()
         c.random_seed, c.samples_per_image, c.feature_count = self.randomSeed, self.samplesPerImage, self.featureCount
@@ -321,7 +339,21 @@ class RandomForestImage:
 
     def __init__(self, ctx, h):
         self.ctx, self.h = ctx, h
-        ctx._adopt(self)
+        if ctx is not None:
+            ctx._adopt(self)
+
+    def _check(self, st):
+        _check(self.ctx, st)
+
+    def bind(self, ctx):
+        """Attach a forest made without a context (from_files / from_trees with ctx=None) to the context
+        that predicts with it; the library binds it at its first device call as well."""
+        if self.ctx is None:
+            self.ctx = ctx
+            ctx._adopt(self)
+        elif self.ctx is not ctx:
+            raise ValueError("forest belongs to another context")
+        return self
 
     @classmethod
     def train(cls, ctx, images, configuration, tree_count, num_labels, mode=MODE_BATCHED):
@@ -372,9 +404,49 @@ class RandomForestImage:
         hist = np.ascontiguousarray(np.concatenate([h for _, h in trees]), np.uint32)
         pr = None if prior is None else np.ascontiguousarray(prior, np.uint32)
         h = ctypes.c_void_p()
-        ctx.check(lib.cf_forest_create(ctx.h, len(trees), _ptr(counts), _ptr(nodes), _ptr(hist), num_classes,
-                                       _ptr(pr), ctypes.byref(h)))
+        _check(ctx, lib.cf_forest_create(ctx.h if ctx is not None else None, len(trees), _ptr(counts),
+                                         _ptr(nodes), _ptr(hist), num_classes, _ptr(pr), ctypes.byref(h)))
         return cls(ctx, h)
+
+    @classmethod
+    def from_files(cls, tree_files, ctx=None, histogramBias=0.0, useLabelsPrior=True):
+        """RandomForestImage(treeFiles, deviceIds, accelerationMode, histogramBias, useLabelsPrior)
+        (random_forest_image.cpp:47-107): reads tree<N>.json[.gz] files (RandomTreeImport::readJSON,
+        import.cpp:96-265) and normalizes every tree with its own label prior.  `configuration` holds the
+        first tree's training configuration, `palette` [C][3] the colour of each label."""
+        paths = [os.fsencode(str(p)) for p in tree_files]
+        arr = (ctypes.c_char_p * len(paths))(*paths)
+        c = _TrainConfig()
+        pal = np.zeros((32, 3), np.uint8)
+        h = ctypes.c_void_p()
+        _check(ctx, lib.cf_forest_import(ctx.h if ctx is not None else None, arr, len(paths), ctypes.byref(c),
+                                         _ptr(pal), ctypes.byref(h)))
+        f = cls(ctx, h)
+        f.configuration = TrainingConfiguration.from_c(c)
+        f.palette = pal[:f.num_classes].copy()
+        f.normalizeHistograms(histogramBias, useLabelsPrior)
+        return f
+
+    def export_tree(self, tree, path, configuration, palette=None, folder_training=""):
+        """One tree as JSON, gzip-compressed when `path` ends in .gz (RandomTreeExport::writeJSON,
+        export.cpp:180-313).  palette [C][3] gives the label colours of the histogram keys."""
+        pal = None if palette is None else np.ascontiguousarray(palette, np.uint8).reshape(-1, 3)
+        if pal is not None and len(pal) < self.num_classes:
+            raise ValueError("palette needs a colour per class")
+        c = configuration.to_c()
+        self._check(lib.cf_forest_export_tree(self.h, tree, os.fsencode(str(path)), ctypes.byref(c), _ptr(pal),
+                                              os.fsencode(str(folder_training))))
+
+    def export(self, output_folder, configuration, palette=None, folder_training=""):
+        """Every tree as <output_folder>/tree<N>.json.gz (RandomTreeExport::writeJSON's file names,
+        export.cpp:292-313); returns the paths."""
+        os.makedirs(output_folder, exist_ok=True)
+        paths = []
+        for t in range(self.num_trees):
+            p = os.path.join(str(output_folder), f"tree{t}.json.gz")
+            self.export_tree(t, p, configuration, palette, folder_training)
+            paths.append(p)
+        return paths
 
     @property
     def num_trees(self):
@@ -388,7 +460,7 @@ class RandomForestImage:
         n = lib.cf_forest_tree_size(self.h, t)
         nodes = np.zeros(n, NODE_DTYPE)
         hist = np.zeros((n, self.num_classes), np.uint32)
-        self.ctx.check(lib.cf_forest_get_tree(self.h, t, _ptr(nodes), _ptr(hist)))
+        self._check(lib.cf_forest_get_tree(self.h, t, _ptr(nodes), _ptr(hist)))
         return nodes, hist
 
     def root_samples(self, t):
@@ -408,18 +480,19 @@ class RandomForestImage:
         return {"evaluations": ev.value, "seconds": sec.value, "replays": replays}
 
     def normalizeHistograms(self, histogramBias=0.0, useLabelsPrior=True):
-        self.ctx.check(lib.cf_forest_normalize(self.h, histogramBias, int(useLabelsPrior)))
+        self._check(lib.cf_forest_normalize(self.h, histogramBias, int(useLabelsPrior)))
 
     def posteriors(self, t):
         n = lib.cf_forest_tree_size(self.h, t)
         out = np.zeros((n, self.num_classes), np.float64)
-        self.ctx.check(lib.cf_forest_get_posteriors(self.h, t, _ptr(out)))
+        self._check(lib.cf_forest_get_posteriors(self.h, t, _ptr(out)))
         return out
 
     def predict(self, images, index, probabilities=True):
         C, H, W = self.num_classes, images.height, images.width
         probs = np.zeros((C, H, W), np.float32) if probabilities else None
         labels = np.zeros((H, W), np.uint8)
+        self.bind(images.ctx)
         self.ctx.check(lib.cf_predict(self.ctx.h, self.h, images.h, index, _ptr(probs), _ptr(labels)))
         return labels, probs
 
@@ -428,6 +501,7 @@ class RandomForestImage:
         C, H, W = self.num_classes, images.height, images.width
         probs = np.zeros((len(idx), C, H, W), np.float32) if probabilities else None
         labels = np.zeros((len(idx), H, W), np.uint8)
+        self.bind(images.ctx)
         self.ctx.check(lib.cf_predict_batch(self.ctx.h, self.h, images.h, _ptr(idx), len(idx), _ptr(probs),
                                             _ptr(labels)))
         return labels, probs
@@ -436,6 +510,7 @@ class RandomForestImage:
         """Outputs into caller-owned device memory (integer addresses, e.g. torch tensors' data_ptr();
         0 skips an output): probs [n][C][H][W] float32, labels [n][H][W] uint8."""
         idx = np.ascontiguousarray(indices, np.int32)
+        self.bind(images.ctx)
         self.ctx.check(lib.cf_predict_batch_device(self.ctx.h, self.h, images.h, _ptr(idx), len(idx),
                                                    ctypes.c_void_p(probs_ptr or None),
                                                    ctypes.c_void_p(labels_ptr or None)))
@@ -493,7 +568,7 @@ class Server:
     """Serving loop over a resident forest (cf_server_*): submit frames as they arrive, wait for results."""
 
     def __init__(self, ctx, forest, budget_mb=8192.0, max_batch=64):
-        self.ctx, self.forest = ctx, forest
+        self.ctx, self.forest = ctx, forest.bind(ctx)
         self.C = forest.num_classes
         h = ctypes.c_void_p()
         ctx.check(lib.cf_server_create(ctx.h, forest.h, float(budget_mb), max_batch, ctypes.byref(h)))

@@ -35,6 +35,29 @@ void require(bool ok, const char* msg) {
     if (!ok) throw Error(CF_ERR_ARG, msg);
 }
 
+// message of the last failed call made without a context (host-only forest work)
+thread_local std::string hostError;
+
+// Calls that need no device accept a null context; their errors go to hostError.
+template<class Fn>
+int guardedHost(cf_ctx* ctx, Fn&& fn) {
+    if (ctx) return guarded(ctx, std::forward<Fn>(fn));
+    try {
+        fn();
+        hostError.clear();
+        return CF_OK;
+    } catch (const Error& e) {
+        hostError = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        hostError = "host allocation failed";
+        return CF_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        hostError = e.what();
+        return CF_ERR_RUNTIME;
+    }
+}
+
 void checkSamples(const cf_images* im, const cf_sample* s, int n) {
     for (int i = 0; i < n; i++) {
         if (s[i].image >= static_cast<uint32_t>(im->count)) throw Error(CF_ERR_ARG, "sample image index out of range");
@@ -94,8 +117,14 @@ void singleNodeBins(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n,
 
 namespace cf {
 
+void bindForest(cf_forest* fo, cf_ctx* ctx) {
+    if (!fo->ctx) fo->ctx = ctx;
+    if (fo->ctx != ctx) throw Error(CF_ERR_ARG, "forest belongs to another context");
+}
+
 void uploadForest(cf_forest* fo) {
     if (fo->uploaded) return;
+    if (!fo->ctx) throw Error(CF_ERR_ARG, "forest has no context");
     if (fo->post.size() != fo->trees.size()) throw Error(CF_ERR_RUNTIME, "histograms not normalized");
     // one 32-byte record per node: feature, threshold and the left child (right = left + 1), or
     // -(leaf index)-1 for a leaf, so a traversal step is one record load
@@ -168,7 +197,7 @@ int cf_ctx_destroy(cf_ctx* ctx) {
     return CF_OK;
 }
 
-const char* cf_last_error(const cf_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+const char* cf_last_error(const cf_ctx* ctx) { return ctx ? ctx->error.c_str() : hostError.c_str(); }
 
 int cf_ctx_synchronize(cf_ctx* ctx) {
     return guarded(ctx, [&] { CF_HIP(hipStreamSynchronize(ctx->stream)); });
@@ -572,7 +601,7 @@ int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* im, const cf_train_config
 
 int cf_forest_create(cf_ctx* ctx, int numTrees, const int* numNodes, const cf_tree_node* nodes,
         const uint32_t* histograms, int C, const uint32_t* prior, cf_forest** out) {
-    return guarded(ctx, [&] {
+    return guardedHost(ctx, [&] {
         require(numNodes && nodes && histograms && out, "null argument");
         require(numTrees > 0 && C > 0 && C <= kMaxClasses, "invalid sizes");
         auto fo = std::make_unique<cf_forest>();
@@ -604,6 +633,10 @@ int cf_forest_create(cf_ctx* ctx, int numTrees, const int* numNodes, const cf_tr
 
 int cf_forest_destroy(cf_forest* fo) {
     if (!fo) return CF_ERR_ARG;
+    if (!fo->ctx) {
+        delete fo;
+        return CF_OK;
+    }
     std::lock_guard<std::recursive_mutex> lock(fo->ctx->mu);
     delete fo;
     return CF_OK;
@@ -650,14 +683,16 @@ int cf_forest_train_stats(const cf_forest* fo, double* evaluations, double* seco
 // normalizeHistogram (random_tree.cpp:54-104) for every node.
 int cf_forest_normalize(cf_forest* fo, double bias, int usePrior) {
     if (!fo) return CF_ERR_ARG;
-    return guarded(fo->ctx, [&] {
+    return guardedHost(fo->ctx, [&] {
         const int C = fo->C;
         std::vector<double> w(C);
-        double psum = 0;
-        for (int i = 0; i < C; i++) psum += fo->prior[i];
-        for (int i = 0; i < C; i++) w[i] = fo->prior[i] / psum;
         fo->post.assign(fo->trees.size(), {});
         for (size_t t = 0; t < fo->trees.size(); t++) {
+            // each tree weighs by its own label prior (RandomTree::normalizeHistograms, random_tree.h:822-847)
+            const std::vector<uint32_t>& prior = fo->treePriors.empty() ? fo->prior : fo->treePriors[t];
+            double psum = 0;
+            for (int i = 0; i < C; i++) psum += prior[i];
+            for (int i = 0; i < C; i++) w[i] = prior[i] / psum;
             const TreeBuild& tb = fo->trees[t];
             std::vector<double>& out = fo->post[t];
             out.assign(tb.nodes.size() * C, 0.0);
@@ -681,6 +716,23 @@ int cf_forest_normalize(cf_forest* fo, double bias, int usePrior) {
     });
 }
 
+int cf_forest_export_tree(cf_forest* fo, int tree, const char* path, const cf_train_config* config,
+        const uint8_t* palette, const char* folder_training) {
+    if (!fo) return CF_ERR_ARG;
+    return guardedHost(fo->ctx, [&] {
+        require(path && config, "null argument");
+        writeTreeFile(fo, tree, path, *config, palette, folder_training);
+    });
+}
+
+int cf_forest_import(cf_ctx* ctx, const char* const* paths, int count, cf_train_config* config_out, uint8_t* palette,
+        cf_forest** out) {
+    return guardedHost(ctx, [&] {
+        require(out != nullptr, "null output");
+        *out = readTreeFiles(ctx, paths, count, config_out, palette).release();
+    });
+}
+
 int cf_forest_get_posteriors(const cf_forest* fo, int t, double* out) {
     if (!fo || !out || t < 0 || t >= static_cast<int>(fo->post.size())) return CF_ERR_ARG;
     std::memcpy(out, fo->post[t].data(), fo->post[t].size() * sizeof(double));
@@ -693,6 +745,7 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         require(fo && im && indices, "null argument");
         require(count > 0, "no frames");
         for (int i = 0; i < count; i++) require(indices[i] >= 0 && indices[i] < im->count, "frame index out of range");
+        bindForest(fo, ctx);
         uploadForest(fo);
         const size_t plane = static_cast<size_t>(im->W) * im->H;
         // device staging lives in the forest and only grows: no allocation per call at a steady size
@@ -722,6 +775,7 @@ int cf_predict_batch_device(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int
         require(fo && im && indices, "null argument");
         require(count > 0, "no frames");
         for (int i = 0; i < count; i++) require(indices[i] >= 0 && indices[i] < im->count, "frame index out of range");
+        bindForest(fo, ctx);
         uploadForest(fo);
         fo->dFrames.resize(count);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));

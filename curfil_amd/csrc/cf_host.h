@@ -58,6 +58,9 @@ struct cf_forest {
     std::vector<int> treeIds;
     std::vector<int> rsSeeds;  // RandomSource seed counter of each tree after pixel sampling
     std::vector<uint32_t> prior;
+    // label prior of each tree when the trees were trained apart (imported
+    // tree files carry their own); empty: every tree uses `prior`
+    std::vector<std::vector<uint32_t>> treePriors;
     std::vector<std::vector<double>> post;  // normalized histograms [nodes][C] per tree
     bool uploaded = false;
     cf::DevBuf<cf::SplitRec> dNodes;
@@ -71,6 +74,19 @@ struct cf_forest {
 };
 
 namespace cf {
+// a forest made without a context (imported, or built from node tables) takes
+// the context of its first device call; later calls must use the same one
+void bindForest(cf_forest* fo, cf_ctx* ctx);
 // packs a normalized forest into device node records (once; later calls return at once)
 void uploadForest(cf_forest* fo);
+
+// one tree as a JSON file, gzip-compressed when `path` ends in ".gz"
+// (RandomTreeExport::writeJSON, export.cpp:180-313); palette [C][3] or null
+void writeTreeFile(const cf_forest* fo, int tree, const char* path, const cf_train_config& cfg,
+        const uint8_t* palette, const char* folderTraining);
+// a forest from tree files, plain or gzip (RandomForestImage(treeFiles),
+// random_forest_image.cpp:47-107 with RandomTreeImport::readJSON,
+// import.cpp:96-265); palette receives [C][3] when not null
+std::unique_ptr<cf_forest> readTreeFiles(cf_ctx* ctx, const char* const* paths, int count, cf_train_config* cfgOut,
+        uint8_t* palette);
 }  // namespace cf

@@ -340,6 +340,7 @@ int cf_server_create(cf_ctx* ctx, cf_forest* fo, double budget_mb, int max_batch
         sv->fo = fo;
         sv->budget = static_cast<size_t>(budget_mb * 1024.0 * 1024.0);
         sv->maxBatch = max_batch;
+        bindForest(fo, ctx);
         uploadForest(fo);
         CF_HIP(hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking));
         CF_HIP(hipStreamCreateWithFlags(&sv->comp, hipStreamNonBlocking));

@@ -207,6 +207,25 @@ int cf_forest_normalize(cf_forest* forest, double histogram_bias, int use_labels
 /* leaf posteriors [nodes][C] (double) of one tree, after normalize */
 int cf_forest_get_posteriors(const cf_forest* forest, int tree, double* out);
 
+/* ---- tree files ------------------------------------------------------------- */
+/* one tree as JSON (every value a string, as Boost.PropertyTree writes it),
+ * gzip-compressed when `path` ends in ".gz"; replaces RandomTreeExport::writeJSON
+ * (export.cpp:180-313).  palette [C][3] gives the colour of each label in the
+ * histogram keys "R,G,B (label)"; NULL writes "l,l,l (l)".  Floats are
+ * written with enough digits to read back unchanged. */
+int cf_forest_export_tree(cf_forest* forest, int tree, const char* path, const cf_train_config* config,
+        const uint8_t* palette, const char* folder_training);
+/* a forest from tree files (plain or gzip JSON); replaces
+ * RandomForestImage(treeFiles, ...) (random_forest_image.cpp:47-107) with
+ * RandomTreeImport::readJSON (import.cpp:96-265).  Every tree keeps its own
+ * label prior for cf_forest_normalize.  config_out (may be NULL) receives the
+ * first tree's configuration, palette (may be NULL, else [32][3]) the colour of
+ * each label.  Fails with CF_ERR_ARG on a malformed file, a histogram whose
+ * sum is below the node's sample count, or trees of different configurations
+ * or class counts. */
+int cf_forest_import(cf_ctx* ctx, const char* const* paths, int count, cf_train_config* config_out, uint8_t* palette,
+        cf_forest** out);
+
 /* ---- prediction ------------------------------------------------------------- */
 /* RandomForestImage::predict (random_forest_image.cpp:258-352) with the CPU
  * branch's arithmetic (:307-345): probs [C][H][W] float, labels [H][W] uint8.

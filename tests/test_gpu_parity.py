@@ -391,3 +391,28 @@ def test_predict_batch_device_outputs_equal_host_outputs(ctx, config0):
     dp.close()
     dl.close()
 
+
+
+def test_exported_and_imported_forest_predicts_the_same(ctx, config0, tmp_path):
+    """import_export_test.cpp on the device path: a trained forest written as tree<N>.json.gz and read back
+    (without a context, then bound by its first prediction) has the same trees, and predicts the same labels
+    and posteriors, bit for bit."""
+    rgb, depth, labels, im, ds = config0
+    params = dict(CONFIG0)
+    params.update(samplesPerImage=2000, maxDepth=9, featureCount=150, thresholds=15)
+    cfg = cf.TrainingConfiguration(**params)
+    forest = cf.RandomForestImage.train(ctx, im, cfg, tree_count=3, num_labels=6, mode=cf.MODE_BATCHED)
+    forest.normalizeHistograms(0.0, True)
+    paths = forest.export(tmp_path, cfg, folder_training="synthetic")
+    back = cf.RandomForestImage.from_files(paths, histogramBias=0.0, useLabelsPrior=True)
+    assert back.ctx is None
+    for t in range(3):
+        gn, gh = forest.tree(t)
+        bn, bh = back.tree(t)
+        _assert_trees_equal(bn, bh, gn, gh)
+        assert np.array_equal(back.posteriors(t), forest.posteriors(t))
+    for img in (0, 2):
+        fl, fp = forest.predict(im, img)
+        bl, bp = back.predict(im, img)
+        assert back.ctx is ctx
+        assert np.array_equal(fl, bl) and _same_float_bits(np.nan_to_num(fp), np.nan_to_num(bp))
