@@ -80,7 +80,7 @@ def _load():
         "cf_forest_create": [P, I, P, P, P, I, P, ctypes.POINTER(P)],
         "cf_forest_destroy": [P], "cf_forest_num_trees": [P], "cf_forest_num_classes": [P],
         "cf_forest_tree_size": [P, I], "cf_forest_get_tree": [P, I, P, P],
-        "cf_forest_get_root_samples": [P, I, P, I], "cf_forest_get_prior": [P, P],
+        "cf_forest_get_root_samples": [P, I, P, I], "cf_forest_get_prior": [P, P], "cf_forest_get_tree_prior": [P, I, P],
         "cf_forest_train_stats": [P, P, P], "cf_forest_normalize": [P, D, I],
         "cf_forest_get_posteriors": [P, I, P],
         "cf_forest_export_tree": [P, I, ctypes.c_char_p, P, P, ctypes.c_char_p],
@@ -469,9 +469,13 @@ class RandomForestImage:
         lib.cf_forest_get_root_samples(self.h, t, _ptr(out), n)
         return out
 
-    def prior(self):
+    def prior(self, t=None):
+        """the forest's label prior, or tree t's (its own when it trained on a subset of the images)"""
         out = np.zeros(self.num_classes, np.uint32)
-        lib.cf_forest_get_prior(self.h, _ptr(out))
+        if t is None:
+            lib.cf_forest_get_prior(self.h, _ptr(out))
+        elif lib.cf_forest_get_tree_prior(self.h, t, _ptr(out)) < 0:
+            raise IndexError(t)
         return out
 
     def train_stats(self):

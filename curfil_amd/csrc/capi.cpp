@@ -494,29 +494,32 @@ int cf_forest_sample(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int
     return guarded(ctx, [&] {
         require(im && cfg && out, "null argument");
         require(numTrees > 0, "cannot construct empty forest");
-        if (cfg->subsampling_pixel_uniform) throw Error(CF_ERR_UNSUPPORTED, "pixelUniform subsampling is not implemented");
-        if (cfg->max_images > 0 && cfg->max_images < im->count) {
-            throw Error(CF_ERR_UNSUPPORTED, "maxImages image subsampling is not implemented");
-        }
         auto fo = std::make_unique<cf_forest>();
         fo->ctx = ctx;
         fo->prior = labelPrior(im, numLabels);
         fo->C = static_cast<int>(fo->prior.size());
         // RandomForestImage::train: SEED from the configured seed, tree t uses
-        // RandomSource(SEED + t) (random_forest_image.cpp:150-173).  Trees
+        // RandomSource(SEED + t) (random_forest_image.cpp:150-173): first the tree's
+        // images (maxImages), then its label prior over them and its pixels.  Trees
         // sample independently, so they sample concurrently.
         HostRandomSource master(cfg->random_seed);
         const int SEED = master.uniformSampler(0xFFFF).next();
         fo->roots.resize(numTrees);
         fo->rsSeeds.resize(numTrees);
         fo->treeIds.resize(numTrees);
+        std::vector<std::vector<uint32_t>> priors(numTrees);
         std::vector<std::string> errors(numTrees);
         std::vector<std::thread> threads;
         for (int t = 0; t < numTrees; t++) {
             threads.emplace_back([&, t] {
                 try {
                     HostRandomSource rs(SEED + t);
-                    fo->roots[t] = sampleClassUniform(im, fo->prior, *cfg, rs, cfg->samples_per_image / numTrees);
+                    const std::vector<int> ids = sampleImages(im->count, cfg->max_images, rs);
+                    priors[t] = static_cast<int>(ids.size()) == im->count ? fo->prior : labelPrior(im, ids, numLabels);
+                    const size_t perImage = cfg->samples_per_image / numTrees;
+                    fo->roots[t] = cfg->subsampling_pixel_uniform ? samplePixelUniform(im, ids, *cfg, rs, perImage)
+                                                                  : sampleClassUniform(im, ids, priors[t], *cfg, rs,
+                                                                            perImage);
                     fo->rsSeeds[t] = rs.seed();
                     fo->treeIds[t] = t;
                 } catch (const std::exception& e) {
@@ -528,6 +531,13 @@ int cf_forest_sample(cf_ctx* ctx, cf_images* im, const cf_train_config* cfg, int
         for (const auto& e : errors) {
             if (!e.empty()) throw Error(CF_ERR_RUNTIME, e);
         }
+        // trees that saw different images have different priors (RandomTreeImage keeps its own)
+        bool same = true;
+        for (const auto& p : priors) {
+            if (p.size() != fo->prior.size()) throw Error(CF_ERR_RUNTIME, "different number of classes in trees");
+            same = same && p == fo->prior;
+        }
+        if (!same) fo->treePriors = priors;
         *out = fo.release();
     });
 }
@@ -671,6 +681,15 @@ int cf_forest_get_prior(const cf_forest* fo, uint32_t* prior) {
     if (!fo || !prior) return CF_ERR_ARG;
     std::memcpy(prior, fo->prior.data(), fo->prior.size() * sizeof(uint32_t));
     return static_cast<int>(fo->prior.size());
+}
+
+int cf_forest_get_tree_prior(const cf_forest* fo, int t, uint32_t* prior) {
+    if (!fo || !prior) return CF_ERR_ARG;
+    const size_t numTrees = std::max(fo->trees.size(), fo->roots.size());   // sampled, grown or imported
+    if (t < 0 || static_cast<size_t>(t) >= numTrees) return CF_ERR_ARG;
+    const std::vector<uint32_t>& p = fo->treePriors.empty() ? fo->prior : fo->treePriors[t];
+    std::memcpy(prior, p.data(), p.size() * sizeof(uint32_t));
+    return static_cast<int>(p.size());
 }
 
 int cf_forest_train_stats(const cf_forest* fo, double* evaluations, double* seconds) {

@@ -40,12 +40,23 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         std::vector<TreeJob>& jobs, int mode, std::vector<TreeBuild>& out, TrainStats& stats);
 
 // RandomTreeImage::calculateLabelPriorDistribution (random_tree_image.cpp:885-909)
+// over the images `ids`, or over every image
+std::vector<uint32_t> labelPrior(const cf_images* images, const std::vector<int>& ids, int numLabels);
 std::vector<uint32_t> labelPrior(const cf_images* images, int numLabels);
+
+// the images one tree trains on: a reservoir of maxImages images drawn with the
+// tree's RandomSource, or all (random_forest_image.cpp:161-171)
+std::vector<int> sampleImages(int count, int maxImages, HostRandomSource& rs);
 
 // RandomTreeImage::subsampleTrainingDataClassUniform with one sampling chunk
 // (random_tree_image.cpp:911-1059) followed by the root sort
 // (random_tree_image.cpp:776-802, image index in place of the image address).
-std::vector<cf_sample> sampleClassUniform(const cf_images* images, const std::vector<uint32_t>& prior,
+std::vector<cf_sample> sampleClassUniform(const cf_images* images, const std::vector<int>& ids,
+        const std::vector<uint32_t>& prior, const cf_train_config& cfg, HostRandomSource& rs, size_t subsampleCount);
+
+// RandomTreeImage::subsampleTrainingDataPixelUniform (random_tree_image.cpp:840-883)
+// followed by the same root sort
+std::vector<cf_sample> samplePixelUniform(const cf_images* images, const std::vector<int>& ids,
         const cf_train_config& cfg, HostRandomSource& rs, size_t subsampleCount);
 
 }  // namespace cf

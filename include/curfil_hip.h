@@ -197,6 +197,11 @@ int cf_forest_tree_size(const cf_forest* forest, int tree);
 int cf_forest_get_tree(const cf_forest* forest, int tree, cf_tree_node* nodes, uint32_t* histograms);
 int cf_forest_get_root_samples(const cf_forest* forest, int tree, cf_sample* out, int capacity);
 int cf_forest_get_prior(const cf_forest* forest, uint32_t* prior);
+/* the label prior of one tree (RandomTreeImage::getClassLabelPriorDistribution,
+ * random_tree_image.h:1149-1151): differs from the forest's when the tree
+ * trained on a subset of the images (maxImages) or was read from its own file.
+ * Returns the number of entries. */
+int cf_forest_get_tree_prior(const cf_forest* forest, int tree, uint32_t* prior);
 /* training statistics of the last train call: evaluated feature-threshold
  * pairs, device seconds, per-level timing count */
 int cf_forest_train_stats(const cf_forest* forest, double* evaluations, double* seconds);

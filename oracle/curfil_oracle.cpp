@@ -684,27 +684,105 @@ static int trainTree(const Dataset& ds, int treeId, const std::vector<Sample>& s
     return 0;
 }
 
-// calculateLabelPriorDistribution: random_tree_image.cpp:885-909.
-static std::vector<uint32_t> labelPrior(const Dataset& ds, int numLabels) {
+// calculateLabelPriorDistribution: random_tree_image.cpp:885-909, over the
+// images `ids`.  The loop `for (label = 0; label < map.size(); label++)` reads
+// map[label], which inserts missing labels and so runs up to the largest label.
+static std::vector<uint32_t> labelPrior(const Dataset& ds, const std::vector<int>& ids, int numLabels) {
     std::map<int, size_t> m;
-    for (const auto& im : ds.images) {
+    for (int i : ids) {
+        const Image& im = ds.images[i];
         for (int y = 0; y < im.H; ++y)
             for (int x = 0; x < im.W; ++x) m[im.lab[static_cast<size_t>(y) * im.W + x]]++;
     }
     std::vector<uint32_t> prior(std::max<size_t>(numLabels, m.size()), 0);
-    for (size_t l = 0; l < m.size(); l++) prior[l] = static_cast<uint32_t>(m[static_cast<int>(l)]);
+    for (size_t l = 0; l < m.size(); l++) {
+        if (l >= prior.size()) throw std::runtime_error("label does not fit the classes");
+        prior[l] = static_cast<uint32_t>(m[static_cast<int>(l)]);
+    }
     return prior;
+}
+
+static std::vector<uint32_t> labelPrior(const Dataset& ds, int numLabels) {
+    std::vector<int> ids(ds.images.size());
+    for (size_t i = 0; i < ids.size(); i++) ids[i] = static_cast<int>(i);
+    return labelPrior(ds, ids, numLabels);
+}
+
+// RandomForestImage::train's maxImages subsampling (random_forest_image.cpp:161-171).
+static std::vector<int> sampleImages(int count, int maxImages, RandomSource& rs) {
+    std::vector<int> ids;
+    if (maxImages > 0 && count > maxImages) {
+        Reservoir<int> r(static_cast<size_t>(maxImages));
+        Sampler sampler = rs.uniformSampler(0, 10 * count);
+        for (int i = 0; i < count; i++) r.sample(sampler, i);
+        ids = r.reservoir;
+    } else {
+        for (int i = 0; i < count; i++) ids.push_back(i);
+    }
+    return ids;
+}
+
+// block sort of RandomTreeImage::train (random_tree_image.cpp:776-802), image
+// index instead of image address
+static void sortRoots(std::vector<Sample>& all) {
+    const size_t BLOCK = 200000;
+    for (size_t b = 0; b < all.size(); b += BLOCK) {
+        auto e = std::min(all.size(), b + BLOCK);
+        std::sort(all.begin() + b, all.begin() + e, [](const Sample& a, const Sample& c) {
+            if (a.image != c.image) return a.image < c.image;
+            if (a.label != c.label) return a.label < c.label;
+            if (a.y / 10 != c.y / 10) return a.y / 10 < c.y / 10;
+            if (a.x / 10 != c.x / 10) return a.x / 10 < c.x / 10;
+            return false;
+        });
+    }
+}
+
+// subsampleTrainingDataPixelUniform: random_tree_image.cpp:840-883, then the block sort.
+static int samplePixelUniform(const Dataset& ds, const std::vector<int>& ids, const Config& cfg, RandomSource& rs,
+        size_t subsampleCount, std::vector<Sample>& out) {
+    auto ignored = [&](int l) {
+        for (int i = 0; i < cfg.numIgnored; i++) if (cfg.ignored[i] == l) return true;
+        return false;
+    };
+    const uint8_t flip = cfg.horizontalFlipping ? 2 : 0;
+    Sampler rgenImage = rs.uniformSampler(static_cast<int>(ids.size()));
+    std::vector<Sample> all;
+    for (size_t n = 0; n < subsampleCount * ids.size(); ++n) {
+        const int imageId = ids[rgenImage.getNext()];
+        const Image& im = ds.images[imageId];
+        Sampler rgenX = rs.uniformSampler(im.W);
+        Sampler rgenY = rs.uniformSampler(im.H);
+        for (int tries = 0;; tries++) {
+            if (tries > 100000000) return -5;
+            const int x = rgenX.getNext(), y = rgenY.getNext();
+            const int label = im.lab[static_cast<size_t>(y) * im.W + x];
+            if (ignored(label)) continue;
+            Sample s{};
+            s.x = static_cast<uint16_t>(x);
+            s.y = static_cast<uint16_t>(y);
+            s.image = static_cast<uint32_t>(imageId);
+            s.depth_mm = pixelDepth(im, x, y);
+            s.label = static_cast<uint8_t>(label);
+            s.flip = flip;
+            all.push_back(s);
+            break;
+        }
+    }
+    sortRoots(all);
+    out.swap(all);
+    return 0;
 }
 
 // subsampleTrainingDataClassUniform with one TBB chunk:
 // random_tree_image.cpp:911-1059; then the block sort by
 // (image, label, y/10, x/10) of random_tree_image.cpp:776-802 (image index
 // instead of image address).
-static int sampleClassUniform(const Dataset& ds, const std::vector<uint32_t>& prior, const Config& cfg,
-        RandomSource& rs, size_t subsampleCount, std::vector<Sample>& out) {
+static int sampleClassUniform(const Dataset& ds, const std::vector<int>& ids, const std::vector<uint32_t>& prior,
+        const Config& cfg, RandomSource& rs, size_t subsampleCount, std::vector<Sample>& out) {
     const size_t numLabelsAll = prior.size();
     size_t numLabels = numLabelsAll - static_cast<size_t>(cfg.numIgnored);
-    const size_t spc = static_cast<size_t>(std::ceil(ds.images.size() * static_cast<double>(subsampleCount) /
+    const size_t spc = static_cast<size_t>(std::ceil(ids.size() * static_cast<double>(subsampleCount) /
             static_cast<double>(numLabels)));
     const int randomSeed = rs.uniformSampler(0xFFFFFF).getNext();
     auto ignored = [&](int l) {
@@ -717,7 +795,7 @@ static int sampleClassUniform(const Dataset& ds, const std::vector<uint32_t>& pr
     {
         RandomSource crs(randomSeed);
         Sampler sampler = crs.uniformSampler(0x7FFFFFFF);
-        for (size_t i = 0; i < ds.images.size(); i++) {
+        for (int i : ids) {
             const Image& im = ds.images[i];
             for (int y = 0; y < im.H; y++) {
                 for (int x = 0; x < im.W; x++) {
@@ -747,17 +825,7 @@ static int sampleClassUniform(const Dataset& ds, const std::vector<uint32_t>& pr
         all.insert(all.begin(), merged.reservoir.begin(), merged.reservoir.end());
     }
     if (all.size() != spc * numLabels) return -2;
-    const size_t BLOCK = 200000;
-    for (size_t b = 0; b < all.size(); b += BLOCK) {
-        auto e = std::min(all.size(), b + BLOCK);
-        std::sort(all.begin() + b, all.begin() + e, [](const Sample& a, const Sample& c) {
-            if (a.image != c.image) return a.image < c.image;
-            if (a.label != c.label) return a.label < c.label;
-            if (a.y / 10 != c.y / 10) return a.y / 10 < c.y / 10;
-            if (a.x / 10 != c.x / 10) return a.x / 10 < c.x / 10;
-            return false;
-        });
-    }
+    sortRoots(all);
     out.swap(all);
     return 0;
 }
@@ -911,7 +979,8 @@ int orc_label_prior(void* p, int numLabels, uint32_t* prior, int cap) {
 struct OrcForest {
     std::vector<TreeBuild> trees;
     std::vector<std::vector<Sample>> roots;
-    std::vector<uint32_t> prior;
+    std::vector<uint32_t> prior;                     // over all images
+    std::vector<std::vector<uint32_t>> treePriors;   // over each tree's images
     int C = 0;
 };
 
@@ -919,7 +988,7 @@ void* orc_train_forest(void* p, const Config* cfg, int numTrees, int numLabels, 
     const Dataset& ds = *static_cast<Dataset*>(p);
     auto* fo = new OrcForest();
     *status = 0;
-    if (cfg->horizontalFlipping || cfg->subsamplingPixelUniform || cfg->maxImages > 0) {
+    if (cfg->horizontalFlipping) {
         *status = -3;  // not restated
         return fo;
     }
@@ -929,10 +998,16 @@ void* orc_train_forest(void* p, const Config* cfg, int numTrees, int numLabels, 
     const int SEED = master.uniformSampler(0xFFFF).getNext();
     fo->trees.resize(numTrees);
     fo->roots.resize(numTrees);
+    fo->treePriors.resize(numTrees);
     for (int t = 0; t < numTrees; t++) {
         RandomSource rs(SEED + t);
+        const std::vector<int> ids = sampleImages(static_cast<int>(ds.images.size()), cfg->maxImages, rs);
+        fo->treePriors[t] = labelPrior(ds, ids, numLabels);
+        if (fo->treePriors[t].size() != fo->prior.size()) { *status = -6; return fo; }
         std::vector<Sample> sub;
-        int st = sampleClassUniform(ds, fo->prior, *cfg, rs, cfg->samplesPerImage / numTrees, sub);
+        const size_t perImage = cfg->samplesPerImage / numTrees;
+        int st = cfg->subsamplingPixelUniform ? samplePixelUniform(ds, ids, *cfg, rs, perImage, sub)
+                                              : sampleClassUniform(ds, ids, fo->treePriors[t], *cfg, rs, perImage, sub);
         if (st != 0) { *status = st; return fo; }
         fo->roots[t] = sub;
         st = trainTree(ds, t, sub, fo->C, *cfg, rs, argOrder, fo->trees[t]);
@@ -956,6 +1031,10 @@ void orc_forest_roots(void* f, int t, Sample* out) {
 }
 void orc_forest_prior(void* f, uint32_t* out) {
     const auto& p = static_cast<OrcForest*>(f)->prior;
+    std::memcpy(out, p.data(), p.size() * sizeof(uint32_t));
+}
+void orc_forest_tree_prior(void* f, int t, uint32_t* out) {
+    const auto& p = static_cast<OrcForest*>(f)->treePriors[t];
     std::memcpy(out, p.data(), p.size() * sizeof(uint32_t));
 }
 

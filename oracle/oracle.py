@@ -69,6 +69,7 @@ def _load():
     lib.orc_forest_root_count.argtypes = [P, I]
     lib.orc_forest_roots.argtypes = [P, I, P]
     lib.orc_forest_prior.argtypes = [P, P]
+    lib.orc_forest_tree_prior.argtypes = [P, I, P]
     lib.orc_train_tree.argtypes = [P, P, I, I, I, P, I, I, P, P, P, I]
     lib.orc_predict.argtypes = [P, I, I, P, P, P, I, P, P]
     lib.orc_feature_generation_hash.argtypes = [I, P, P]
@@ -84,8 +85,10 @@ def _p(a):
 
 
 def config(randomSeed, samplesPerImage, featureCount, minSampleCount, maxDepth, boxRadius, regionSize, thresholds,
-           numThreads=1, maxImages=0, useDepthImages=True, horizontalFlipping=False, ignored_labels=()):
+           numThreads=1, maxImages=0, useDepthImages=True, horizontalFlipping=False, ignored_labels=(),
+           subsamplingType="classUniform"):
     c = _Config()
+    c.subsamplingPixelUniform = int(subsamplingType == "pixelUniform")
     c.randomSeed, c.samplesPerImage, c.featureCount = randomSeed, samplesPerImage, featureCount
     c.minSampleCount, c.maxDepth, c.boxRadius = minSampleCount, maxDepth, boxRadius
     c.regionSize, c.thresholds, c.numThreads, c.maxImages = regionSize, thresholds, numThreads, maxImages
@@ -211,9 +214,13 @@ class Forest:
         lib.orc_forest_roots(self.h, t, _p(out))
         return out
 
-    def prior(self):
+    def prior(self, t=None):
+        """the label prior over all images, or over tree t's images (maxImages)"""
         out = np.zeros(self.num_classes, np.uint32)
-        lib.orc_forest_prior(self.h, _p(out))
+        if t is None:
+            lib.orc_forest_prior(self.h, _p(out))
+        else:
+            lib.orc_forest_tree_prior(self.h, t, _p(out))
         return out
 
     def __del__(self):

@@ -416,3 +416,25 @@ def test_exported_and_imported_forest_predicts_the_same(ctx, config0, tmp_path):
         bl, bp = back.predict(im, img)
         assert back.ctx is ctx
         assert np.array_equal(fl, bl) and _same_float_bits(np.nan_to_num(fp), np.nan_to_num(bp))
+
+
+@pytest.mark.parametrize("variant", [dict(subsamplingType="pixelUniform"), dict(maxImages=3),
+                                     dict(maxImages=2, subsamplingType="pixelUniform")])
+def test_sampling_variants_train_like_oracle(ctx, config0, variant):
+    """pixelUniform subsampling (missing-depth pixels included) and maxImages image subsampling: the same
+    root samples, per-tree label priors and trees as the oracle, and each tree normalises with its prior."""
+    rgb, depth, labels, im, ds = config0
+    params = dict(CONFIG0, maxDepth=6, **variant)
+    cfg = cf.TrainingConfiguration(**params)
+    forest = cf.RandomForestImage.train(ctx, im, cfg, tree_count=2, num_labels=6, mode=cf.MODE_BATCHED)
+    of = ds.train_forest(orc.config(**params), 2, 6)
+    assert np.array_equal(forest.prior(), of.prior())
+    forest.normalizeHistograms(0.0, True)
+    for t in range(2):
+        assert np.array_equal(forest.root_samples(t), of.roots(t))
+        assert np.array_equal(forest.prior(t), of.prior(t))
+        gn, gh = forest.tree(t)
+        on, oh = of.tree(t)
+        _assert_trees_equal(gn, gh, on, oh)
+        post = np.array([orc.normalize_histogram(h, of.prior(t), 0.0, True) for h in oh])
+        assert np.array_equal(forest.posteriors(t), post)

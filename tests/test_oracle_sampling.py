@@ -1,0 +1,69 @@
+"""Pixel sampling of the oracle (CPU): pixelUniform subsampling and maxImages image subsampling.
+
+The reference's tests hold no vectors for either (its only end-to-end training test needs PNG input,
+CIELab and a CUDA device), so these functions are "parity unpinned" beyond the invariants below:
+  * pixelUniform (random_tree_image.cpp:840-883): samplesPerImage/trees samples per image of the tree,
+    every label read from the label image at the sample's pixel, no ignored label, invalid depth as
+    Depth::INVALID (-1), the root block sort (random_tree_image.cpp:776-802), images drawn uniformly;
+  * maxImages (random_forest_image.cpp:161-171): a tree samples from at most maxImages images, and its
+    label prior counts exactly the labels of those images (random_tree_image.cpp:885-909);
+  * trees with maxImages unset keep the forest's prior.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from helpers import CONFIG0, config0_dataset
+
+
+@pytest.fixture(scope="module")
+def data():
+    rgb, depth, labels = config0_dataset()
+    return rgb, depth, labels, orc.Dataset.from_u8(rgb, depth, labels)
+
+
+def _sorted_in_blocks(roots):
+    key = (roots["image"].astype(np.int64) << 40) | (roots["label"].astype(np.int64) << 32) | \
+          ((roots["y"] // 10).astype(np.int64) << 16) | (roots["x"] // 10).astype(np.int64)
+    return all(np.all(np.diff(key[b:b + 200000]) >= 0) for b in range(0, len(key), 200000))
+
+
+def test_pixel_uniform_invariants(data):
+    rgb, depth, labels, ds = data
+    params = dict(CONFIG0, maxDepth=2, subsamplingType="pixelUniform")
+    of = ds.train_forest(orc.config(**params), 2, 6)
+    for t in range(2):
+        r = of.roots(t)
+        assert len(r) == (params["samplesPerImage"] // 2) * len(labels)
+        assert np.array_equal(r["label"], labels[r["image"], r["y"], r["x"]])
+        assert not np.any(r["label"] == 0)            # ignored
+        raw = depth[r["image"], r["y"], r["x"]].astype(np.int32)
+        assert np.array_equal(r["depth_mm"], np.where(raw > 0, raw, -1))
+        assert np.any(r["depth_mm"] == -1)            # pixels with missing depth are kept
+        assert _sorted_in_blocks(r)
+        counts = np.bincount(r["image"], minlength=len(labels))
+        assert counts.min() > 0.7 * counts.mean()     # images drawn uniformly
+        assert np.array_equal(of.prior(t), of.prior())
+    assert not np.array_equal(of.roots(0), of.roots(1))
+
+
+def test_max_images_subsets_and_priors(data):
+    rgb, depth, labels, ds = data
+    params = dict(CONFIG0, maxDepth=2, maxImages=2)
+    of = ds.train_forest(orc.config(**params), 3, 6)
+    full = np.bincount(labels.ravel(), minlength=6)
+    assert np.array_equal(of.prior(), full)
+    subsets = []
+    for t in range(3):
+        r = of.roots(t)
+        imgs = np.unique(r["image"])
+        assert len(imgs) == 2
+        subsets.append(tuple(imgs))
+        assert np.array_equal(of.prior(t), np.bincount(labels[imgs].ravel(), minlength=6))
+        # class-uniform over the tree's images: samplesPerClass = ceil(2 * (1000 // 3) / 5)
+        assert len(r) == int(np.ceil(2 * (1000 // 3) / 5)) * 5
+    assert len(set(subsets)) > 1
+    # more images allowed than there are: every image, the forest's prior
+    of = ds.train_forest(orc.config(**dict(params, maxImages=10)), 1, 6)
+    assert np.array_equal(of.prior(0), full)
+    assert len(np.unique(of.roots(0)["image"])) == 4
