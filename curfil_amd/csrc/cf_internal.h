@@ -218,7 +218,7 @@ void launchResponsesPaired(hipStream_t s, const DevImage* imgs, const cf_sample*
 
 // Partition
 void launchPartitionDirections(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
-        int numChunks, const SplitRec* splits, int C, uint8_t* dir, int* chunkLeft, uint32_t* leftHist);
+        int numChunks, const SplitRec* splits, int C, uint8_t* dir, int2* chunkCounts, uint32_t* leftHist);
 void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk* chunks, int numChunks,
         const uint8_t* dir, const int2* dst, cf_sample* out);
 

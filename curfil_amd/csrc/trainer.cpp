@@ -9,7 +9,11 @@
 //   * children ids are ++idNode (left, right), rolled back by 2 when a side is
 //     empty (the node stays a leaf);
 //   * a child keeps growing iff its histogram is not pure and it holds at
-//     least minSampleCount samples.
+//     least minSampleCount samples;
+//   * with horizontal flipping, samples set to Both count twice in the
+//     histograms and are evaluated in both orientations; a Both sample that the
+//     split separates from its flipped self continues as NoFlip plus a Flip
+//     copy on the other side (random_tree.h:1295-1360).
 // The device keeps the level's samples contiguous per node; evaluation runs
 // either node by node (CF_MODE_PLAIN, the reference GPU path's shape: one
 // launch sequence and one host sync per node) or for every node of every
@@ -104,7 +108,6 @@ bool isPure(const uint32_t* h, int C) {
 
 void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int C, std::vector<TreeJob>& jobs,
         int mode, std::vector<TreeBuild>& out, TrainStats& stats) {
-    if (cfg.horizontal_flipping) throw Error(CF_ERR_UNSUPPORTED, "horizontal flipping is not implemented");
     if (cfg.max_depth <= 0) throw Error(CF_ERR_ARG, "maxDepth must be positive");
     if (C < 1 || C > kMaxClasses) throw Error(CF_ERR_ARG, "number of classes must be in [1, 32]");
     if (cfg.feature_count == 0 || cfg.thresholds == 0) throw Error(CF_ERR_ARG, "featureCount and thresholds must be positive");
@@ -116,11 +119,18 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
 
     out.assign(jobs.size(), TreeBuild());
     size_t total = 0;
-    for (auto& j : jobs) total += j.roots.size();
+    bool anyBoth = false;
+    for (auto& j : jobs) {
+        total += j.roots.size();
+        for (const cf_sample& x : j.roots) anyBoth = anyBoth || x.flip == 2;
+    }
+    // a Both sample splits at most once into a NoFlip sample and a Flip copy, which
+    // never split again: a level holds at most twice the root samples
+    const size_t capacity = std::max<size_t>(anyBoth ? 2 * total : total, 1);
     DevBuf<cf_sample> level[2];
-    level[0].resize(std::max<size_t>(total, 1));
-    level[1].resize(std::max<size_t>(total, 1));
-    DevBuf<uint8_t> dir(std::max<size_t>(total, 1));
+    level[0].resize(capacity);
+    level[1].resize(capacity);
+    DevBuf<uint8_t> dir(capacity);
 
     std::vector<ActiveTree> active;
     {
@@ -165,12 +175,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     // node histograms) unless a sample is evaluated in both orientations
     // (one 8-bit bin per sample and feature), T exceeds the 8-bit bins, or
     // CURFIL_EVAL_DIRECT is set.
-    bool twoPass = mode == CF_MODE_BATCHED && T <= kMaxBinThresholds && std::getenv("CURFIL_EVAL_DIRECT") == nullptr;
-    for (const auto& j : jobs) {
-        for (const cf_sample& x : j.roots) {
-            if (x.flip == 2) twoPass = false;
-        }
-    }
+    const bool twoPass = mode == CF_MODE_BATCHED && T <= kMaxBinThresholds && !anyBoth &&
+            std::getenv("CURFIL_EVAL_DIRECT") == nullptr;
 
     FeatTables tabs;
     DevBuf<FeatGenJob> dJobs;
@@ -182,7 +188,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<NodeBest> dBest;
     DevBuf<double> dAllScores;
     DevBuf<SplitRec> dSplits;
-    DevBuf<int> dChunkLeft;
+    DevBuf<int2> dChunkCounts;
     DevBuf<uint32_t> dLeftHist;
     DevBuf<int2> dDst;
     DevBuf<uint32_t> dKeys, dKeys2, dVals, dPerm;
@@ -495,7 +501,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             splits[i] = sr;
         }
         dSplits.resize(numNodes);
-        dChunkLeft.resize(chunks.size());
+        dChunkCounts.resize(chunks.size());
         dLeftHist.resize(static_cast<size_t>(numNodes) * C);
         dChunks.resize(chunks.size());
         CF_HIP(hipMemcpyAsync(dSplits.get(), splits.data(), numNodes * sizeof(SplitRec), hipMemcpyHostToDevice, s));
@@ -503,11 +509,12 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         CF_HIP(hipMemsetAsync(dLeftHist.get(), 0, static_cast<size_t>(numNodes) * C * sizeof(uint32_t), s));
         timer.begin();
         launchPartitionDirections(s, images->desc.get(), level[cur].get(), dChunks.get(), static_cast<int>(chunks.size()),
-                dSplits.get(), C, dir.get(), dChunkLeft.get(), dLeftHist.get());
+                dSplits.get(), C, dir.get(), dChunkCounts.get(), dLeftHist.get());
         timer.end(stats.partitionMs);
-        std::vector<int> chunkLeft(chunks.size());
+        std::vector<int2> chunkCounts(chunks.size());   // left and right entries per chunk
         std::vector<uint32_t> leftHist(static_cast<size_t>(numNodes) * C);
-        CF_HIP(hipMemcpyAsync(chunkLeft.data(), dChunkLeft.get(), chunks.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        CF_HIP(hipMemcpyAsync(chunkCounts.data(), dChunkCounts.get(), chunks.size() * sizeof(int2), hipMemcpyDeviceToHost,
+                s));
         CF_HIP(hipMemcpyAsync(leftHist.data(), dLeftHist.get(), leftHist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         CF_HIP(hipStreamSynchronize(s));
 
@@ -524,9 +531,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             na.begin = nextOff;
             for (size_t k = 0; k < at.nodes.size(); k++, slot++) {
                 const LevelNode& ln = nodes[slot];
-                int nL = 0;
-                for (int c = ln.chunkBegin; c < ln.chunkEnd; c++) nL += chunkLeft[c];
-                const int nR = ln.count - nL;
+                int nL = 0, nR = 0;
+                for (int c = ln.chunkBegin; c < ln.chunkEnd; c++) {
+                    nL += chunkCounts[c].x;
+                    nR += chunkCounts[c].y;
+                }
                 const int leftId = ++at.idNode;
                 const int rightId = ++at.idNode;
                 if (nL == 0 || nR == 0) {
@@ -571,8 +580,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 }
                 int doneL = 0, doneR = 0;
                 for (int c = ln.chunkBegin; c < ln.chunkEnd; c++) {
-                    const int cl = chunkLeft[c];
-                    const int cr = chunks[c].count - cl;
+                    const int cl = chunkCounts[c].x;
+                    const int cr = chunkCounts[c].y;
                     dst[c].x = childBegin[0] >= 0 ? childBegin[0] + doneL : -1;
                     dst[c].y = childBegin[1] >= 0 ? childBegin[1] + doneR : -1;
                     doneL += cl;

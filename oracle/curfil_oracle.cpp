@@ -587,8 +587,8 @@ struct FrontierNode {
 
 // One tree: RandomTreeImage::doTrain (random_tree_image.cpp:714-730) +
 // RandomTreeTrain::train level loop (random_tree.h:1249-1415) +
-// evaluateBestSplits CPU branch (random_tree_image.cpp:239-502).
-// Horizontal flipping is not restated (the oracle rejects it).
+// evaluateBestSplits CPU branch (random_tree_image.cpp:239-502), horizontal
+// flipping included.
 static int trainTree(const Dataset& ds, int treeId, const std::vector<Sample>& subsamples, int C,
         const Config& cfg, RandomSource& rs, int argOrder, TreeBuild& out) {
     out.nodes.clear();
@@ -637,10 +637,22 @@ static int trainTree(const Dataset& ds, int treeId, const std::vector<Sample>& s
             auto& n = frontier[i];
             const Feature f = featureFromSoA(feats.data(), F, best[i].f);
             const float th = thr[static_cast<size_t>(best[i].t) * F + best[i].f];
+            // SplitFunction::split and the partition loop (random_tree.h:122-144, 1295-1360): a Both
+            // sample whose flipped response falls on the other side continues as NoFlip on its side,
+            // and a Flip copy of it joins the other side at the same point of that side's order
             std::vector<Sample> L, R;
             for (const Sample& s : n.samples) {
-                const double r = response(ds.images[s.image], f, s.x, s.y, s.depth_mm, s.flip == 1);
-                if (r <= th) L.push_back(s); else R.push_back(s);
+                const Image& im = ds.images[s.image];
+                const bool left = response(im, f, s.x, s.y, s.depth_mm, s.flip == 1) <= th;
+                if (s.flip == 2 && (response(im, f, s.x, s.y, s.depth_mm, true) <= th) != left) {
+                    Sample self = s, copy = s;
+                    self.flip = 0;
+                    copy.flip = 1;
+                    (left ? L : R).push_back(self);
+                    (left ? R : L).push_back(copy);
+                    continue;
+                }
+                (left ? L : R).push_back(s);
             }
             const int leftId = ++idNode;
             const int rightId = ++idNode;
@@ -988,10 +1000,6 @@ void* orc_train_forest(void* p, const Config* cfg, int numTrees, int numLabels, 
     const Dataset& ds = *static_cast<Dataset*>(p);
     auto* fo = new OrcForest();
     *status = 0;
-    if (cfg->horizontalFlipping) {
-        *status = -3;  // not restated
-        return fo;
-    }
     fo->prior = labelPrior(ds, numLabels);
     fo->C = static_cast<int>(fo->prior.size());
     RandomSource master(cfg->randomSeed);

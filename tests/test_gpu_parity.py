@@ -438,3 +438,22 @@ def test_sampling_variants_train_like_oracle(ctx, config0, variant):
         _assert_trees_equal(gn, gh, on, oh)
         post = np.array([orc.normalize_histogram(h, of.prior(t), 0.0, True) for h in oh])
         assert np.array_equal(forest.posteriors(t), post)
+
+
+@pytest.mark.parametrize("mode", [cf.MODE_PLAIN, cf.MODE_BATCHED])
+def test_horizontal_flipping_trains_like_oracle(ctx, config0, mode):
+    """Horizontal flipping: Both samples evaluated in both orientations and split into NoFlip + Flip
+    copies when a split separates them; trees equal the oracle's node for node."""
+    rgb, depth, labels, im, ds = config0
+    params = dict(CONFIG0, maxDepth=7, horizontalFlipping=True)
+    forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=2, num_labels=6,
+                                        mode=mode)
+    of = ds.train_forest(orc.config(**params), 2, 6)
+    for t in range(2):
+        assert np.array_equal(forest.root_samples(t), of.roots(t))
+        gn, gh = forest.tree(t)
+        on, oh = of.tree(t)
+        _assert_trees_equal(gn, gh, on, oh)
+    grown = [gn["num_samples"][gn["left"][i]] + gn["num_samples"][gn["right"][i]] - gn["num_samples"][i]
+             for i in np.nonzero(gn["is_leaf"] == 0)[0]]
+    assert max(grown) > 0

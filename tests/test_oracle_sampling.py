@@ -7,7 +7,10 @@ CIELab and a CUDA device), so these functions are "parity unpinned" beyond the i
     Depth::INVALID (-1), the root block sort (random_tree_image.cpp:776-802), images drawn uniformly;
   * maxImages (random_forest_image.cpp:161-171): a tree samples from at most maxImages images, and its
     label prior counts exactly the labels of those images (random_tree_image.cpp:885-909);
-  * trees with maxImages unset keep the forest's prior.
+  * trees with maxImages unset keep the forest's prior;
+  * horizontal flipping (random_tree.h:122-144, 521-527, 1295-1360): root samples are Both and count twice
+    in the root histogram; every split's children histograms add up to the parent's; a tree grown with
+    flipping differs from one grown without.
 """
 import numpy as np
 import pytest
@@ -67,3 +70,33 @@ def test_max_images_subsets_and_priors(data):
     of = ds.train_forest(orc.config(**dict(params, maxImages=10)), 1, 6)
     assert np.array_equal(of.prior(0), full)
     assert len(np.unique(of.roots(0)["image"])) == 4
+
+
+def _check_histograms(nodes, hist):
+    for i, n in enumerate(nodes):
+        if n["is_leaf"] == 0:
+            assert np.array_equal(hist[n["left"]] + hist[n["right"]], hist[i]), i
+
+
+def test_horizontal_flipping_invariants(data):
+    rgb, depth, labels, ds = data
+    params = dict(CONFIG0, maxDepth=7, horizontalFlipping=True)
+    of = ds.train_forest(orc.config(**params), 1, 6)
+    r = of.roots(0)
+    assert np.all(r["flip"] == 2)
+    nodes, hist = of.tree(0)
+    assert np.array_equal(hist[0], 2 * np.bincount(r["label"], minlength=6))
+    assert nodes["num_samples"][0] == len(r)
+    _check_histograms(nodes, hist)
+    # a node's histogram weighs at least its samples and at most twice them
+    assert np.all(hist.sum(1) >= nodes["num_samples"]) and np.all(hist.sum(1) <= 2 * nodes["num_samples"])
+    # Both samples separated by a split continue as NoFlip + Flip: children hold more samples than the parent
+    inner = np.nonzero(nodes["is_leaf"] == 0)[0]
+    assert len(inner) > 3
+    grown = [nodes["num_samples"][nodes["left"][i]] + nodes["num_samples"][nodes["right"][i]] - nodes["num_samples"][i]
+             for i in inner]
+    assert min(grown) >= 0 and max(grown) > 0
+    plain = ds.train_forest(orc.config(**dict(params, horizontalFlipping=False)), 1, 6)
+    pn, ph = plain.tree(0)
+    _check_histograms(pn, ph)
+    assert len(pn) != len(nodes) or not np.array_equal(pn["feature_id"], nodes["feature_id"])
