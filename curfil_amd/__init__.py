@@ -90,6 +90,15 @@ def _load():
         "cf_device_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_device_free": [P, P],
         "cf_device_to_host": [P, P, P, ctypes.c_size_t],
         "cf_synth_rgbd": [U32, U32, I, I, I, I, D, P, P, P],
+        "cf_images_set_rgbd_u8_padded": [P, I, P, P, I, I],
+        "cf_label_map_create": [ctypes.POINTER(P)], "cf_label_map_destroy": [P],
+        "cf_label_map_encode": [P, P, ctypes.POINTER(I)], "cf_label_map_find": [P, P],
+        "cf_label_map_size": [P], "cf_label_map_colors": [P, P],
+        "cf_dataset_load": [P, I, I, I, P, P, I, ctypes.POINTER(P)], "cf_dataset_destroy": [P],
+        "cf_dataset_info": [P, P, P, P, P], "cf_dataset_image_size": [P, I, P, P],
+        "cf_dataset_get": [P, I, P, P, P], "cf_dataset_to_images": [P, P, ctypes.POINTER(P)],
+        "cf_png_read_info": [ctypes.c_char_p, P, P, P, P], "cf_png_read": [ctypes.c_char_p, P, ctypes.c_size_t],
+        "cf_png_write": [ctypes.c_char_p, I, I, I, I, P],
         "cf_server_create": [P, P, D, I, ctypes.POINTER(P)],
         "cf_server_submit": [P, P, P, I, I, ctypes.POINTER(ctypes.c_uint64)],
         "cf_server_wait": [P, ctypes.c_uint64, P, P, P],
@@ -166,12 +175,22 @@ class Context:
 class Images:
     """A set of equally sized RGB-D images with device-resident integrals."""
 
-    def __init__(self, ctx, count, width, height):
+    def __init__(self, ctx, count, width, height, _handle=None):
         self.ctx, self.count, self.width, self.height = ctx, count, width, height
-        h = ctypes.c_void_p()
-        ctx.check(lib.cf_images_create(ctx.h, count, width, height, ctypes.byref(h)))
+        h = _handle
+        if h is None:
+            h = ctypes.c_void_p()
+            ctx.check(lib.cf_images_create(ctx.h, count, width, height, ctypes.byref(h)))
         self.h = h
         ctx._adopt(self)
+
+    def set_rgbd_u8_padded(self, index, rgb, depth):
+        """an image smaller than the set, padded as loadImages pads integrated images"""
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        depth = np.ascontiguousarray(depth, dtype=np.uint16)
+        h, w = depth.shape
+        assert rgb.shape == (h, w, 3)
+        self.ctx.check(lib.cf_images_set_rgbd_u8_padded(self.h, index, _ptr(rgb), _ptr(depth), w, h))
 
     def set_rgbd_u8(self, index, rgb, depth):
         rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
@@ -529,6 +548,154 @@ class RandomForestImage:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- datasets (PNG files)
+
+def parse_color(text):
+    """RGBColor(colorString) (image.cpp:66-91): "r,g,b" with every value in 0..255."""
+    parts = str(text).split(",")
+    try:
+        if len(parts) != 3:
+            raise ValueError("expected three values")
+        rgb = tuple(int(p) for p in parts)
+    except ValueError as e:
+        raise ValueError(f"illegal color: '{text}': {e}") from None
+    if any(v < 0 or v > 255 for v in rgb):
+        raise ValueError(f"illegal color: '{text}': color value out of range [0-255]: {text}")
+    return rgb
+
+
+class LabelMap:
+    """The colour -> label map of ground-truth images (LabelImage::encodeColor / decodeLabel,
+    image.cpp:656-734): labels in the order colours are met; shared by training and test sets."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(None, lib.cf_label_map_create(ctypes.byref(h)))
+        self.h = h
+
+    def encode(self, rgb):
+        c = np.ascontiguousarray(rgb, np.uint8)
+        out = ctypes.c_int()
+        _check(None, lib.cf_label_map_encode(self.h, _ptr(c), ctypes.byref(out)))
+        return out.value
+
+    def find(self, rgb):
+        r = lib.cf_label_map_find(self.h, _ptr(np.ascontiguousarray(rgb, np.uint8)))
+        return None if r < 0 else r
+
+    def __len__(self):
+        return lib.cf_label_map_size(self.h)
+
+    def colors(self):
+        out = np.zeros((max(len(self), 1), 3), np.uint8)
+        lib.cf_label_map_colors(self.h, _ptr(out))
+        return out[:len(self)]
+
+    def close(self):
+        if self.h:
+            lib.cf_label_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def list_image_filenames(folder):
+    """listImageFilenames (image.cpp:844-866): every file under `folder` whose name contains
+    '_colors.png', recursively, sorted by path."""
+    out = []
+    for root, _, files in os.walk(folder):
+        out.extend(os.path.join(root, f) for f in files if "_colors.png" in f)
+    return sorted(out)
+
+
+class Dataset:
+    """Images loaded from `*_colors.png` / `*_depth.png` / `*_ground_truth.png` triples."""
+
+    def __init__(self, h, label_map, files):
+        self.h, self.label_map, self.files = h, label_map, list(files)
+        c, w, hh, pad = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.cf_dataset_info(h, ctypes.byref(c), ctypes.byref(w), ctypes.byref(hh), ctypes.byref(pad))
+        self.count, self.width, self.height, self.padding_label = c.value, w.value, hh.value, pad.value
+
+    @classmethod
+    def load(cls, color_files, useDepthImages=True, useDepthFilling=False, padding_color=None, label_map=None,
+             threads=0):
+        label_map = label_map if label_map is not None else LabelMap()
+        files = [os.fsencode(str(f)) for f in color_files]
+        arr = (ctypes.c_char_p * len(files))(*files)
+        pc = None if padding_color is None else np.ascontiguousarray(padding_color, np.uint8)
+        h = ctypes.c_void_p()
+        _check(None, lib.cf_dataset_load(arr, len(files), int(useDepthImages), int(useDepthFilling), _ptr(pc),
+                                         label_map.h, threads, ctypes.byref(h)))
+        return cls(h, label_map, color_files)
+
+    def image_size(self, index):
+        w, h = ctypes.c_int(), ctypes.c_int()
+        _check(None, lib.cf_dataset_image_size(self.h, index, ctypes.byref(w), ctypes.byref(h)))
+        return w.value, h.value
+
+    def get(self, index):
+        """rgb [h][w][3] u8, depth [h][w] u16 mm (after filling), labels [h][w] u8, at the image's own size"""
+        w, h = self.image_size(index)
+        rgb = np.zeros((h, w, 3), np.uint8)
+        depth = np.zeros((h, w), np.uint16)
+        labels = np.zeros((h, w), np.uint8)
+        _check(None, lib.cf_dataset_get(self.h, index, _ptr(rgb), _ptr(depth), _ptr(labels)))
+        return rgb, depth, labels
+
+    def to_images(self, ctx):
+        """all images on the device at the largest size, labels set"""
+        h = ctypes.c_void_p()
+        ctx.check(lib.cf_dataset_to_images(self.h, ctx.h, ctypes.byref(h)))
+        return Images(ctx, self.count, self.width, self.height, _handle=h)
+
+    def close(self):
+        if self.h:
+            lib.cf_dataset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def load_images(folder, useCIELab=False, useDepthImages=True, useDepthFilling=False, ignoredColors=(),
+                label_map=None, threads=0):
+    """loadImages(folder, useCIELab, useDepthImages, useDepthFilling, ignoredColors, numLabels)
+    (image.cpp:869-965).  Returns the Dataset; its label_map holds the colours met so far, and
+    len(label_map) is the reference's numLabels."""
+    if useCIELab:
+        raise CurfilError(CF_ERR_UNSUPPORTED, "CIELab conversion is not available (VIGRA's RGB2LabFunctor)")
+    files = list_image_filenames(folder)
+    if not files:
+        raise CurfilError(CF_ERR_ARG, f"no '*_colors.png' images in {folder}")
+    padding = parse_color(ignoredColors[0]) if ignoredColors else (0, 0, 0)
+    return Dataset.load(files, useDepthImages, useDepthFilling, padding, label_map, threads)
+
+
+def read_png(path):
+    """PNG samples as stored: [h][w] for one channel, else [h][w][channels]; uint8 for 8-bit, uint16 for 16-bit"""
+    w, h, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    p = os.fsencode(str(path))
+    _check(None, lib.cf_png_read_info(p, ctypes.byref(w), ctypes.byref(h), ctypes.byref(c), ctypes.byref(d)))
+    out = np.zeros((h.value, w.value, c.value), np.uint16)
+    _check(None, lib.cf_png_read(p, _ptr(out), out.size))
+    out = out.astype(np.uint8) if d.value == 8 else out
+    return out[:, :, 0] if c.value == 1 else out
+
+
+def write_png(path, pixels):
+    """[h][w][3] uint8 RGB, or [h][w] uint8 / uint16 grayscale"""
+    a = np.ascontiguousarray(pixels)
+    if a.ndim == 3 and a.shape[2] == 3 and a.dtype == np.uint8:
+        ch, bits = 3, 8
+    elif a.ndim == 2 and a.dtype in (np.uint8, np.uint16):
+        ch, bits = 1, 8 * a.dtype.itemsize
+    else:
+        raise ValueError("write_png takes [h][w][3] uint8 or [h][w] uint8/uint16")
+    _check(None, lib.cf_png_write(os.fsencode(str(path)), a.shape[1], a.shape[0], ch, bits, _ptr(a)))
 
 
 def synth_rgbd(seed, width, height, sites, classes, zero_frac, class_seed=4711):

@@ -28,6 +28,8 @@ SOURCES = [
     "synth.cpp",
     "server.cpp",
     "tree_json.cpp",
+    "png_io.cpp",
+    "dataset.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

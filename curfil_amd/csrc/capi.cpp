@@ -35,13 +35,12 @@ void require(bool ok, const char* msg) {
     if (!ok) throw Error(CF_ERR_ARG, msg);
 }
 
-// message of the last failed call made without a context (host-only forest work)
-thread_local std::string hostError;
 
 // Calls that need no device accept a null context; their errors go to hostError.
 template<class Fn>
 int guardedHost(cf_ctx* ctx, Fn&& fn) {
     if (ctx) return guarded(ctx, std::forward<Fn>(fn));
+    std::string& hostError = hostErrorSlot();
     try {
         fn();
         hostError.clear();
@@ -116,6 +115,11 @@ void singleNodeBins(cf_ctx* ctx, cf_images* im, const cf_sample* samples, int n,
 }  // namespace
 
 namespace cf {
+
+std::string& hostErrorSlot() {
+    thread_local std::string message;
+    return message;
+}
 
 void bindForest(cf_forest* fo, cf_ctx* ctx) {
     if (!fo->ctx) fo->ctx = ctx;
@@ -197,7 +201,7 @@ int cf_ctx_destroy(cf_ctx* ctx) {
     return CF_OK;
 }
 
-const char* cf_last_error(const cf_ctx* ctx) { return ctx ? ctx->error.c_str() : hostError.c_str(); }
+const char* cf_last_error(const cf_ctx* ctx) { return ctx ? ctx->error.c_str() : hostErrorSlot().c_str(); }
 
 int cf_ctx_synchronize(cf_ctx* ctx) {
     return guarded(ctx, [&] { CF_HIP(hipStreamSynchronize(ctx->stream)); });
@@ -263,6 +267,40 @@ int cf_images_set_rgbd_u8(cf_images* im, int index, const uint8_t* rgb, const ui
                 im->col.get() + static_cast<size_t>(index) * 3 * px, im->dep.get() + static_cast<size_t>(index) * px);
         int32_t* hd = im->depthHost.data() + static_cast<size_t>(index) * px;
         for (size_t p = 0; p < px; p++) hd[p] = depth[p];
+        im->depthSet[index] = 1;
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cf_images_set_rgbd_u8_padded(cf_images* im, int index, const uint8_t* rgb, const uint16_t* depth, int width,
+        int height) {
+    if (!im) return CF_ERR_ARG;
+    if (width == im->W && height == im->H) return cf_images_set_rgbd_u8(im, index, rgb, depth);
+    return guarded(im->ctx, [&] {
+        require(index >= 0 && index < im->count, "image index out of range");
+        require(rgb && depth, "null input");
+        require(width > 0 && height > 0 && width <= im->W && height <= im->H, "image larger than the set's size");
+        cf_ctx* ctx = im->ctx;
+        const size_t px = static_cast<size_t>(width) * height;
+        const size_t slot = static_cast<size_t>(im->W) * im->H;
+        ctx->scratch0.resize(px * 3 + px * 2 + 64);
+        ctx->scratch1.resize(px * 5 * sizeof(uint32_t));
+        ctx->scratch2.resize(px * 3 * sizeof(float) + px * sizeof(int2) + 256);
+        uint8_t* dRgb = ctx->scratch0.get();
+        uint16_t* dDepth = reinterpret_cast<uint16_t*>(ctx->scratch0.get() + ((px * 3 + 15) / 16) * 16);
+        float* tCol = reinterpret_cast<float*>(ctx->scratch2.get());
+        int2* tDep = reinterpret_cast<int2*>(ctx->scratch2.get() + ((px * 3 * sizeof(float) + 255) / 256) * 256);
+        CF_HIP(hipMemcpyAsync(dRgb, rgb, px * 3, hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dDepth, depth, px * 2, hipMemcpyHostToDevice, ctx->stream));
+        launchIntegralU8(ctx->stream, dRgb, dDepth, width, height, reinterpret_cast<uint32_t*>(ctx->scratch1.get()),
+                tCol, tDep);
+        launchIntegralPad(ctx->stream, tCol, tDep, width, height, im->W, im->H,
+                im->col.get() + static_cast<size_t>(index) * 3 * slot, im->dep.get() + static_cast<size_t>(index) * slot);
+        int32_t* hd = im->depthHost.data() + static_cast<size_t>(index) * slot;
+        std::fill(hd, hd + slot, 0);
+        for (int y = 0; y < height; y++) {
+            for (int x = 0; x < width; x++) hd[static_cast<size_t>(y) * im->W + x] = depth[static_cast<size_t>(y) * width + x];
+        }
         im->depthSet[index] = 1;
         CF_HIP(hipStreamSynchronize(ctx->stream));
     });

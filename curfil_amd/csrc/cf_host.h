@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "cf_internal.h"
@@ -85,6 +86,19 @@ struct cf_forest {
 };
 
 namespace cf {
+// the message of the last failed call made without a context, per host thread
+std::string& hostErrorSlot();
+
+// a decoded PNG: samples as stored (palette images expanded to RGB), row-major,
+// interleaved; 8-bit images hold values 0..255
+struct PngImage {
+    int width = 0, height = 0, channels = 0, bitDepth = 0, colorType = 0;
+    std::vector<uint16_t> samples;
+};
+PngImage readPng(const std::string& path);
+// 8-bit RGB (pixels [H][W][3] u8) or 8/16-bit grayscale ([H][W] u8 / u16)
+void writePng(const std::string& path, int width, int height, int channels, int bitDepth, const void* pixels);
+
 // a forest made without a context (imported, or built from node tables) takes
 // the context of its first device call; later calls must use the same one
 void bindForest(cf_forest* fo, cf_ctx* ctx);

@@ -114,6 +114,9 @@ void launchIntegralU8(hipStream_t s, const uint8_t* rgb, const uint16_t* depth, 
         float* col, int2* dep);
 void launchIntegralF32(hipStream_t s, const float* color, const int32_t* depth, int W, int H, uint32_t* tmp,
         double* work, float* col, int2* dep);
+// integral of a w x h image (srcCol [3][h][w], srcDep [h][w]) into a W x H slot, NaN / 0 outside
+void launchIntegralPad(hipStream_t s, const float* srcCol, const int2* srcDep, int w, int h, int W, int H, float* col,
+        int2* dep);
 
 // Feature generation job of one (tree, level).
 struct FeatGenJob {

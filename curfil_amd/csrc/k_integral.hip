@@ -141,4 +141,26 @@ void launchIntegralF32(hipStream_t s, const float* color, const int32_t* depth, 
     hipLaunchKernelGGL(k_integral_rows, dim3((jobs * 64 + 255) / 256), dim3(256), 0, s, tmp, W, H, 3, col, dep);
 }
 
+// An image smaller than its set's slot, integrated at its own size and then
+// padded the way loadImages pads integrated images (RGBDImage::resizeImage,
+// image.cpp:449-475): colour NaN and depth sums / valid counts 0 outside it.
+__global__ void __launch_bounds__(256) k_integral_pad(const float* __restrict__ srcCol,
+        const int2* __restrict__ srcDep, int w, int h, int W, int H, float* __restrict__ col, int2* __restrict__ dep) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const bool inside = x < w && y < h;
+    const size_t plane = static_cast<size_t>(W) * H, src = static_cast<size_t>(y) * w + x;
+    const size_t dst = static_cast<size_t>(y) * W + x;
+    const float nan = __int_as_float(0x7fc00000);
+#pragma unroll
+    for (int c = 0; c < 3; c++) col[c * plane + dst] = inside ? srcCol[c * static_cast<size_t>(w) * h + src] : nan;
+    dep[dst] = inside ? srcDep[src] : make_int2(0, 0);
+}
+
+void launchIntegralPad(hipStream_t s, const float* srcCol, const int2* srcDep, int w, int h, int W, int H, float* col,
+        int2* dep) {
+    hipLaunchKernelGGL(k_integral_pad, dim3((W + 255) / 256, H), dim3(256), 0, s, srcCol, srcDep, w, h, W, H, col, dep);
+}
+
 }  // namespace cf

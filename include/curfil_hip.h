@@ -90,6 +90,8 @@ enum {
 /* replaces ImageFeatureEvaluation::initDevice / selectDevice (random_tree_image_gpu.cu:1722-1760) */
 int cf_ctx_create(int device, cf_ctx** out);
 int cf_ctx_destroy(cf_ctx* ctx);
+/* the message of the last failure on ctx; with NULL, of the last failure of a
+ * call made without a context on this host thread (tree files, datasets, PNG) */
 const char* cf_last_error(const cf_ctx* ctx);
 int cf_ctx_synchronize(cf_ctx* ctx);
 
@@ -108,6 +110,12 @@ int cf_images_set_rgbd_u8(cf_images* images, int index, const uint8_t* rgb, cons
  * colour integral follows the reference's sequential double Kahan recurrence
  * exactly (image.cpp:236-277). */
 int cf_images_set_rgbd_f32(cf_images* images, int index, const float* color, const int32_t* depth_mm);
+/* an image smaller than the set (width <= W, height <= H) as loadImages pads
+ * it: integrated at its own size, then colour NaN and depth 0 outside it
+ * (RGBDImage::resizeImage on integrated images, image.cpp:449-475 via
+ * image.cpp:945-960).  Equal sizes: cf_images_set_rgbd_u8. */
+int cf_images_set_rgbd_u8_padded(cf_images* images, int index, const uint8_t* rgb, const uint16_t* depth_mm,
+        int width, int height);
 /* label image [H][W] (host copy, read by pixel sampling) */
 int cf_images_set_labels(cf_images* images, int index, const uint8_t* labels);
 /* read back the integrals of one image: color [3][H][W], depth sum and valid count [H][W] */
@@ -279,6 +287,54 @@ int cf_server_wait(cf_server* server, uint64_t id, float* probs, uint8_t* labels
 int cf_server_stats(cf_server* server, double* out, int n);
 const char* cf_server_error(cf_server* server);
 int cf_server_destroy(cf_server* server);
+
+/* ---- datasets (PNG files) --------------------------------------------------- */
+/* The colour -> label map of the ground-truth images (the reference's
+ * process-wide `colors` map with LabelImage::encodeColor / decodeLabel,
+ * image.cpp:656-734): labels are given in the order colours are met.  Training
+ * and test sets, and imported trees' colours, share one map. */
+typedef struct cf_label_map cf_label_map;
+int cf_label_map_create(cf_label_map** out);
+int cf_label_map_destroy(cf_label_map* map);
+/* the label of a colour, added as the next label if new */
+int cf_label_map_encode(cf_label_map* map, const uint8_t* rgb, int* label);
+/* the label of a colour, or -1 */
+int cf_label_map_find(const cf_label_map* map, const uint8_t* rgb);
+int cf_label_map_size(const cf_label_map* map);
+/* colours by label [size][3]; returns the size */
+int cf_label_map_colors(const cf_label_map* map, uint8_t* rgb);
+
+/* loadImages (image.cpp:869-965) over the given `*_colors.png` files (the
+ * caller lists them as listImageFilenames does, image.cpp:844-866): each with
+ * its `*_depth.png` (16-bit grayscale mm; skipped, depth 1000 everywhere, when
+ * use_depth_images is 0; the reference's checks of values and average depth)
+ * and `*_ground_truth.png` (RGB colours -> labels through `map`, column by
+ * column), depth filling when asked (image.cpp:279-377).  The padding colour
+ * (the first ignored colour, NULL = black) is encoded first, as getPaddingLabel
+ * does.  Files are decoded on `threads` host threads (0 = 8); labels are
+ * encoded in file order.  Errors (missing files, non-RGB colour images, a bad
+ * depth image) fail the call; cf_last_error(NULL) has the message. */
+typedef struct cf_dataset cf_dataset;
+int cf_dataset_load(const char* const* color_files, int count, int use_depth_images, int use_depth_filling,
+        const uint8_t* padding_color, cf_label_map* map, int threads, cf_dataset** out);
+int cf_dataset_destroy(cf_dataset* dataset);
+/* number of images, the largest width and height, the padding label */
+int cf_dataset_info(const cf_dataset* dataset, int* count, int* width, int* height, int* padding_label);
+/* an image's own size */
+int cf_dataset_image_size(const cf_dataset* dataset, int index, int* width, int* height);
+/* an image at its own size: rgb [h][w][3], depth [h][w] (after filling), labels [h][w]; any may be NULL */
+int cf_dataset_get(const cf_dataset* dataset, int index, uint8_t* rgb, uint16_t* depth_mm, uint8_t* labels);
+/* every image on the device at the largest size (smaller images padded as
+ * loadImages pads them, labels padded with the padding label) */
+int cf_dataset_to_images(const cf_dataset* dataset, cf_ctx* ctx, cf_images** out);
+
+/* PNG files (VIGRA importImage / exportImage in the reference).  read_info:
+ * size, channels (1 gray, 2 gray+alpha, 3 RGB incl. palette images, 4 RGBA)
+ * and bit depth (8 or 16); read: samples [h][w][channels] as uint16. */
+int cf_png_read_info(const char* path, int* width, int* height, int* channels, int* bit_depth);
+int cf_png_read(const char* path, uint16_t* samples, size_t capacity);
+/* 8-bit RGB (pixels [h][w][3] u8) or 8/16-bit grayscale ([h][w] u8 / u16) */
+int cf_png_write(const char* path, int width, int height, int channels, int bit_depth, const void* pixels);
 
 /* ---- synthetic workload (benchmark / tests; no counterpart in the reference) -- */
 /* One labelled RGB-D frame: Voronoi labels (site k -> label k mod (classes+1),

@@ -457,3 +457,51 @@ def test_horizontal_flipping_trains_like_oracle(ctx, config0, mode):
     grown = [gn["num_samples"][gn["left"][i]] + gn["num_samples"][gn["right"][i]] - gn["num_samples"][i]
              for i in np.nonzero(gn["is_leaf"] == 0)[0]]
     assert max(grown) > 0
+
+
+def test_png_dataset_on_device_matches_arrays(ctx, config0, tmp_path):
+    """loadImages from PNG files: equal-size images give the same integrals, labels and trained trees as the
+    same frames set from arrays; a smaller image is padded as the reference pads integrated images (its own
+    integral in the corner, colour NaN and depth 0 outside)."""
+    rgb, depth, labels, im, ds = config0
+    pal = np.array([[0, 0, 0], [200, 10, 10], [10, 200, 10], [10, 10, 200], [250, 250, 0], [0, 128, 255]],
+                   np.uint8)
+    for i in range(4):
+        cf.write_png(tmp_path / f"f{i}_colors.png", rgb[i])
+        cf.write_png(tmp_path / f"f{i}_depth.png", depth[i])
+        cf.write_png(tmp_path / f"f{i}_ground_truth.png", pal[labels[i]])
+    lm = cf.LabelMap()
+    for c in pal:                      # colours known beforehand keep their labels
+        lm.encode(c)
+    data = cf.load_images(str(tmp_path), ignoredColors=["0,0,0"], label_map=lm)
+    dim = data.to_images(ctx)
+    for i in range(4):
+        a, b = dim.integrals(i), im.integrals(i)
+        assert all(_same_float_bits(x, y) for x, y in zip(a, b))
+    params = dict(CONFIG0, maxDepth=6)
+    f1 = cf.RandomForestImage.train(ctx, dim, cf.TrainingConfiguration(**params), tree_count=1, num_labels=6)
+    f2 = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=1, num_labels=6)
+    _assert_trees_equal(*f1.tree(0), *f2.tree(0))
+
+    small = tmp_path / "small"
+    small.mkdir()
+    cf.write_png(small / "s_colors.png", rgb[0][:50, :70])
+    cf.write_png(small / "s_depth.png", depth[0][:50, :70])
+    cf.write_png(small / "s_ground_truth.png", pal[labels[0][:50, :70]])
+    cf.write_png(small / "t_colors.png", rgb[1])
+    cf.write_png(small / "t_depth.png", depth[1])
+    cf.write_png(small / "t_ground_truth.png", pal[labels[1]])
+    mixed = cf.load_images(str(small), ignoredColors=["0,0,0"], label_map=lm)
+    assert (mixed.width, mixed.height) == (160, 120)
+    mim = mixed.to_images(ctx)
+    col, dep, val = mim.integrals(0)
+    own = cf.Images(ctx, 1, 70, 50)
+    own.set_rgbd_u8(0, rgb[0][:50, :70], depth[0][:50, :70])
+    ocol, odep, oval = own.integrals(0)
+    assert _same_float_bits(col[:, :50, :70], ocol)
+    assert np.array_equal(dep[:50, :70], odep) and np.array_equal(val[:50, :70], oval)
+    assert np.isnan(col[:, 50:, :]).all() and np.isnan(col[:, :, 70:]).all()
+    assert (dep[50:] == 0).all() and (val[:, 70:] == 0).all()
+    assert all(_same_float_bits(x, y) for x, y in zip(mim.integrals(1), im.integrals(1)))
+    forest = cf.RandomForestImage.train(ctx, mim, cf.TrainingConfiguration(**params), tree_count=1, num_labels=6)
+    assert len(forest.tree(0)[0]) > 1
