@@ -53,7 +53,8 @@ class _TrainConfig(ctypes.Structure):
                 ("thresholds", ctypes.c_uint16), ("pad0", ctypes.c_uint16), ("num_threads", ctypes.c_int32),
                 ("max_images", ctypes.c_int32), ("use_depth_images", ctypes.c_int32),
                 ("horizontal_flipping", ctypes.c_int32), ("subsampling_pixel_uniform", ctypes.c_int32),
-                ("num_ignored", ctypes.c_int32), ("ignored_labels", ctypes.c_int32 * 8)]
+                ("num_ignored", ctypes.c_int32), ("ignored_labels", ctypes.c_int32 * 8),
+                ("use_depth_filling", ctypes.c_int32), ("use_cielab", ctypes.c_int32)]
 
 
 def _load():
@@ -94,11 +95,15 @@ def _load():
         "cf_label_map_create": [ctypes.POINTER(P)], "cf_label_map_destroy": [P],
         "cf_label_map_encode": [P, P, ctypes.POINTER(I)], "cf_label_map_find": [P, P],
         "cf_label_map_size": [P], "cf_label_map_colors": [P, P],
-        "cf_dataset_load": [P, I, I, I, P, P, I, ctypes.POINTER(P)], "cf_dataset_destroy": [P],
+        "cf_dataset_load": [P, I, I, I, I, P, P, I, ctypes.POINTER(P)], "cf_dataset_destroy": [P],
+        "cf_dataset_get_color": [P, I, P], "cf_images_set_rgbd_f32_padded": [P, I, P, P, I, I],
+        "cf_pixel_accuracy": [P, P, ctypes.c_size_t, I, P, I, P, I, P], "cf_confusion_normalize": [P, I],
+        "cf_confusion_class_accuracy": [P, I, I, I, P, I, P],
         "cf_dataset_info": [P, P, P, P, P], "cf_dataset_image_size": [P, I, P, P],
         "cf_dataset_get": [P, I, P, P, P], "cf_dataset_to_images": [P, P, ctypes.POINTER(P)],
         "cf_png_read_info": [ctypes.c_char_p, P, P, P, P], "cf_png_read": [ctypes.c_char_p, P, ctypes.c_size_t],
         "cf_png_write": [ctypes.c_char_p, I, I, I, I, P],
+        "cf_tool_train": [I, ctypes.POINTER(ctypes.c_char_p)], "cf_tool_predict": [I, ctypes.POINTER(ctypes.c_char_p)],
         "cf_server_create": [P, P, D, I, ctypes.POINTER(P)],
         "cf_server_submit": [P, P, P, I, I, ctypes.POINTER(ctypes.c_uint64)],
         "cf_server_wait": [P, ctypes.c_uint64, P, P, P],
@@ -337,7 +342,8 @@ class TrainingConfiguration:
                    c.box_radius, c.region_size, c.thresholds, numThreads=c.num_threads, maxImages=c.max_images,
                    subsamplingType="pixelUniform" if c.subsampling_pixel_uniform else "classUniform",
                    ignored_labels=[c.ignored_labels[i] for i in range(c.num_ignored)],
-                   useDepthImages=bool(c.use_depth_images), horizontalFlipping=bool(c.horizontal_flipping))
+                   useDepthImages=bool(c.use_depth_images), horizontalFlipping=bool(c.horizontal_flipping),
+                   useDepthFilling=bool(c.use_depth_filling), useCIELab=bool(c.use_cielab))
 
     def to_c(self):
         c = _TrainConfig()
@@ -347,6 +353,7 @@ class TrainingConfiguration:
         c.num_threads, c.max_images = self.numThreads, self.maxImages
         c.use_depth_images, c.horizontal_flipping = int(self.useDepthImages), int(self.horizontalFlipping)
         c.subsampling_pixel_uniform = int(self.subsamplingType == "pixelUniform")
+        c.use_depth_filling, c.use_cielab = int(self.useDepthFilling), int(self.useCIELab)
         c.num_ignored = len(self.ignored_labels)
         for i, l in enumerate(self.ignored_labels):
             c.ignored_labels[i] = l
@@ -622,15 +629,22 @@ class Dataset:
 
     @classmethod
     def load(cls, color_files, useDepthImages=True, useDepthFilling=False, padding_color=None, label_map=None,
-             threads=0):
+             threads=0, useCIELab=False):
         label_map = label_map if label_map is not None else LabelMap()
         files = [os.fsencode(str(f)) for f in color_files]
         arr = (ctypes.c_char_p * len(files))(*files)
         pc = None if padding_color is None else np.ascontiguousarray(padding_color, np.uint8)
         h = ctypes.c_void_p()
-        _check(None, lib.cf_dataset_load(arr, len(files), int(useDepthImages), int(useDepthFilling), _ptr(pc),
-                                         label_map.h, threads, ctypes.byref(h)))
+        _check(None, lib.cf_dataset_load(arr, len(files), int(useDepthImages), int(useDepthFilling), int(useCIELab),
+                                         _ptr(pc), label_map.h, threads, ctypes.byref(h)))
         return cls(h, label_map, color_files)
+
+    def color(self, index):
+        """colour planes [3][h][w] float as the forest sees them (RGB values or CIELab)"""
+        w, h = self.image_size(index)
+        out = np.zeros((3, h, w), np.float32)
+        _check(None, lib.cf_dataset_get_color(self.h, index, _ptr(out)))
+        return out
 
     def image_size(self, index):
         w, h = ctypes.c_int(), ctypes.c_int()
@@ -666,13 +680,74 @@ def load_images(folder, useCIELab=False, useDepthImages=True, useDepthFilling=Fa
     """loadImages(folder, useCIELab, useDepthImages, useDepthFilling, ignoredColors, numLabels)
     (image.cpp:869-965).  Returns the Dataset; its label_map holds the colours met so far, and
     len(label_map) is the reference's numLabels."""
-    if useCIELab:
-        raise CurfilError(CF_ERR_UNSUPPORTED, "CIELab conversion is not available (VIGRA's RGB2LabFunctor)")
     files = list_image_filenames(folder)
     if not files:
-        raise CurfilError(CF_ERR_ARG, f"no '*_colors.png' images in {folder}")
+        raise CurfilError(CF_ERR_ARG, f"found no files in {folder}")
     padding = parse_color(ignoredColors[0]) if ignoredColors else (0, 0, 0)
-    return Dataset.load(files, useDepthImages, useDepthFilling, padding, label_map, threads)
+    return Dataset.load(files, useDepthImages, useDepthFilling, padding, label_map, threads, useCIELab)
+
+
+class ConfusionMatrix:
+    """ConfusionMatrix (predict.h / predict.cpp:44-140): counts [C][C], rows = labels, columns = predictions."""
+
+    def __init__(self, numClasses, ignoredLabels=()):
+        self.data = np.zeros((numClasses, numClasses), np.float64)
+        self.ignoredLabels = np.ascontiguousarray(list(ignoredLabels), np.int32)
+        self.normalized = False
+
+    @property
+    def numClasses(self):
+        return self.data.shape[0]
+
+    def __iadd__(self, other):
+        if self.normalized:
+            raise RuntimeError("confusion matrix is already normalized")
+        if other.numClasses != self.numClasses:
+            raise RuntimeError(f"different number of classes in confusion matrix: {self.numClasses} and "
+                               f"{other.numClasses}")
+        self.data += other.data
+        return self
+
+    def normalize(self):
+        if self.normalized:
+            raise RuntimeError("confusion matrix is already normalized")
+        _check(None, lib.cf_confusion_normalize(_ptr(self.data), self.numClasses))
+        self.normalized = True
+
+    def averageClassAccuracy(self, includeVoid=True):
+        out = ctypes.c_double()
+        _check(None, lib.cf_confusion_class_accuracy(_ptr(self.data), self.numClasses, int(self.normalized),
+                                                     int(includeVoid), _ptr(self.ignoredLabels),
+                                                     len(self.ignoredLabels), ctypes.byref(out)))
+        return out.value
+
+
+def calculatePixelAccuracy(prediction, groundTruth, includeVoid=True, ignoredLabels=(), confusionMatrix=None):
+    """calculatePixelAccuracy (predict.cpp:142-199)"""
+    p = np.ascontiguousarray(prediction, np.uint8)
+    g = np.ascontiguousarray(groundTruth, np.uint8)
+    if p.shape != g.shape:
+        raise ValueError("prediction and ground truth differ in size")
+    ign = np.ascontiguousarray(list(ignoredLabels), np.int32)
+    conf = None
+    if confusionMatrix is not None:
+        need = int(g.max()) + 1 if g.size else 0
+        if confusionMatrix.numClasses < need:       # the reference grows the matrix to the labels seen
+            confusionMatrix.data = np.zeros((need, need), np.float64)
+        conf = confusionMatrix.data
+    out = ctypes.c_double()
+    _check(None, lib.cf_pixel_accuracy(_ptr(p), _ptr(g), p.size, int(includeVoid), _ptr(ign), len(ign), _ptr(conf),
+                                       0 if conf is None else conf.shape[0], ctypes.byref(out)))
+    return out.value
+
+
+def run_tool(name, args):
+    """Runs curfil_train ('train') or curfil_predict ('predict') in this process with the given command-line
+    arguments (the same as the executables in curfil_amd/bin take); returns the exit status."""
+    fn = {"train": lib.cf_tool_train, "predict": lib.cf_tool_predict}[name]
+    argv = [f"curfil_{name}"] + [str(a) for a in args]
+    arr = (ctypes.c_char_p * len(argv))(*[os.fsencode(a) for a in argv])
+    return fn(len(argv), arr)
 
 
 def read_png(path):

@@ -30,6 +30,9 @@ SOURCES = [
     "tree_json.cpp",
     "png_io.cpp",
     "dataset.cpp",
+    "metrics.cpp",
+    "tool_train.cpp",
+    "tool_predict.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -96,7 +99,26 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_tools()
     return LIB
+
+
+TOOLS = ["curfil_train", "curfil_predict"]
+BIN = os.path.join(HERE, "bin")
+
+
+def build_tools():
+    """The command-line tools (host C++ over the C ABI), linked against the library next to them."""
+    os.makedirs(BIN, exist_ok=True)
+    include = os.path.join(os.path.dirname(HERE), "include")
+    for t in TOOLS:
+        out = os.path.join(BIN, t)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", include, os.path.join(CSRC, "tools", t + ".cpp"), "-o",
+               out + ".tmp", "-L", HERE, "-lcurfil_hip", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"tool build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

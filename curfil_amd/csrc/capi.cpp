@@ -333,6 +333,49 @@ int cf_images_set_rgbd_f32(cf_images* im, int index, const float* color, const i
     });
 }
 
+int cf_images_set_rgbd_f32_padded(cf_images* im, int index, const float* color, const int32_t* depth, int width,
+        int height) {
+    if (!im) return CF_ERR_ARG;
+    if (width == im->W && height == im->H) return cf_images_set_rgbd_f32(im, index, color, depth);
+    return guarded(im->ctx, [&] {
+        require(index >= 0 && index < im->count, "image index out of range");
+        require(color && depth, "null input");
+        require(width > 0 && height > 0 && width <= im->W && height <= im->H, "image larger than the set's size");
+        cf_ctx* ctx = im->ctx;
+        const size_t px = static_cast<size_t>(width) * height;
+        const size_t slot = static_cast<size_t>(im->W) * im->H;
+        for (size_t p = 0; p < px; p++) {
+            if (depth[p] > 50000) throw Error(CF_ERR_ARG, "illegal depth value (> 50000 mm)");
+        }
+        auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+        ctx->scratch0.resize(align(px * 3 * sizeof(float)) + px * sizeof(int32_t));
+        ctx->scratch1.resize(px * 5 * sizeof(uint32_t));
+        ctx->scratch2.resize(align(px * 3 * sizeof(double)) + align(px * 3 * sizeof(float)) + px * sizeof(int2));
+        float* dCol = reinterpret_cast<float*>(ctx->scratch0.get());
+        int32_t* dDep = reinterpret_cast<int32_t*>(ctx->scratch0.get() + align(px * 3 * sizeof(float)));
+        double* work = reinterpret_cast<double*>(ctx->scratch2.get());
+        float* tCol = reinterpret_cast<float*>(ctx->scratch2.get() + align(px * 3 * sizeof(double)));
+        int2* tDep = reinterpret_cast<int2*>(ctx->scratch2.get() + align(px * 3 * sizeof(double)) +
+                align(px * 3 * sizeof(float)));
+        CF_HIP(hipMemcpyAsync(dCol, color, px * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        CF_HIP(hipMemcpyAsync(dDep, depth, px * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        launchIntegralF32(ctx->stream, dCol, dDep, width, height, reinterpret_cast<uint32_t*>(ctx->scratch1.get()), work,
+                tCol, tDep);
+        launchIntegralPad(ctx->stream, tCol, tDep, width, height, im->W, im->H,
+                im->col.get() + static_cast<size_t>(index) * 3 * slot, im->dep.get() + static_cast<size_t>(index) * slot);
+        int32_t* hd = im->depthHost.data() + static_cast<size_t>(index) * slot;
+        std::fill(hd, hd + slot, 0);
+        for (int y = 0; y < height; y++) {
+            for (int x = 0; x < width; x++) {
+                const int32_t d = depth[static_cast<size_t>(y) * width + x];
+                hd[static_cast<size_t>(y) * im->W + x] = d > 0 ? d : 0;
+            }
+        }
+        im->depthSet[index] = 1;
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int cf_images_set_labels(cf_images* im, int index, const uint8_t* labels) {
     if (!im) return CF_ERR_ARG;
     return guarded(im->ctx, [&] {

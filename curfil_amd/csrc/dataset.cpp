@@ -1,5 +1,6 @@
 // Training and test datasets from PNG files: loadImagePair / loadImages
-// (image.cpp:818-965), RGBDImage loading (image.cpp:123-184, 396-442), depth
+// (image.cpp:818-965), RGBDImage loading (image.cpp:123-184, 396-442), CIELab
+// colour (image.cpp:89-96), depth
 // filling (image.cpp:279-377), dummy depth (image.cpp:379-394), label images
 // and the colour -> label map (image.cpp:660-760), and padding to the largest
 // image (image.cpp:449-475, 763-780).
@@ -12,6 +13,7 @@
 // encodes in completion order).
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -61,6 +63,32 @@ std::vector<uint8_t> rgb8(const PngImage& im, const std::string& path, const cha
     if (im.bitDepth != 8) throw Error(CF_ERR_UNSUPPORTED, std::string(what) + " '" + path + "': 16-bit colour");
     std::vector<uint8_t> out(im.samples.size());
     for (size_t i = 0; i < out.size(); i++) out[i] = static_cast<uint8_t>(im.samples[i]);
+    return out;
+}
+
+// CIELab colour planes [3][H][W] from 8-bit RGB: VIGRA's RGB2LabFunctor<double>
+// with its default maximum 255 (image.cpp:89-96), i.e. linear RGB -> XYZ
+// (D65 matrix), then L = 116 Y^(1/3) - 16 (or kappa * Y below epsilon),
+// a = 500 (Xn^(1/3) - Y^(1/3)), b = 200 (Y^(1/3) - Zn^(1/3)) with the white
+// point (0.950456, 1, 1.088754); rounded once to float as the reference stores
+// its colour planes.
+std::vector<float> labPlanes(const std::vector<uint8_t>& rgb, int W, int H) {
+    const size_t px = static_cast<size_t>(W) * H;
+    std::vector<float> out(3 * px);
+    const double third = 1.0 / 3.0, kappa = 24389.0 / 27.0, epsilon = 216.0 / 24389.0;
+    for (size_t i = 0; i < px; i++) {
+        const double r = rgb[3 * i] / 255.0, g = rgb[3 * i + 1] / 255.0, b = rgb[3 * i + 2] / 255.0;
+        const double X = 0.412453 * r + 0.357580 * g + 0.180423 * b;
+        const double Y = 0.212671 * r + 0.715160 * g + 0.072169 * b;
+        const double Z = 0.019334 * r + 0.119193 * g + 0.950227 * b;
+        const double xg = std::pow(X / 0.950456, third);
+        const double yg = std::pow(Y, third);
+        const double zg = std::pow(Z / 1.088754, third);
+        const double L = Y < epsilon ? kappa * Y : 116.0 * yg - 16.0;
+        out[i] = static_cast<float>(L);
+        out[px + i] = static_cast<float>(500.0 * (xg - yg));
+        out[2 * px + i] = static_cast<float>(200.0 * (yg - zg));
+    }
     return out;
 }
 
@@ -150,6 +178,7 @@ struct cf_dataset {
     std::vector<cf::LoadedPair> pairs;
     int W = 0, H = 0;          // the largest width and height (every image is padded to it)
     int paddingLabel = 0;
+    bool cielab = false;       // colour planes in CIELab (float) instead of 8-bit RGB
     std::string error;
 };
 
@@ -214,20 +243,21 @@ int cf_label_map_colors(const cf_label_map* map, uint8_t* rgb) {
     return static_cast<int>(map->colors.size());
 }
 
-int cf_dataset_load(const char* const* colorFiles, int count, int useDepthImages, int useDepthFilling,
+int cf_dataset_load(const char* const* colorFiles, int count, int useDepthImages, int useDepthFilling, int useCIELab,
         const uint8_t* paddingColor, cf_label_map* map, int threads, cf_dataset** out) {
     if (!out || !map) return CF_ERR_ARG;
     *out = nullptr;
     auto ds = std::make_unique<cf_dataset>();
+    ds->cielab = useCIELab != 0;
     const int st = hostCall(nullptr, [&] {
         if (!colorFiles || count <= 0) throw Error(CF_ERR_ARG, "no images to load");
         for (int i = 0; i < count; i++) {
             if (!colorFiles[i]) throw Error(CF_ERR_ARG, "null file name");
         }
-        // getPaddingLabel (image.cpp:859-867): the first ignored colour, else black, encoded first
-        const uint8_t black[3] = {0, 0, 0};
-        const uint8_t* pc = paddingColor ? paddingColor : black;
-        ds->paddingLabel = map->encode({pc[0], pc[1], pc[2]});
+        // getPaddingLabel (image.cpp:859-867): the first ignored colour, else black, encoded first;
+        // without a padding colour (single test images, predict.cpp:254) nothing is encoded and
+        // images must not need padding
+        ds->paddingLabel = paddingColor ? map->encode({paddingColor[0], paddingColor[1], paddingColor[2]}) : -1;
         ds->pairs.resize(count);
         std::vector<std::string> errors(count);
         const int nt = std::max(1, std::min(threads > 0 ? threads : 8, count));
@@ -296,6 +326,21 @@ int cf_dataset_get(const cf_dataset* ds, int index, uint8_t* rgb, uint16_t* dept
     return CF_OK;
 }
 
+int cf_dataset_get_color(const cf_dataset* ds, int index, float* color) {
+    if (!ds || !color || index < 0 || index >= static_cast<int>(ds->pairs.size())) return CF_ERR_ARG;
+    const LoadedPair& p = ds->pairs[index];
+    const size_t px = static_cast<size_t>(p.W) * p.H;
+    if (ds->cielab) {
+        const std::vector<float> lab = labPlanes(p.rgb, p.W, p.H);
+        std::memcpy(color, lab.data(), lab.size() * sizeof(float));
+    } else {
+        for (size_t i = 0; i < px; i++) {
+            for (int c = 0; c < 3; c++) color[c * px + i] = p.rgb[3 * i + c];
+        }
+    }
+    return CF_OK;
+}
+
 int cf_dataset_to_images(const cf_dataset* ds, cf_ctx* ctx, cf_images** out) {
     if (!ds || !ctx || !out) return CF_ERR_ARG;
     *out = nullptr;
@@ -305,10 +350,21 @@ int cf_dataset_to_images(const cf_dataset* ds, cf_ctx* ctx, cf_images** out) {
     std::vector<uint8_t> lab(static_cast<size_t>(ds->W) * ds->H);
     for (size_t i = 0; i < ds->pairs.size() && st == CF_OK; i++) {
         const LoadedPair& p = ds->pairs[i];
-        st = cf_images_set_rgbd_u8_padded(im, static_cast<int>(i), p.rgb.data(), p.depth.data(), p.W, p.H);
+        if (ds->cielab) {
+            const std::vector<float> lab = labPlanes(p.rgb, p.W, p.H);
+            const std::vector<int32_t> d32(p.depth.begin(), p.depth.end());
+            st = cf_images_set_rgbd_f32_padded(im, static_cast<int>(i), lab.data(), d32.data(), p.W, p.H);
+        } else {
+            st = cf_images_set_rgbd_u8_padded(im, static_cast<int>(i), p.rgb.data(), p.depth.data(), p.W, p.H);
+        }
         if (st != CF_OK) break;
         // LabelImage::resizeImage (image.cpp:763-780): padding label outside the image
-        std::fill(lab.begin(), lab.end(), static_cast<uint8_t>(ds->paddingLabel));
+        if ((p.W != ds->W || p.H != ds->H) && ds->paddingLabel < 0) {
+            st = CF_ERR_ARG;
+            ctx->error = "images of different sizes need a padding colour";
+            break;
+        }
+        std::fill(lab.begin(), lab.end(), static_cast<uint8_t>(std::max(ds->paddingLabel, 0)));
         for (int y = 0; y < p.H; y++) {
             std::memcpy(&lab[static_cast<size_t>(y) * ds->W], &p.labels[static_cast<size_t>(y) * p.W], p.W);
         }

@@ -1,0 +1,4 @@
+// curfil_predict: the command-line tool; its body is cf_tool_predict in libcurfil_hip.so.
+#include "curfil_hip.h"
+
+int main(int argc, char** argv) { return cf_tool_predict(argc, argv); }

@@ -569,8 +569,8 @@ void writeTreeFile(const cf_forest* fo, int tree, const char* path, const cf_tra
     w.put("maxImages", std::to_string(cfg.max_images));
     w.put("imageCacheSize", "0");
     w.put("subsamplingType", cfg.subsampling_pixel_uniform ? "pixelUniform" : "classUniform");
-    w.put("useCIELab", "false");
-    w.put("useDepthFilling", "false");
+    w.put("useCIELab", cfg.use_cielab ? "true" : "false");
+    w.put("useDepthFilling", cfg.use_depth_filling ? "true" : "false");
     w.put("useDepthImages", cfg.use_depth_images ? "true" : "false");
     w.openArr("ignoredColors");
     for (int i = 0; i < cfg.num_ignored && i < 8; i++) {
@@ -624,6 +624,11 @@ std::unique_ptr<cf_forest> readTreeFiles(cf_ctx* ctx, const char* const* paths, 
         c.use_depth_images = udi ? (udi->asBool() ? 1 : 0) : 1;
         const JVal* st = doc.find("subsamplingType");
         c.subsampling_pixel_uniform = st ? (st->text() == "pixelUniform" ? 1 : 0) : 1;
+        // optional keys and their defaults as RandomTreeImport::readJSON reads them (import.cpp:210-248)
+        const JVal* lab = doc.find("useCIELab");
+        c.use_cielab = lab ? (lab->asBool() ? 1 : 0) : 1;
+        const JVal* fill = doc.find("useDepthFilling");
+        c.use_depth_filling = fill ? (fill->asBool() ? 1 : 0) : 0;
         ParsedTree pt = readTree(doc.at("tree"), pal, paletteSet);
         // ignored colours as labels, through the colours of the histogram keys
         const JVal* ig = doc.find("ignoredColors");
@@ -651,6 +656,7 @@ std::unique_ptr<cf_forest> readTreeFiles(cf_ctx* ctx, const char* const* paths, 
                 c.thresholds != first.thresholds || c.num_threads != first.num_threads ||
                 c.max_images != first.max_images || c.subsampling_pixel_uniform != first.subsampling_pixel_uniform ||
                 c.use_depth_images != first.use_depth_images || c.num_ignored != first.num_ignored ||
+                c.use_cielab != first.use_cielab || c.use_depth_filling != first.use_depth_filling ||
                 std::memcmp(c.ignored_labels, first.ignored_labels, sizeof(c.ignored_labels)) != 0) {
             throw Error(CF_ERR_ARG, "different configurations");
         }

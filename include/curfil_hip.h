@@ -78,6 +78,8 @@ typedef struct cf_train_config {
     int32_t subsampling_pixel_uniform; /* 0: classUniform (default) */
     int32_t num_ignored;
     int32_t ignored_labels[8];
+    int32_t use_depth_filling;         /* recorded in tree files; the loader fills depth */
+    int32_t use_cielab;                /* recorded in tree files; the loader converts colour */
 } cf_train_config;
 
 /* Training execution mode. */
@@ -115,6 +117,9 @@ int cf_images_set_rgbd_f32(cf_images* images, int index, const float* color, con
  * (RGBDImage::resizeImage on integrated images, image.cpp:449-475 via
  * image.cpp:945-960).  Equal sizes: cf_images_set_rgbd_u8. */
 int cf_images_set_rgbd_u8_padded(cf_images* images, int index, const uint8_t* rgb, const uint16_t* depth_mm,
+        int width, int height);
+/* the same for float colour planes [3][height][width] (e.g. CIELab) and int32 depth */
+int cf_images_set_rgbd_f32_padded(cf_images* images, int index, const float* color, const int32_t* depth_mm,
         int width, int height);
 /* label image [H][W] (host copy, read by pixel sampling) */
 int cf_images_set_labels(cf_images* images, int index, const uint8_t* labels);
@@ -310,13 +315,14 @@ int cf_label_map_colors(const cf_label_map* map, uint8_t* rgb);
  * use_depth_images is 0; the reference's checks of values and average depth)
  * and `*_ground_truth.png` (RGB colours -> labels through `map`, column by
  * column), depth filling when asked (image.cpp:279-377).  The padding colour
- * (the first ignored colour, NULL = black) is encoded first, as getPaddingLabel
- * does.  Files are decoded on `threads` host threads (0 = 8); labels are
+ * (the first ignored colour, else black) is encoded first, as getPaddingLabel
+ * does; NULL encodes none (test images loaded one at a time, which are not
+ * padded; the padding label is then -1).  Files are decoded on `threads` host threads (0 = 8); labels are
  * encoded in file order.  Errors (missing files, non-RGB colour images, a bad
  * depth image) fail the call; cf_last_error(NULL) has the message. */
 typedef struct cf_dataset cf_dataset;
 int cf_dataset_load(const char* const* color_files, int count, int use_depth_images, int use_depth_filling,
-        const uint8_t* padding_color, cf_label_map* map, int threads, cf_dataset** out);
+        int use_cielab, const uint8_t* padding_color, cf_label_map* map, int threads, cf_dataset** out);
 int cf_dataset_destroy(cf_dataset* dataset);
 /* number of images, the largest width and height, the padding label */
 int cf_dataset_info(const cf_dataset* dataset, int* count, int* width, int* height, int* padding_label);
@@ -324,6 +330,10 @@ int cf_dataset_info(const cf_dataset* dataset, int* count, int* width, int* heig
 int cf_dataset_image_size(const cf_dataset* dataset, int index, int* width, int* height);
 /* an image at its own size: rgb [h][w][3], depth [h][w] (after filling), labels [h][w]; any may be NULL */
 int cf_dataset_get(const cf_dataset* dataset, int index, uint8_t* rgb, uint16_t* depth_mm, uint8_t* labels);
+/* the colour planes [3][h][w] as the forest sees them: RGB values, or CIELab
+ * (VIGRA's RGB2LabFunctor<double> restated, image.cpp:89-96; parity unpinned)
+ * when the set was loaded with use_cielab */
+int cf_dataset_get_color(const cf_dataset* dataset, int index, float* color);
 /* every image on the device at the largest size (smaller images padded as
  * loadImages pads them, labels padded with the padding label) */
 int cf_dataset_to_images(const cf_dataset* dataset, cf_ctx* ctx, cf_images** out);
@@ -335,6 +345,28 @@ int cf_png_read_info(const char* path, int* width, int* height, int* channels, i
 int cf_png_read(const char* path, uint16_t* samples, size_t capacity);
 /* 8-bit RGB (pixels [h][w][3] u8) or 8/16-bit grayscale ([h][w] u8 / u16) */
 int cf_png_write(const char* path, int width, int height, int channels, int bit_depth, const void* pixels);
+
+/* ---- prediction quality ------------------------------------------------------ */
+/* calculatePixelAccuracy (predict.cpp:142-199) over n pixels: with include_void
+ * every pixel counts, else pixels whose ground truth is an ignored label are
+ * skipped; confusion [C][C] (may be NULL; rows: labels, columns: predictions)
+ * is incremented for the pixels counted. */
+int cf_pixel_accuracy(const uint8_t* prediction, const uint8_t* ground_truth, size_t n, int include_void,
+        const int* ignored_labels, int num_ignored, double* confusion, int num_classes, double* accuracy);
+/* ConfusionMatrix::normalize (predict.cpp:102-140): rows scaled to sum 1 (zero rows stay zero) */
+int cf_confusion_normalize(double* confusion, int num_classes);
+/* ConfusionMatrix::averageClassAccuracy (predict.cpp:71-100): mean diagonal of the
+ * row-normalised matrix, without the ignored labels unless include_void */
+int cf_confusion_class_accuracy(const double* confusion, int num_classes, int normalized, int include_void,
+        const int* ignored_labels, int num_ignored, double* out);
+
+/* ---- command-line tools --------------------------------------------------------- */
+/* main() of curfil_train (train_main.cpp:40-187) and curfil_predict
+ * (predict_main.cpp:59-166 with test(), predict.cpp:202-400): the same options,
+ * positional order and defaults; log lines on stderr; returns the exit status.
+ * The executables curfil_amd/bin/curfil_train and curfil_predict call these. */
+int cf_tool_train(int argc, char** argv);
+int cf_tool_predict(int argc, char** argv);
 
 /* ---- synthetic workload (benchmark / tests; no counterpart in the reference) -- */
 /* One labelled RGB-D frame: Voronoi labels (site k -> label k mod (classes+1),

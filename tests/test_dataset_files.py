@@ -9,6 +9,10 @@
   millimetres; dummy depth 1000 without depth images; depth filling (a hand-worked known answer); the
   reference's rejections (non-RGB colour or label image, non-grayscale depth, depth > 50000, average
   depth outside [0.01, 10] m, sizes that differ, missing files).
+* CIELab colour (VIGRA's RGB2LabFunctor, image.cpp:89-96): parity unpinned (no vectors in the reference);
+  checked against published CIE L*a*b* values of white, black and the primaries (D65).
+* ConfusionMatrix / calculatePixelAccuracy (predict.cpp:44-199): the reference's known answers
+  (hyperopt_test.cpp:108-174) and a hand-worked 3-class case.
 """
 import os
 import struct
@@ -227,8 +231,6 @@ def test_loader_rejections(tmp_path):
     os.remove(os.path.join(missing, "f_ground_truth.png"))
     with pytest.raises(cf.CurfilError, match="cannot open"):
         cf.load_images(missing)
-    with pytest.raises(cf.CurfilError, match="CIELab"):
-        cf.load_images(case("lab"), useCIELab=True)
 
 
 def test_parse_color():
@@ -237,3 +239,60 @@ def test_parse_color():
         cf.parse_color("0,256,0")
     with pytest.raises(ValueError, match="illegal color"):
         cf.parse_color("red")
+
+
+def test_cielab_against_published_values(tmp_path):
+    colors = np.array([[255, 255, 255], [0, 0, 0], [255, 0, 0], [0, 255, 0], [0, 0, 255]], np.uint8)
+    rgb = np.repeat(colors[None], 2, axis=0)                       # 2 x 5 image
+    _write_triple(str(tmp_path), "lab", rgb, np.full((2, 5), 1500, np.uint16), PALETTE[[1, 1, 2, 2, 3]][None]
+                  .repeat(2, axis=0))
+    lab = cf.load_images(str(tmp_path), useCIELab=True).color(0)[:, 0, :].T
+    # CIE L*a*b* (D65) of sRGB white, black, red, green, blue; linear RGB 0 / 1 equals sRGB 0 / 255
+    want = np.array([[100.0, 0.0, 0.0], [0.0, 0.0, 0.0], [53.24, 80.09, 67.20], [87.73, -86.18, 83.18],
+                     [32.30, 79.19, -107.86]])
+    assert np.abs(lab - want).max() < 0.05
+    assert np.abs(lab[0]).max() < 100.0001 and abs(lab[0][1]) < 1e-4 and abs(lab[0][2]) < 1e-4
+    plain = cf.load_images(str(tmp_path)).color(0)
+    assert np.array_equal(plain.transpose(1, 2, 0), rgb.astype(np.float32))
+
+
+def test_confusion_matrix_known_answers():
+    # hyperopt_test.cpp:108-174
+    m = cf.ConfusionMatrix(3)
+    m.data[:] = [[1.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.0, 0.25, 0.75]]
+    assert m.averageClassAccuracy(True) == 0.75
+    m = cf.ConfusionMatrix(3, ignoredLabels=[0])
+    m.data[:] = [[1.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.0, 0.25, 0.75]]
+    assert m.averageClassAccuracy(True) == 1.0 - (1.0 - (1.0 + 0.5 + 0.75) / 3.0)
+    assert m.averageClassAccuracy(False) == 1.0 - (1.0 - (0.5 + 0.75) / 2.0)
+
+
+def test_pixel_accuracy_hand_worked():
+    gt = np.array([[0, 1, 1, 2], [2, 2, 1, 0]], np.uint8)
+    pr = np.array([[1, 1, 2, 2], [2, 0, 1, 0]], np.uint8)
+    # with void: correct 5 of 8; without label 0: pixels (1,1,2,2,2,1): correct 4 of 6
+    assert cf.calculatePixelAccuracy(pr, gt, True, [0]) == 5 / 8
+    cm = cf.ConfusionMatrix(3, [0])
+    assert cf.calculatePixelAccuracy(pr, gt, False, [0], cm) == 4 / 6
+    assert cm.data.tolist() == [[0, 0, 0], [0, 2, 1], [1, 0, 2]]
+    cm.normalize()
+    assert np.allclose(cm.data, [[0, 0, 0], [0, 2 / 3, 1 / 3], [1 / 3, 0, 2 / 3]])
+    assert abs(cm.averageClassAccuracy(False) - 2 / 3) < 1e-15
+    assert abs(cm.averageClassAccuracy(True) - 4 / 9) < 1e-15
+    with pytest.raises(RuntimeError, match="already normalized"):
+        cm.normalize()
+
+
+def test_tools_parse_options_like_the_reference(capfd):
+    assert cf.run_tool("train", ["--help"]) == 1
+    out = capfd.readouterr().out
+    for opt in ("folderTraining", "numThresholds", "ignoreColor", "horizontalFlipping", "subsamplingType"):
+        assert "--" + opt in out
+    assert cf.run_tool("train", ["folder", "3"]) == 1
+    assert "'--samplesPerImage' is required" in capfd.readouterr().err
+    assert cf.run_tool("predict", ["out", "test"]) == 1
+    assert "'--treeFile' is required" in capfd.readouterr().err
+    assert cf.run_tool("predict", ["out", "test", "t.json.gz", "--histogramBias", "1.5"]) == 1
+    assert "illegal histogram bias" in capfd.readouterr().err
+    assert cf.run_tool("train", ["f", "1", "1", "1", "1", "1", "1", "1", "1", "--mode", "cpu"]) == 1
+    assert "not available" in capfd.readouterr().err

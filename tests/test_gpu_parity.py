@@ -6,6 +6,7 @@ chosen splits, tree structure and labels; scores compared against the oracle
 1e-5 relative.
 """
 import math
+import re
 
 import numpy as np
 import pytest
@@ -505,3 +506,57 @@ def test_png_dataset_on_device_matches_arrays(ctx, config0, tmp_path):
     assert all(_same_float_bits(x, y) for x, y in zip(mim.integrals(1), im.integrals(1)))
     forest = cf.RandomForestImage.train(ctx, mim, cf.TrainingConfiguration(**params), tree_count=1, num_labels=6)
     assert len(forest.tree(0)[0]) > 1
+
+
+def test_command_line_tools_end_to_end(ctx, config0, tmp_path, capfd):
+    """curfil_train on a PNG folder writes tree files whose trees equal the library's training on the same
+    frames; curfil_predict on a test folder writes prediction images equal to the library's prediction
+    with the imported forest, and logs the pixel accuracies calculatePixelAccuracy gives."""
+    rgb, depth, labels, im, ds = config0
+    pal = np.array([[0, 0, 0], [200, 10, 10], [10, 200, 10], [10, 10, 200], [250, 250, 0], [0, 128, 255]],
+                   np.uint8)
+    train, test, out, pred = (tmp_path / d for d in ("train", "test", "out", "pred"))
+    for d in (train, test):
+        d.mkdir()
+    for i in range(4):
+        for folder, r, dp, lb in ((train, rgb, depth, labels),):
+            cf.write_png(folder / f"f{i}_colors.png", r[i])
+            cf.write_png(folder / f"f{i}_depth.png", dp[i])
+            cf.write_png(folder / f"f{i}_ground_truth.png", pal[lb[i]])
+    trgb, tdepth, tlabels = cf.synth_dataset(2, 160, 120, sites=12, classes=5, zero_frac=0.05, seed0=99)
+    for i in range(2):
+        cf.write_png(test / f"t{i}_colors.png", trgb[i])
+        cf.write_png(test / f"t{i}_depth.png", tdepth[i])
+        cf.write_png(test / f"t{i}_ground_truth.png", pal[tlabels[i]])
+    args = [train, 2, 1000, 100, 32, 7, 60, 10, 10, out, "--useCIELab=false", "--ignoreColor", "0,0,0"]
+    assert cf.run_tool("train", args) == 0
+    paths = [out / "tree0.json.gz", out / "tree1.json.gz"]
+    assert all(p.exists() for p in paths)
+    back = cf.RandomForestImage.from_files(paths)
+    data = cf.load_images(str(train), ignoredColors=["0,0,0"])
+    dim = data.to_images(ctx)
+    ign = [data.label_map.find((0, 0, 0))]
+    cfg = cf.TrainingConfiguration(4711, 1000, 100, 32, 7, 60, 10, 10, ignored_labels=ign)
+    ref = cf.RandomForestImage.train(ctx, dim, cfg, tree_count=2, num_labels=len(data.label_map))
+    for t in range(2):
+        _assert_trees_equal(*back.tree(t), *ref.tree(t))
+    assert back.palette.tolist() == data.label_map.colors().tolist()
+    capfd.readouterr()
+
+    assert cf.run_tool("predict", [pred, test] + paths) == 0
+    log = capfd.readouterr().err
+    lm = cf.LabelMap()
+    for c in back.palette:
+        lm.encode(c)
+    tset = cf.Dataset.load(cf.list_image_filenames(str(test)), label_map=lm)
+    tim = tset.to_images(ctx)
+    accs = []
+    for i in range(2):
+        lab, _ = back.predict(tim, i)
+        written = cf.read_png(pred / f"t{i}_colors_prediction.png")
+        assert np.array_equal(written, back.palette[lab])
+        gt = tset.get(i)[2]
+        accs.append(cf.calculatePixelAccuracy(lab, gt, True, ign))
+    m = re.search(r"pixel accuracy: ([0-9.]+)", log)
+    assert m and abs(float(m.group(1)) - 100 * np.mean(accs)) < 1e-4, log[-2000:]
+    assert "confusion matrix" in log
