@@ -187,25 +187,32 @@ struct BinBlock {
     int image;   // the image of every sample in the block
 };
 void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder);
+// keys (tree * images + image) and values (level index) of one tree's level segment; with flipN > 0
+// also the flipped evaluation of every Both sample at flipN + index (value | kFlipEntry), and key
+// absentKey for the other samples there
+constexpr uint32_t kFlipEntry = 0x80000000u;
 void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
-        uint32_t* vals);
+        uint32_t* vals, int flipN = 0, uint32_t absentKey = 0);
+// label per level index, bit 7 set for samples still set to Both (counted in both orientations)
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels);
 size_t radixSortTempBytes(int n, int bits);
 void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
         const uint32_t* valsIn, uint32_t* valsOut, int n, int bits);
 void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts);
-// bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order
+// bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order;
+// permutation entries with kFlipEntry are evaluated flipped into bins[flipOffset + g * N + index]
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
         const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
-        const FeatTables& tabs, int N, uint4* bins);
+        const FeatTables& tabs, int N, uint4* bins, size_t flipOffset = 0);
 // padded threshold width of the bin kernel for T thresholds
 int binThresholdWidth(int T);
 // features and +inf-padded thresholds in evaluation order: gFeat[trees * F], gThr[trees * F * binThresholdWidth(T)]
 void launchBinTables(hipStream_t s, const FeatTables& tabs, int trees, const uint16_t* evalOrder, FeatRec* gFeat,
         float* gThr);
 // G[slot][ng * 16][T + 1][C] from the bin rows of the chunks' samples
+// (samples still set to Both also count their flipped row at flipOffset)
 void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
-        int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G);
+        int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G, size_t flipOffset = 0);
 void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
         const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot, int fOffset = 0,
         int Fb = -1, const uint16_t* evalOrder = nullptr);

@@ -33,7 +33,7 @@ constexpr int kBinPerThread = 4;
 struct BinSample {
     double rinv;   // depthReciprocal(depth in metres)
     int xy;        // x | y << 16
-    int lab;       // flip << 8 | (invalid depth) << 10;  < 0: empty slot
+    int lab;       // flip << 8 | (invalid depth) << 10 | (flip entry) << 12;  < 0: empty slot
 };
 
 // #{t : th[t] < q} over a row padded with +inf (never below q), T for NaN.
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
         const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
         const uint32_t* __restrict__ perm, const BinBlock* __restrict__ blocks, const int2* __restrict__ work,
         int g0, const FeatRec* __restrict__ gFeat, const float* __restrict__ gThr, int F, int T, int N,
-        uint4* __restrict__ bins) {
+        uint4* __restrict__ bins, size_t flipOffset) {
     // work[blockIdx] = (sample block, feature group within the batch); the
     // host interleaves the XCDs' lists (workgroup w runs on XCD w % 8) so each
     // XCD sweeps one (tree, image) segment for every group of the batch
@@ -77,6 +77,7 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
 
     const int pos0 = (g0 + g) * kBinGroup;
     uint4* out = bins + static_cast<size_t>(g) * N;
+    uint4* outFlip = bins + flipOffset + static_cast<size_t>(g) * N;
     for (int base = 0; base < bb.count; base += kBinThreads * kBinPerThread) {
         BinSample rs[kBinPerThread];
         int outIdx[kBinPerThread];
@@ -89,13 +90,17 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
             rs[k].rinv = 0.0;
             outIdx[k] = -1;
             if (i < bb.count) {
-                const int s = static_cast<int>(perm[bb.begin + i]);
+                const uint32_t v = perm[bb.begin + i];
+                const int s = static_cast<int>(v & ~kFlipEntry);
                 const cf_sample x = samples[s];
                 const bool valid = x.depth_mm > 0;
+                // a flip entry evaluates a Both sample flipped; other entries in the sample's own
+                // orientation (Both: unflipped); bit 12 selects the flipped bin rows
+                const bool flipEntry = (v & kFlipEntry) != 0;
                 outIdx[k] = s;
                 rs[k].rinv = valid ? depthReciprocal(depthToFloat(x.depth_mm)) : 0.0;
                 rs[k].xy = x.x | (x.y << 16);
-                rs[k].lab = (x.flip << 8) | ((valid ? 0 : 1) << 10);
+                rs[k].lab = ((flipEntry ? 1 : x.flip) << 8) | ((valid ? 0 : 1) << 10) | (flipEntry ? 1 << 12 : 0);
             }
         }
         uint32_t row[kBinPerThread][4];
@@ -122,7 +127,7 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                 for (int k = 0; k < kBinPerThread; k++) {
                     const int lab = rs[k].lab;
                     int idx[8];
-                    ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                    ok[k] = lab >= 0 && ((lab >> 10) & 1) == 0 &&
                             responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
                                     ((lab >> 8) & 3) == 1, idx);
                     if (ok[k]) {   // lanes whose box leaves the image issue no gathers
@@ -142,7 +147,7 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                 for (int k = 0; k < kBinPerThread; k++) {
                     const int lab = rs[k].lab;
                     int idx[8];
-                    ok[k] = lab >= 0 && (lab >> 10) == 0 &&
+                    ok[k] = lab >= 0 && ((lab >> 10) & 1) == 0 &&
                             responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
                                     ((lab >> 8) & 3) == 1, idx);
                     const float* colA = col + static_cast<size_t>(fr.c1) * plane;
@@ -163,7 +168,10 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
         }
 #pragma unroll
         for (int k = 0; k < kBinPerThread; k++) {
-            if (outIdx[k] >= 0) out[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
+            if (outIdx[k] >= 0) {
+                uint4* o = (rs[k].lab & (1 << 12)) ? outFlip : out;
+                o[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
+            }
         }
     }
 }
@@ -190,7 +198,8 @@ __global__ void __launch_bounds__(256) k_bin_tables(const FeatRec* __restrict__ 
 // and at most 4 lanes can hit one address (all lanes counting one feature, as
 // for samples of an almost pure node, would collide on few bins).
 __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bins, const uint8_t* __restrict__ labels,
-        const Chunk* __restrict__ chunks, int g0, int ng, int F, int T, int C, int N, uint32_t* __restrict__ G) {
+        const Chunk* __restrict__ chunks, int g0, int ng, int F, int T, int C, int N, uint32_t* __restrict__ G,
+        size_t flipOffset) {
     extern __shared__ uint32_t h[];
     const int chunkIdx = blockIdx.x / ng;
     const int g = blockIdx.x % ng;
@@ -203,6 +212,7 @@ __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bin
     const int fi = threadIdx.x & (kBinGroup - 1);
     const int sub = threadIdx.x >> 4;   // 16 samples per pass of the workgroup
     const uint8_t* row = reinterpret_cast<const uint8_t*>(bins + static_cast<size_t>(g) * N + ch.begin);
+    const uint8_t* rowFlip = reinterpret_cast<const uint8_t*>(bins + flipOffset + static_cast<size_t>(g) * N + ch.begin);
     const uint8_t* lab = labels + ch.begin;
     uint32_t* hf = h + fi * binsPerF;
     if (fi < nf) {
@@ -221,7 +231,14 @@ __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bin
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
                 // bins are 0..T; the clamp keeps the LDS index inside the histogram whatever a row holds
-                if (l[u] >= 0) atomicAdd(&hf[min(b[u], T) * C + l[u]], 1u);
+                if (l[u] >= 0) {
+                    atomicAdd(&hf[min(b[u], T) * C + (l[u] & 0x7f)], 1u);
+                    if (l[u] & 0x80) {   // still Both: its flipped evaluation counts too
+                        const int i = base + u * 16 + sub;
+                        atomicAdd(&hf[min(static_cast<int>(rowFlip[static_cast<size_t>(i) * kBinGroup + fi]), T) * C +
+                                (l[u] & 0x7f)], 1u);
+                    }
+                }
             }
         }
     }
@@ -261,11 +278,16 @@ __global__ void __launch_bounds__(64) k_feature_eval_order(const FeatRec* __rest
 // Radix-sort keys (tree * images + image) and values (level index) of one
 // tree's level segment.
 __global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict__ samples, int begin, int n,
-        uint32_t keyBase, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+        uint32_t keyBase, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int flipN, uint32_t absentKey) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[begin + i] = keyBase + samples[begin + i].image;
+    const cf_sample x = samples[begin + i];
+    keys[begin + i] = keyBase + x.image;
     vals[begin + i] = static_cast<uint32_t>(begin + i);
+    if (flipN > 0) {
+        keys[flipN + begin + i] = x.flip == 2 ? keyBase + x.image : absentKey;
+        vals[flipN + begin + i] = static_cast<uint32_t>(begin + i) | kFlipEntry;
+    }
 }
 
 // starts[key] = first position of key in the sorted keys (entries of absent keys are left alone)
@@ -278,7 +300,7 @@ __global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__
 __global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restrict__ samples, int n,
         uint8_t* __restrict__ labels) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) labels[i] = samples[i].label;
+    if (i < n) labels[i] = static_cast<uint8_t>(samples[i].label | (samples[i].flip == 2 ? 0x80 : 0));
 }
 
 int binThresholdWidth(int T) {
@@ -302,9 +324,10 @@ void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, ui
 }
 
 void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
-        uint32_t* vals) {
+        uint32_t* vals, int flipN, uint32_t absentKey) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals);
+    hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals,
+            flipN, absentKey);
 }
 
 void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts) {
@@ -332,13 +355,13 @@ void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t*
 
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
         const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
-        const FeatTables& tabs, int N, uint4* bins) {
+        const FeatTables& tabs, int N, uint4* bins, size_t flipOffset) {
     if (numWork == 0) return;
     const int TM = binThresholdWidth(tabs.T);
     const dim3 grid(static_cast<unsigned>(numWork)), block(kBinThreads);
 #define CF_EVAL_BINS(TMV)                                                                                     \
     hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, 0, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
-            blocks, work, g0, gFeat, gThr, tabs.F, tabs.T, N, bins)
+            blocks, work, g0, gFeat, gThr, tabs.F, tabs.T, N, bins, flipOffset)
     switch (TM) {
         case 8: CF_EVAL_BINS(8); break;
         case 12: CF_EVAL_BINS(12); break;
@@ -353,12 +376,12 @@ void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* sample
 }
 
 void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
-        int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G) {
+        int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G, size_t flipOffset) {
     if (numChunks == 0 || ng == 0) return;
     const size_t lds = static_cast<size_t>(kBinGroup) * (tabs.T + 1) * C * sizeof(uint32_t);
     if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
     hipLaunchKernelGGL(k_hist_bins, dim3(static_cast<unsigned>(numChunks) * ng), dim3(256), lds, s, bins, labels,
-            chunks, g0, ng, tabs.F, tabs.T, C, N, G);
+            chunks, g0, ng, tabs.F, tabs.T, C, N, G, flipOffset);
 }
 
 }  // namespace cf

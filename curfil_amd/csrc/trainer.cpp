@@ -172,10 +172,9 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     int cur = 0;
 
     // Batched levels use the two-pass evaluation (image-major bins, then
-    // node histograms) unless a sample is evaluated in both orientations
-    // (one 8-bit bin per sample and feature), T exceeds the 8-bit bins, or
-    // CURFIL_EVAL_DIRECT is set.
-    const bool twoPass = mode == CF_MODE_BATCHED && T <= kMaxBinThresholds && !anyBoth &&
+    // node histograms; samples set to Both get a second, flipped bin row)
+    // unless T exceeds the 8-bit bins or CURFIL_EVAL_DIRECT is set.
+    const bool twoPass = mode == CF_MODE_BATCHED && T <= kMaxBinThresholds &&
             std::getenv("CURFIL_EVAL_DIRECT") == nullptr;
 
     FeatTables tabs;
@@ -331,31 +330,36 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         auto evalTwoPass = [&]() {
             int N = 0;
             for (int a = 0; a < numActive; a++) N = std::max(N, active[a].begin + active[a].n);
-            // image-major order of the level: (tree, image) keys, stable radix sort of level indices
-            dKeys.resize(N);
-            dKeys2.resize(N);
-            dVals.resize(N);
-            dPerm.resize(N);
+            // image-major order of the level: (tree, image) keys, stable radix sort of level indices.  With
+            // Both samples a second entry per level index evaluates it flipped; the entries of samples that
+            // are not Both get the key one past the last (tree, image) and sort to the end.
+            const int M = anyBoth ? 2 * N : N;
+            dKeys.resize(M);
+            dKeys2.resize(M);
+            dVals.resize(M);
+            dPerm.resize(M);
             dLab8.resize(N);
+            const uint64_t keyRange = static_cast<uint64_t>(numActive) * static_cast<uint64_t>(images->count);
             for (int a = 0; a < numActive; a++) {
                 launchLevelKeys(s, level[cur].get(), active[a].begin, active[a].n,
-                        static_cast<uint32_t>(a) * static_cast<uint32_t>(images->count), dKeys.get(), dVals.get());
+                        static_cast<uint32_t>(a) * static_cast<uint32_t>(images->count), dKeys.get(), dVals.get(),
+                        anyBoth ? N : 0, static_cast<uint32_t>(keyRange));
             }
-            const uint64_t keyRange = static_cast<uint64_t>(numActive) * static_cast<uint64_t>(images->count);
             int bits = 1;
-            while ((1ull << bits) < keyRange) bits++;
-            const size_t sortBytes = radixSortTempBytes(N, bits);
+            while ((1ull << bits) <= keyRange) bits++;
+            const size_t sortBytes = radixSortTempBytes(M, bits);
             dSortTemp.resize(std::max<size_t>(sortBytes, 1));
-            radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), N, bits);
+            radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), M, bits);
             launchLevelLabels(s, level[cur].get(), N, dLab8.get());
             // (tree, image) segments of the sorted order -> blocks of <= 1024 samples within a segment
             const int numKeys = static_cast<int>(keyRange);
-            dStarts.resize(numKeys);
-            CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, numKeys * sizeof(int), s));
-            launchKeyStarts(s, dKeys2.get(), N, dStarts.get());
-            std::vector<int> starts(numKeys);
-            CF_HIP(hipMemcpyAsync(starts.data(), dStarts.get(), numKeys * sizeof(int), hipMemcpyDeviceToHost, s));
+            dStarts.resize(numKeys + 1);
+            CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, (numKeys + 1) * sizeof(int), s));
+            launchKeyStarts(s, dKeys2.get(), M, dStarts.get());
+            std::vector<int> starts(numKeys + 1);
+            CF_HIP(hipMemcpyAsync(starts.data(), dStarts.get(), (numKeys + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
+            if (starts[numKeys] < 0) starts[numKeys] = M;   // the end of the last segment
             std::vector<BinBlock> blocks;
             std::vector<std::pair<int, int>> segBlocks;   // per segment: first block, block count
             {
@@ -364,7 +368,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 while (k < numKeys) {
                     int k2 = k + 1;
                     while (k2 < numKeys && starts[k2] < 0) k2++;
-                    const int b = starts[k], e = k2 < numKeys ? starts[k2] : N;
+                    const int b = starts[k], e = starts[k2];
                     const int first = static_cast<int>(blocks.size());
                     for (int p = b; p < e; p += 1024) {
                         blocks.push_back(BinBlock{k / images->count, p, std::min(1024, e - p), k % images->count});
@@ -416,9 +420,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             CF_HIP(hipMemcpyAsync(dSlotTree.get(), st.data(), numNodes * sizeof(int), hipMemcpyHostToDevice, s));
             // feature groups in batches that fit the bin-row budget; node slots in batches that fit the histogram budget
             const int groupsTotal = (F + kBinGroup - 1) / kBinGroup;
-            const size_t rowBytes = static_cast<size_t>(N) * sizeof(uint4);
+            // a group's rows: one per level index, and a second (flipped) one with Both samples
+            const size_t rowBytes = static_cast<size_t>(N) * sizeof(uint4) * (anyBoth ? 2 : 1);
             const int ng = static_cast<int>(std::max<size_t>(1, std::min<size_t>(groupsTotal, binsBudget / rowBytes)));
-            dBins.resize(static_cast<size_t>(ng) * N);
+            const size_t flipOffset = anyBoth ? static_cast<size_t>(ng) * N : 0;
+            dBins.resize(static_cast<size_t>(ng) * N * (anyBoth ? 2 : 1));
             std::vector<int2> work;
             int workGroups = -1;
             for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
@@ -431,7 +437,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 }
                 timer.begin();
                 launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(), dWork.get(),
-                        static_cast<int>(work.size()), g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get());
+                        static_cast<int>(work.size()), g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get(), flipOffset);
                 timer.end(stats.evalKernelMs);
                 stats.evalLaunches += 1;
                 const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
@@ -447,7 +453,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     CF_HIP(hipMemsetAsync(dG.get(), 0, static_cast<size_t>(ns) * perSlotB * sizeof(uint32_t), s));
                     timer.begin();
                     launchHistBins(s, dBins.get(), dLab8.get(), dChunks.get(), static_cast<int>(bc.size()), g0, ngb, tabs,
-                            C, N, dG.get());
+                            C, N, dG.get(), flipOffset);
                     timer.end(stats.histMs);
                     timer.begin();
                     launchScores(s, dG.get(), dNodeHist.get() + static_cast<size_t>(s0) * C, dSlotTree.get() + s0, ns,
