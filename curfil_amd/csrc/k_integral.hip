@@ -126,6 +126,37 @@ __global__ void k_integral_kahan_f32(const float* __restrict__ color, int W, int
     }
 }
 
+// The same recurrence with the previous row of partial sums in LDS and the
+// left / above-left terms in registers (the values are the ones the
+// recurrence stored, so results are identical); one wave per channel.
+__global__ void __launch_bounds__(64) k_integral_kahan_f32_lds(const float* __restrict__ color, int W, int H,
+        float* __restrict__ col) {
+    extern __shared__ double rows[];   // [3][W]: row y-1 ahead of x, row y behind it
+    const int ch = threadIdx.x;
+    if (ch >= 3) return;
+    const size_t plane = static_cast<size_t>(W) * H;
+    const float* in = color + ch * plane;
+    float* out = col + ch * plane;
+    double* row = rows + static_cast<size_t>(ch) * W;
+    double c = 0.0;
+    for (int y = 0; y < H; ++y) {
+        double left = 0.0, aboveLeft = 0.0;
+        double above = y > 0 ? row[0] : 0.0;
+        for (int x = 0; x < W; ++x) {
+            const double nextAbove = (y > 0 && x + 1 < W) ? row[x + 1] : 0.0;   // not yet overwritten
+            const double cur = static_cast<double>(in[static_cast<size_t>(y) * W + x]);
+            const double dy = __dsub_rn(__dadd_rn(left, above), aboveLeft) - c;
+            const double t = __dadd_rn(cur, dy);
+            c = __dsub_rn(__dsub_rn(t, cur), dy);
+            row[x] = t;
+            out[static_cast<size_t>(y) * W + x] = __double2float_rn(t);
+            left = t;
+            aboveLeft = above;
+            above = nextAbove;
+        }
+    }
+}
+
 void launchIntegralU8(hipStream_t s, const uint8_t* rgb, const uint16_t* depth, int W, int H, uint32_t* tmp,
         float* col, int2* dep) {
     hipLaunchKernelGGL(k_integral_cols_u8, dim3((W + 255) / 256), dim3(256), 0, s, rgb, depth, W, H, tmp);
@@ -135,7 +166,12 @@ void launchIntegralU8(hipStream_t s, const uint8_t* rgb, const uint16_t* depth, 
 
 void launchIntegralF32(hipStream_t s, const float* color, const int32_t* depth, int W, int H, uint32_t* tmp,
         double* work, float* col, int2* dep) {
-    hipLaunchKernelGGL(k_integral_kahan_f32, dim3(1), dim3(64), 0, s, color, W, H, work, col);
+    const size_t lds = static_cast<size_t>(3) * W * sizeof(double);
+    if (lds <= 64 * 1024) {
+        hipLaunchKernelGGL(k_integral_kahan_f32_lds, dim3(1), dim3(64), lds, s, color, W, H, col);
+    } else {
+        hipLaunchKernelGGL(k_integral_kahan_f32, dim3(1), dim3(64), 0, s, color, W, H, work, col);
+    }
     hipLaunchKernelGGL(k_integral_cols_depth_i32, dim3((W + 255) / 256), dim3(256), 0, s, depth, W, H, tmp);
     const int jobs = H;  // depth job only
     hipLaunchKernelGGL(k_integral_rows, dim3((jobs * 64 + 255) / 256), dim3(256), 0, s, tmp, W, H, 3, col, dep);

@@ -55,10 +55,13 @@ def test_integrals_u8_bit_exact(ctx, W, H):
         assert np.array_equal(dep, odep) and np.array_equal(val, oval)
 
 
-def test_integrals_f32_kahan_bit_exact(ctx):
-    rng = np.random.default_rng(3)
-    color = (rng.integers(0, 1000, (1, 3, 48, 64)).astype(np.float32) / np.float32(100.0))
-    depth = rng.integers(0, 5000, (1, 48, 64)).astype(np.int32)
+@pytest.mark.parametrize("W,H", [(64, 48), (640, 480), (3000, 3)])
+def test_integrals_f32_kahan_bit_exact(ctx, W, H):
+    """the sequential Kahan recurrence (LDS row kernel, and the global-memory kernel for rows wider
+    than its LDS), with signed non-integer values as CIELab planes hold"""
+    rng = np.random.default_rng(W)
+    color = (rng.integers(-10000, 10000, (1, 3, H, W)).astype(np.float32) / np.float32(100.0))
+    depth = rng.integers(0, 5000, (1, H, W)).astype(np.int32)
     im = _images_f32(ctx, color, depth)
     col, dep, val = im.integrals(0)
     ocol, odep, oval = orc.Dataset(color, depth).integrals(0)
@@ -560,3 +563,25 @@ def test_command_line_tools_end_to_end(ctx, config0, tmp_path, capfd):
     m = re.search(r"pixel accuracy: ([0-9.]+)", log)
     assert m and abs(float(m.group(1)) - 100 * np.mean(accs)) < 1e-4, log[-2000:]
     assert "confusion matrix" in log
+
+
+def test_cielab_dataset_integrals_match_oracle(ctx, config0, tmp_path):
+    """a CIELab PNG dataset with one smaller image: its float planes integrate to the oracle's Kahan
+    integrals (padded with NaN / 0 outside the smaller image)"""
+    rgb, depth, labels, im, ds = config0
+    pal = np.array([[0, 0, 0], [200, 10, 10], [10, 200, 10], [10, 10, 200], [250, 250, 0], [0, 128, 255]],
+                   np.uint8)
+    sizes = [(160, 120), (100, 70)]
+    for i, (w, h) in enumerate(sizes):
+        cf.write_png(tmp_path / f"f{i}_colors.png", rgb[i][:h, :w])
+        cf.write_png(tmp_path / f"f{i}_depth.png", depth[i][:h, :w])
+        cf.write_png(tmp_path / f"f{i}_ground_truth.png", pal[labels[i][:h, :w]])
+    data = cf.load_images(str(tmp_path), useCIELab=True)
+    dim = data.to_images(ctx)
+    for i, (w, h) in enumerate(sizes):
+        lab = data.color(i)
+        ocol, odep, oval = orc.Dataset(lab[None], depth[i][:h, :w][None].astype(np.int32)).integrals(0)
+        col, dep, val = dim.integrals(i)
+        assert _same_float_bits(col[:, :h, :w].copy(), ocol)
+        assert np.array_equal(dep[:h, :w], odep) and np.array_equal(val[:h, :w], oval)
+        assert np.isnan(col[:, h:, :]).all() and np.isnan(col[:, :, w:]).all()
