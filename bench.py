@@ -206,6 +206,15 @@ def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):
     batched1_s = time.perf_counter() - t0
     log(f"plain (1 tree, depth 12, {int(plain_nodes)} nodes): {plain_s:.2f}s; batched: {batched1_s:.2f}s")
 
+    # the same tree with horizontal flipping (every root sample evaluated in both orientations)
+    cfgf = cf.TrainingConfiguration(**dict(cfg_kw, samplesPerImage=4000, maxDepth=12, horizontalFlipping=True))
+    ff = cf.RandomForestImage.sample(ctx, images, cfgf, 1, num_labels=5)
+    t0 = time.perf_counter()
+    ff.grow(cfgf, cf.MODE_BATCHED)
+    flip_s = time.perf_counter() - t0
+    pf = ff.profile()
+    log(f"horizontal flipping (1 tree, depth 12, {int(pf['nodes'])} nodes): {flip_s:.2f}s")
+
     # dense prediction with the trained forest
     forest.normalizeHistograms(0.0, True)
     batch = 64
@@ -225,7 +234,12 @@ def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):
         f"{np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms")
     plain_vs_batched = {"workload": "tree 0 only, depth 12", "plain_evals_per_s": plain_evals / plain_s,
                         "batched_evals_per_s": plain_evals / batched1_s,
-                        "speedup": plain_s / batched1_s, "nodes": int(plain_nodes)}
+                        "speedup": plain_s / batched1_s, "nodes": int(plain_nodes),
+                        "horizontal_flipping": {"seconds": flip_s, "nodes": int(pf["nodes"]),
+                                                "responses_per_s": pf["responses"] / flip_s,
+                                                "note": "batched two-pass; responses_per_s counts a sample "
+                                                        "once although Both samples are evaluated in both "
+                                                        "orientations"}}
     predict = {"frames": N, "trees": trees, "batch": batch, "mpix_per_s": mpix, "frames_per_s": N / pred_s,
                "single_frame_p50_ms": float(np.percentile(lat, 50)),
                "single_frame_p99_ms": float(np.percentile(lat, 99)),

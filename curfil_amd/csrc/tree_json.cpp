@@ -96,6 +96,8 @@ public:
 private:
     const std::string& t_;
     size_t i_ = 0;
+    int depth_ = 0;   // nesting; a tree of depth d nests about 2 d levels
+    static constexpr int kMaxDepth = 4096;
 
     [[noreturn]] void fail(const char* what) {
         throw Error(CF_ERR_ARG, std::string("tree file: JSON ") + what + " at offset " + std::to_string(i_));
@@ -156,6 +158,12 @@ private:
         return out;
     }
     JVal value() {
+        if (++depth_ > kMaxDepth) fail("nested too deeply");
+        JVal v = valueAt();
+        depth_--;
+        return v;
+    }
+    JVal valueAt() {
         JVal v;
         const char c = peek();
         if (c == '{') {

@@ -265,3 +265,10 @@ def test_forests_of_mixed_trees_are_rejected(tmp_path):
         p = tmp_path / "cut.json"
         p.write_text(open(FIXTURE).read()[:500])
         cf.RandomForestImage.from_files([p])
+
+
+def test_deeply_nested_file_is_rejected_not_crashing(tmp_path):
+    p = tmp_path / "deep.json"
+    p.write_text('{"tree": ' + "[" * 100000 + "]" * 100000 + "}")
+    with pytest.raises(cf.CurfilError, match="nested too deeply"):
+        cf.RandomForestImage.from_files([p])
