@@ -6,7 +6,7 @@
 // 2^53, so its result is the exact sum rounded once to float (colour) or the
 // exact integer (depth, valid).  Here the sums are exact 32-bit integer prefix
 // sums (wrapping for depth, see boxDepth), built in two passes:
-//   pass A: one thread per column walks down the rows (coalesced along x);
+//   pass A: column prefix sums, 32 row segments per column (coalesced along x);
 //   pass B: one wave per row scans 64-pixel segments with DPP/shuffle scans.
 // Float colour (non-integer values): the reference's sequential recurrence is
 // replayed exactly, one thread per channel (used for small test images).
@@ -14,26 +14,58 @@
 
 namespace cf {
 
-// pass A: column prefix of 5 channels (r, g, b, depth mm, valid) into tmp[5][H][W]
-__global__ void __launch_bounds__(256) k_integral_cols_u8(const uint8_t* __restrict__ rgb,
+// pass A: column prefix of 5 channels (r, g, b, depth mm, valid) into tmp[5][H][W].
+// A workgroup takes 32 columns; its 32 x 32 lanes split the rows into 32
+// segments: each lane sums its segment, the segment totals meet in LDS, and
+// each lane writes its segment's prefix sums starting from the totals of the
+// segments above (integer sums, so the order of additions does not matter).
+constexpr int kColChunk = 32, kRowSegs = 32;
+
+__global__ void __launch_bounds__(kColChunk * kRowSegs) k_integral_cols_u8(const uint8_t* __restrict__ rgb,
         const uint16_t* __restrict__ depth, int W, int H, uint32_t* __restrict__ tmp) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= W) return;
-    uint32_t r = 0, g = 0, b = 0, d = 0, v = 0;
+    __shared__ uint32_t total[kRowSegs][5][kColChunk];
+    const int cx = threadIdx.x % kColChunk, seg = threadIdx.x / kColChunk;
+    const int x = blockIdx.x * kColChunk + cx;
+    const int rowsPer = (H + kRowSegs - 1) / kRowSegs;
+    const int y0 = min(H, seg * rowsPer), y1 = min(H, y0 + rowsPer);
+    const bool col = x < W;
+    uint32_t sum[5] = {0u, 0u, 0u, 0u, 0u};
+    if (col) {
+#pragma unroll 4
+        for (int y = y0; y < y1; y++) {
+            const size_t p = static_cast<size_t>(y) * W + x;
+            const uint32_t mm = depth[p];
+            sum[0] += rgb[3 * p + 0];
+            sum[1] += rgb[3 * p + 1];
+            sum[2] += rgb[3 * p + 2];
+            sum[3] += mm;
+            sum[4] += (mm > 0) ? 1u : 0u;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) total[seg][c][cx] = sum[c];
+    __syncthreads();
+    uint32_t run[5] = {0u, 0u, 0u, 0u, 0u};
+    for (int k = 0; k < seg; k++) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) run[c] += total[k][c][cx];
+    }
+    if (!col) return;
     const size_t plane = static_cast<size_t>(W) * H;
-    for (int y = 0; y < H; y++) {
+#pragma unroll 4
+    for (int y = y0; y < y1; y++) {
         const size_t p = static_cast<size_t>(y) * W + x;
-        r += rgb[3 * p + 0];
-        g += rgb[3 * p + 1];
-        b += rgb[3 * p + 2];
         const uint32_t mm = depth[p];
-        d += mm;
-        v += (mm > 0) ? 1u : 0u;
-        tmp[p] = r;
-        tmp[plane + p] = g;
-        tmp[2 * plane + p] = b;
-        tmp[3 * plane + p] = d;
-        tmp[4 * plane + p] = v;
+        run[0] += rgb[3 * p + 0];
+        run[1] += rgb[3 * p + 1];
+        run[2] += rgb[3 * p + 2];
+        run[3] += mm;
+        run[4] += (mm > 0) ? 1u : 0u;
+        tmp[p] = run[0];
+        tmp[plane + p] = run[1];
+        tmp[2 * plane + p] = run[2];
+        tmp[3 * plane + p] = run[3];
+        tmp[4 * plane + p] = run[4];
     }
 }
 
@@ -159,7 +191,8 @@ __global__ void __launch_bounds__(64) k_integral_kahan_f32_lds(const float* __re
 
 void launchIntegralU8(hipStream_t s, const uint8_t* rgb, const uint16_t* depth, int W, int H, uint32_t* tmp,
         float* col, int2* dep) {
-    hipLaunchKernelGGL(k_integral_cols_u8, dim3((W + 255) / 256), dim3(256), 0, s, rgb, depth, W, H, tmp);
+    hipLaunchKernelGGL(k_integral_cols_u8, dim3((W + kColChunk - 1) / kColChunk), dim3(kColChunk * kRowSegs), 0, s,
+            rgb, depth, W, H, tmp);
     const int jobs = H * 4;
     hipLaunchKernelGGL(k_integral_rows, dim3((jobs * 64 + 255) / 256), dim3(256), 0, s, tmp, W, H, 0, col, dep);
 }
