@@ -177,6 +177,11 @@ void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* 
         int numChunks, const FeatTables& tabs, int C, uint32_t* G);
 
 // ---- two-pass level evaluation (k_bins.hip)
+// LDS a workgroup may use on the current device (MI355X: 160 KB); kernels whose
+// histograms need more than the default 64 KB opt in before launch
+size_t deviceLdsLimit();
+void allowLds(const void* kernel, size_t bytes);
+
 constexpr int kBinGroup = 16;            // features per bin row (16 x 8-bit bins = one uint4 per sample)
 constexpr int kMaxBinThresholds = 255;   // bins 0..T must fit in 8 bits
 // a block of <= 1024 consecutive entries of the image-major permutation, all of one (tree, image)

@@ -21,6 +21,9 @@
 // pairs), so an XCD's L2 holds the planes in use.
 #include <hipcub/hipcub.hpp>
 
+#include <map>
+#include <mutex>
+
 #include "cf_internal.h"
 
 namespace cf {
@@ -303,6 +306,28 @@ __global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restric
     if (i < n) labels[i] = static_cast<uint8_t>(samples[i].label | (samples[i].flip == 2 ? 0x80 : 0));
 }
 
+size_t deviceLdsLimit() {
+    static std::mutex mu;
+    static std::map<int, size_t> limits;
+    int dev = 0;
+    CF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = limits.find(dev);
+    if (it != limits.end()) return it->second;
+    int v = 0;
+    CF_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return limits[dev] = static_cast<size_t>(v);
+}
+
+void allowLds(const void* kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    if (bytes > deviceLdsLimit()) {
+        throw Error(CF_ERR_UNSUPPORTED, "(thresholds + 1) x classes too large for the LDS histograms of 16 features (" +
+                std::to_string(bytes) + " bytes)");
+    }
+    CF_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+}
+
 int binThresholdWidth(int T) {
     static const int widths[] = {8, 12, 16, 20, 24, 32, 64, kMaxBinThresholds};
     for (int w : widths) {
@@ -379,7 +404,7 @@ void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, con
         int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G, size_t flipOffset) {
     if (numChunks == 0 || ng == 0) return;
     const size_t lds = static_cast<size_t>(kBinGroup) * (tabs.T + 1) * C * sizeof(uint32_t);
-    if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
+    allowLds(reinterpret_cast<const void*>(&k_hist_bins), lds);
     hipLaunchKernelGGL(k_hist_bins, dim3(static_cast<unsigned>(numChunks) * ng), dim3(256), lds, s, bins, labels,
             chunks, g0, ng, tabs.F, tabs.T, C, N, G, flipOffset);
 }

@@ -477,9 +477,15 @@ void launchEvalHistograms(hipStream_t s, const ImageSet& imgs, const cf_sample* 
     if (numChunks == 0) return;
     const int tiles = (tabs.F + kFeatTile - 1) / kFeatTile;
     const size_t lds = static_cast<size_t>(kFeatTile) * (tabs.T + 1) * C * sizeof(uint32_t);
-    if (lds > 64 * 1024) throw Error(CF_ERR_ARG, "thresholds x classes too large for the LDS histogram");
     const int groups = (numChunks + kXcds - 1) / kXcds;
     const dim3 grid(groups * kXcds * tiles), block(kEvalThreads);
+    const void* kernel = tabs.T <= 8 ? reinterpret_cast<const void*>(&k_eval_hist<8>)
+            : tabs.T <= 16 ? reinterpret_cast<const void*>(&k_eval_hist<16>)
+            : tabs.T <= 24 ? reinterpret_cast<const void*>(&k_eval_hist<24>)
+            : tabs.T <= 32 ? reinterpret_cast<const void*>(&k_eval_hist<32>)
+            : tabs.T <= 64 ? reinterpret_cast<const void*>(&k_eval_hist<64>)
+            : reinterpret_cast<const void*>(&k_eval_hist<kMaxThresholds>);
+    allowLds(kernel, lds);
     if (tabs.T <= 8) {
         hipLaunchKernelGGL(k_eval_hist<8>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples,
                 chunks, numChunks, tiles, tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, C, G);

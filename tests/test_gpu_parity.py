@@ -585,3 +585,19 @@ def test_cielab_dataset_integrals_match_oracle(ctx, config0, tmp_path):
         assert _same_float_bits(col[:, :h, :w].copy(), ocol)
         assert np.array_equal(dep[:h, :w], odep) and np.array_equal(val[:h, :w], oval)
         assert np.isnan(col[:, h:, :]).all() and np.isnan(col[:, :, w:]).all()
+
+
+@pytest.mark.parametrize("mode", [cf.MODE_PLAIN, cf.MODE_BATCHED])
+def test_many_thresholds_train_like_oracle(ctx, config0, mode):
+    """T = 255 (the widest 8-bit bins; node histograms of 16 features need 96 KB of LDS, above the 64 KB
+    default): trees equal the oracle's; histograms that exceed the device's LDS are refused with a message."""
+    rgb, depth, labels, im, ds = config0
+    params = dict(CONFIG0, featureCount=40, thresholds=255, maxDepth=5)
+    forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=1, num_labels=6,
+                                        mode=mode)
+    of = ds.train_forest(orc.config(**params), 1, 6)
+    _assert_trees_equal(*forest.tree(0), *of.tree(0))
+    # 32 classes (pixel-uniform sampling: most classes have no pixels): 256 x 32 bins of 16 features need 512 KB
+    wide = cf.TrainingConfiguration(**dict(params, subsamplingType="pixelUniform"))
+    with pytest.raises(cf.CurfilError, match="too large for the LDS"):
+        cf.RandomForestImage.train(ctx, im, wide, tree_count=1, num_labels=32, mode=mode)
