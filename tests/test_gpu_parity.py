@@ -601,3 +601,47 @@ def test_many_thresholds_train_like_oracle(ctx, config0, mode):
     wide = cf.TrainingConfiguration(**dict(params, subsamplingType="pixelUniform"))
     with pytest.raises(cf.CurfilError, match="too large for the LDS"):
         cf.RandomForestImage.train(ctx, im, wide, tree_count=1, num_labels=32, mode=mode)
+
+
+def test_thirty_two_classes_train_and_predict_like_oracle(ctx):
+    """32 label values (the reference's GPU limit): the class-wide score and classify paths equal the
+    oracle's trees, posteriors and per-pixel predictions."""
+    rgb, depth, labels = cf.synth_dataset(3, 96, 72, sites=40, classes=31, zero_frac=0.05, seed0=321)
+    im = _images_u8(ctx, rgb, depth, labels)
+    ds = orc.Dataset.from_u8(rgb, depth, labels)
+    params = dict(CONFIG0, samplesPerImage=3200, featureCount=60, thresholds=12, maxDepth=6, minSampleCount=8,
+                  subsamplingType="pixelUniform")
+    C = int(labels.max()) + 1
+    assert C == 32
+    forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=2, num_labels=C,
+                                        mode=cf.MODE_BATCHED)
+    of = ds.train_forest(orc.config(**params), 2, C)
+    forest.normalizeHistograms(0.0, True)
+    trees, posts = [], []
+    for t in range(2):
+        gn, gh = forest.tree(t)
+        on, oh = of.tree(t)
+        _assert_trees_equal(gn, gh, on, oh)
+        post = np.array([orc.normalize_histogram(h, of.prior(), 0.0, True) for h in oh])
+        assert np.array_equal(forest.posteriors(t), post)
+        trees.append(on)
+        posts.append(post)
+    gl, gp = forest.predict(im, 1)
+    ol, op = ds.predict(1, trees, posts, C)
+    assert np.array_equal(gl, ol)
+    assert _same_float_bits(np.nan_to_num(gp), np.nan_to_num(op))
+
+
+def test_colour_only_features_without_depth_images(ctx, config0):
+    """useDepthImages = false: colour features only, every pixel at the dummy depth of 1000 mm
+    (loadDummyDepthValues, image.cpp:379-394)."""
+    rgb, depth, labels, im, ds = config0
+    flat = np.full_like(depth, 1000)
+    im2 = _images_u8(ctx, rgb, flat, labels)
+    ds2 = orc.Dataset.from_u8(rgb, flat, labels)
+    params = dict(CONFIG0, useDepthImages=False, maxDepth=6)
+    forest = cf.RandomForestImage.train(ctx, im2, cf.TrainingConfiguration(**params), tree_count=1, num_labels=6)
+    of = ds2.train_forest(orc.config(**params), 1, 6)
+    gn, gh = forest.tree(0)
+    _assert_trees_equal(gn, gh, *of.tree(0))
+    assert np.all(gn["feature"][gn["is_leaf"] == 0][:, 0] == cf.COLOR)
