@@ -192,18 +192,18 @@ struct BinBlock {
     int image;   // the image of every sample in the block
 };
 void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder);
-// keys (tree * images + image) and values (level index) of one tree's level segment; with flipN > 0
-// also the flipped evaluation of every Both sample at flipN + index (value | kFlipEntry), and key
-// absentKey for the other samples there
+// keys ((tree * images + image) << yBits | y) and values (level index) of one tree's level segment;
+// with flipN > 0 also the flipped evaluation of every Both sample at flipN + index (value |
+// kFlipEntry), and segment absentKey for the other samples there
 constexpr uint32_t kFlipEntry = 0x80000000u;
 void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
-        uint32_t* vals, int flipN = 0, uint32_t absentKey = 0);
+        uint32_t* vals, int flipN, uint32_t absentKey, int yBits);
 // label per level index, bit 7 set for samples still set to Both (counted in both orientations)
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels);
 size_t radixSortTempBytes(int n, int bits);
 void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
         const uint32_t* valsIn, uint32_t* valsOut, int n, int bits);
-void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts);
+void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int yBits, int* starts);
 // bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order;
 // permutation entries with kFlipEntry are evaluated flipped into bins[flipOffset + g * N + index]
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,

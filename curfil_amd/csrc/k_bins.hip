@@ -281,23 +281,29 @@ __global__ void __launch_bounds__(64) k_feature_eval_order(const FeatRec* __rest
 // Radix-sort keys (tree * images + image) and values (level index) of one
 // tree's level segment.
 __global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict__ samples, int begin, int n,
-        uint32_t keyBase, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int flipN, uint32_t absentKey) {
+        uint32_t keyBase, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int flipN, uint32_t absentKey,
+        int yBits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const cf_sample x = samples[begin + i];
-    keys[begin + i] = keyBase + x.image;
+    // (tree, image) segment, then the sample's row: the samples of a row sit in one wave, where
+    // corner reads of a feature that fall on one cache line are merged
+    const uint32_t row = yBits > 0 ? x.y : 0u;
+    keys[begin + i] = ((keyBase + x.image) << yBits) | row;
     vals[begin + i] = static_cast<uint32_t>(begin + i);
     if (flipN > 0) {
-        keys[flipN + begin + i] = x.flip == 2 ? keyBase + x.image : absentKey;
+        keys[flipN + begin + i] = x.flip == 2 ? (((keyBase + x.image) << yBits) | row) : (absentKey << yBits);
         vals[flipN + begin + i] = static_cast<uint32_t>(begin + i) | kFlipEntry;
     }
 }
 
-// starts[key] = first position of key in the sorted keys (entries of absent keys are left alone)
-__global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__ keys, int n, int* __restrict__ starts) {
+// starts[segment] = first position of segment (key >> yBits) in the sorted keys (absent segments left alone)
+__global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__ keys, int n, int yBits,
+        int* __restrict__ starts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (i == 0 || keys[i] != keys[i - 1]) starts[keys[i]] = i;
+    const uint32_t seg = keys[i] >> yBits;
+    if (i == 0 || seg != (keys[i - 1] >> yBits)) starts[seg] = i;
 }
 
 __global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restrict__ samples, int n,
@@ -349,15 +355,15 @@ void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, ui
 }
 
 void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
-        uint32_t* vals, int flipN, uint32_t absentKey) {
+        uint32_t* vals, int flipN, uint32_t absentKey, int yBits) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals,
-            flipN, absentKey);
+            flipN, absentKey, yBits);
 }
 
-void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int* starts) {
+void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int yBits, int* starts) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_key_starts, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, starts);
+    hipLaunchKernelGGL(k_key_starts, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, yBits, starts);
 }
 
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels) {

@@ -340,13 +340,19 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             dPerm.resize(M);
             dLab8.resize(N);
             const uint64_t keyRange = static_cast<uint64_t>(numActive) * static_cast<uint64_t>(images->count);
+            int bits = 1;
+            while ((1ull << bits) <= keyRange) bits++;
+            // within a segment, samples ordered by row when the key still fits 32 bits (CURFIL_ROW_ORDER=0: off)
+            int yBits = 0;
+            while ((1 << yBits) < images->H) yBits++;
+            const char* rowOrder = std::getenv("CURFIL_ROW_ORDER");
+            if (bits + yBits > 32 || (rowOrder && std::strcmp(rowOrder, "0") == 0)) yBits = 0;
+            bits += yBits;
             for (int a = 0; a < numActive; a++) {
                 launchLevelKeys(s, level[cur].get(), active[a].begin, active[a].n,
                         static_cast<uint32_t>(a) * static_cast<uint32_t>(images->count), dKeys.get(), dVals.get(),
-                        anyBoth ? N : 0, static_cast<uint32_t>(keyRange));
+                        anyBoth ? N : 0, static_cast<uint32_t>(keyRange), yBits);
             }
-            int bits = 1;
-            while ((1ull << bits) <= keyRange) bits++;
             const size_t sortBytes = radixSortTempBytes(M, bits);
             dSortTemp.resize(std::max<size_t>(sortBytes, 1));
             radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), M, bits);
@@ -355,7 +361,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             const int numKeys = static_cast<int>(keyRange);
             dStarts.resize(numKeys + 1);
             CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, (numKeys + 1) * sizeof(int), s));
-            launchKeyStarts(s, dKeys2.get(), M, dStarts.get());
+            launchKeyStarts(s, dKeys2.get(), M, yBits, dStarts.get());
             std::vector<int> starts(numKeys + 1);
             CF_HIP(hipMemcpyAsync(starts.data(), dStarts.get(), (numKeys + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
