@@ -296,3 +296,42 @@ def test_tools_parse_options_like_the_reference(capfd):
     assert "illegal histogram bias" in capfd.readouterr().err
     assert cf.run_tool("train", ["f", "1", "1", "1", "1", "1", "1", "1", "1", "--mode", "cpu"]) == 1
     assert "not available" in capfd.readouterr().err
+
+
+REFDATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_testdata")
+
+
+def test_reference_testdata_pngs_decode_like_pil():
+    """the reference's own test images (src/testdata: 640x480 colour, 16-bit depth, ground truth)"""
+    names = sorted(os.listdir(REFDATA))
+    assert len(names) == 12
+    for n in names:
+        got = cf.read_png(os.path.join(REFDATA, n))
+        want = np.array(Image.open(os.path.join(REFDATA, n)))
+        assert np.array_equal(got.astype(np.int64), want.astype(np.int64)), n
+
+
+def test_reference_testdata_loads_as_the_reference_does():
+    ds = cf.load_images(REFDATA, useCIELab=True)
+    assert ds.count == 4 and (ds.width, ds.height) == (640, 480)
+    assert [os.path.basename(f) for f in ds.files] == ["testing1_colors.png", "training1_colors.png",
+                                                       "training2_colors.png", "training3_colors.png"]
+    # black (the padding colour) first, then the other four ground-truth colours as met
+    cols = [tuple(c) for c in ds.label_map.colors()]
+    assert cols[0] == (0, 0, 0) and sorted(cols) == [(0, 0, 0), (0, 0, 255), (50, 205, 50), (255, 0, 0),
+                                                     (255, 255, 255)]
+    rgb, depth, labels = ds.get(1)
+    assert depth[depth > 0].min() >= 1455 and depth.max() <= 9489
+    lab = ds.color(1)
+    assert np.isfinite(lab).all() and lab[0].min() >= 0.0 and lab[0].max() <= 100.0001
+
+
+def test_depth_filling_reference_known_answer(tmp_path):
+    # image_test.cpp:425-456: two valid pixels, (20, 30) at 1.5 m and (30, 20) at 2.5 m, in a 200 x 100
+    # image; after filling every row above 30 holds 2.5 m and the others 1.5 m
+    depth = np.zeros((100, 200), np.uint16)
+    depth[30, 20] = 1500
+    depth[20, 30] = 2500
+    _write_triple(str(tmp_path), "f", np.zeros((100, 200, 3), np.uint8), depth, np.zeros((100, 200, 3), np.uint8))
+    filled = cf.load_images(str(tmp_path), useDepthFilling=True).get(0)[1]
+    assert (filled[:30] == 2500).all() and (filled[30:] == 1500).all()

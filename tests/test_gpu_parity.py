@@ -6,6 +6,7 @@ chosen splits, tree structure and labels; scores compared against the oracle
 1e-5 relative.
 """
 import math
+import os
 import re
 
 import numpy as np
@@ -645,3 +646,35 @@ def test_colour_only_features_without_depth_images(ctx, config0):
     gn, gh = forest.tree(0)
     _assert_trees_equal(gn, gh, *of.tree(0))
     assert np.all(gn["feature"][gn["is_leaf"] == 0][:, 0] == cf.COLOR)
+
+
+REFDATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_testdata")
+
+
+def test_reference_testdata_trains_like_oracle_and_predicts(ctx):
+    """random_tree_image_test.cpp's trainTestGPU on the reference's own images (CIELab colour, seed 4711,
+    500 samples per image, 500 features, 50 thresholds, depth 10, box radius 127, region 16, min samples
+    100): the tree equals the oracle's on the same CIELab planes, and the prediction of testing1 is
+    reported with its pixel accuracy (the reference's test expects about 73%)."""
+    lm = cf.LabelMap()
+    files = [os.path.join(REFDATA, f"training{i}_colors.png") for i in (1, 2, 3)]
+    train = cf.Dataset.load(files, useCIELab=True, padding_color=(0, 0, 0), label_map=lm)
+    im = train.to_images(ctx)
+    C = len(lm)
+    params = dict(randomSeed=4711, samplesPerImage=500, featureCount=500, minSampleCount=100, maxDepth=10,
+                  boxRadius=127, regionSize=16, thresholds=50, maxImages=10)
+    forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=1, num_labels=C)
+    colors = np.stack([train.color(i) for i in range(3)])
+    depth = np.stack([train.get(i)[1] for i in range(3)]).astype(np.int32)
+    labels = np.stack([train.get(i)[2] for i in range(3)])
+    ds = orc.Dataset(colors, depth, labels)
+    of = ds.train_forest(orc.config(**params), 1, C)
+    gn, gh = forest.tree(0)
+    _assert_trees_equal(gn, gh, *of.tree(0))
+    test = cf.Dataset.load([os.path.join(REFDATA, "testing1_colors.png")], useCIELab=True, label_map=lm)
+    tim = test.to_images(ctx)
+    forest.normalizeHistograms(0.0, True)
+    pred, _ = forest.predict(tim, 0)
+    acc = cf.calculatePixelAccuracy(pred, test.get(0)[2], True, [])
+    print(f"testing1 pixel accuracy {100 * acc:.1f}% ({len(gn)} nodes)")
+    assert acc > 0.5
