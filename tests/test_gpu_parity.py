@@ -678,3 +678,29 @@ def test_reference_testdata_trains_like_oracle_and_predicts(ctx):
     acc = cf.calculatePixelAccuracy(pred, test.get(0)[2], True, [])
     print(f"testing1 pixel accuracy {100 * acc:.1f}% ({len(gn)} nodes)")
     assert acc > 0.5
+
+
+def test_reference_testdata_ensemble(ctx):
+    """random_tree_image_test.cpp's trainTestEnsemble (:172-237): 3 trees, 2000 samples per image, depth 14,
+    box radius 120, region 10, min samples 32, seed 64684263; trees equal the oracle's, and testing1's
+    pixel accuracy is reported (the reference's test expects about 76%)."""
+    lm = cf.LabelMap()
+    files = [os.path.join(REFDATA, f"training{i}_colors.png") for i in (1, 2, 3)]
+    train = cf.Dataset.load(files, useCIELab=True, padding_color=(0, 0, 0), label_map=lm)
+    im = train.to_images(ctx)
+    C = len(lm)
+    params = dict(randomSeed=64684263, samplesPerImage=2000, featureCount=500, minSampleCount=32, maxDepth=14,
+                  boxRadius=120, regionSize=10, thresholds=50, maxImages=3)
+    forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=3, num_labels=C)
+    colors = np.stack([train.color(i) for i in range(3)])
+    depth = np.stack([train.get(i)[1] for i in range(3)]).astype(np.int32)
+    labels = np.stack([train.get(i)[2] for i in range(3)])
+    of = orc.Dataset(colors, depth, labels).train_forest(orc.config(**params), 3, C)
+    for t in range(3):
+        _assert_trees_equal(*forest.tree(t), *of.tree(t))
+    test = cf.Dataset.load([os.path.join(REFDATA, "testing1_colors.png")], useCIELab=True, label_map=lm)
+    forest.normalizeHistograms(0.0, True)
+    pred, _ = forest.predict(test.to_images(ctx), 0)
+    acc = cf.calculatePixelAccuracy(pred, test.get(0)[2], True, [])
+    print(f"testing1 pixel accuracy {100 * acc:.1f}% (3 trees, {[len(forest.tree(t)[0]) for t in range(3)]} nodes)")
+    assert acc > 0.5
