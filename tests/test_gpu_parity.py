@@ -704,3 +704,26 @@ def test_reference_testdata_ensemble(ctx):
     acc = cf.calculatePixelAccuracy(pred, test.get(0)[2], True, [])
     print(f"testing1 pixel accuracy {100 * acc:.1f}% (3 trees, {[len(forest.tree(t)[0]) for t in range(3)]} nodes)")
     assert acc > 0.5
+
+
+def test_reference_testdata_export_import(ctx, tmp_path):
+    """import_export_test.cpp's testExportImport (:104-167) on training1 with colour 50,205,50 ignored: the
+    configuration, label priors and every tree read back from tree<N>.json.gz equal what was written."""
+    lm = cf.LabelMap()
+    train = cf.Dataset.load([os.path.join(REFDATA, "training1_colors.png")], useCIELab=True,
+                            padding_color=(50, 205, 50), label_map=lm)
+    im = train.to_images(ctx)
+    ign = lm.find((50, 205, 50))
+    cfg = cf.TrainingConfiguration(4711, 500, 500, 32, 10, 50, 10, 10, numThreads=4, maxImages=10, useCIELab=True,
+                                   ignored_labels=[ign])
+    forest = cf.RandomForestImage.train(ctx, im, cfg, tree_count=3, num_labels=len(lm))
+    paths = forest.export(tmp_path, cfg, palette=lm.colors(), folder_training=REFDATA)
+    back = cf.RandomForestImage.from_files(paths)
+    c = back.configuration
+    assert (c.randomSeed, c.samplesPerImage, c.featureCount, c.minSampleCount, c.maxDepth, c.boxRadius,
+            c.regionSize, c.thresholds, c.numThreads, c.maxImages) == (4711, 500, 500, 32, 10, 50, 10, 10, 4, 10)
+    assert c.ignored_labels == [ign] and c.useCIELab
+    assert back.palette.tolist() == lm.colors().tolist()
+    for t in range(3):
+        assert np.array_equal(back.prior(t), forest.prior())
+        _assert_trees_equal(*back.tree(t), *forest.tree(t))
