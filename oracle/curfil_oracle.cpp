@@ -882,6 +882,17 @@ void orc_sampler_draws(int seed, int lo, int hi, int n, int* out) {
 }
 
 // -- integrals ----------------------------------------------------------------
+// ReservoirSampler<int> (random_tree.h:1069-1122) of the values 0..n-1 with
+// RandomSource(seed).uniformSampler(lo, hi), as random_tree_test.cpp:156-209 runs it.
+int orc_reservoir(int seed, int lo, int hi, int n, int k, int* out) {
+    RandomSource rs(seed);
+    Sampler sampler = rs.uniformSampler(lo, hi);
+    Reservoir<int> r(static_cast<size_t>(k));
+    for (int i = 0; i < n; i++) r.sample(sampler, i);
+    std::copy(r.reservoir.begin(), r.reservoir.end(), out);
+    return static_cast<int>(r.reservoir.size());
+}
+
 void orc_integral_color(const float* in, int W, int H, float* out) { integrateColorChannel(in, W, H, out); }
 void orc_integral_int(const int32_t* in, int W, int H, int32_t* out) { integrateIntChannel(in, W, H, out); }
 

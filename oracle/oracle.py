@@ -69,6 +69,8 @@ def _load():
     lib.orc_forest_root_count.argtypes = [P, I]
     lib.orc_forest_roots.argtypes = [P, I, P]
     lib.orc_forest_prior.argtypes = [P, P]
+    lib.orc_reservoir.argtypes = [I, I, I, I, I, P]
+    lib.orc_reservoir.restype = I
     lib.orc_forest_tree_prior.argtypes = [P, I, P]
     lib.orc_train_tree.argtypes = [P, P, I, I, I, P, I, I, P, P, P, I]
     lib.orc_predict.argtypes = [P, I, I, P, P, P, I, P, P]
@@ -279,6 +281,13 @@ def normalize_histogram(hist, prior, bias, use_prior):
     out = np.zeros(len(hist), np.float64)
     lib.orc_normalize_histogram(_p(hist), _p(prior), len(hist), bias, int(use_prior), _p(out))
     return out
+
+
+def reservoir(seed, lo, hi, n, k):
+    """ReservoirSampler of 0..n-1 keeping k, drawing from RandomSource(seed).uniformSampler(lo, hi)"""
+    out = np.zeros(k, np.int32)
+    got = lib.orc_reservoir(seed, lo, hi, n, k, _p(out))
+    return out[:got]
 
 
 def sampler_draws(seed, lo, hi, n):

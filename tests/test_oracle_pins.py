@@ -327,3 +327,12 @@ def test_normalize_histogram_known_answers(hist, prior, bias, expect):
     out = orc.normalize_histogram(hist, prior, bias, True)
     for o, e in zip(out, expect):
         assert abs(o - e) <= 1e-15 * max(abs(e), 1e-300) or o == e
+
+
+def test_reservoir_sampler_reference_statistics():
+    # random_tree_test.cpp:156-209: 1000 of 0..99999 with RandomSource(4711).uniformSampler(0, 10 * MAX)
+    r = orc.reservoir(4711, 0, 10 * 100000, 100000, 1000).astype(np.float64)
+    assert len(r) == 1000 and len(np.unique(r)) == 1000
+    assert r.min() < 0.01 * 100000 and r.max() > 0.99 * 100000
+    assert abs(r.mean() - 50000.0) <= 0.0005 * 50000.0           # BOOST_REQUIRE_CLOSE(mean, 50000, 0.05%)
+    assert abs(r.std() - 28000.0) <= 0.02 * 28000.0              # BOOST_REQUIRE_CLOSE(stddev, 28000, 2%)
