@@ -155,6 +155,19 @@ extern "C" int cf_tool_train(int argc, char** argv) {
                     " leaves, depth " + std::to_string(depth));
         }
         for (const auto& fc : featureCounts) cli::info("feature " + fc.first + ": " + std::to_string(fc.second));
+        if (o.flag("profile")) {
+            // per-phase device time, as the reference's PROFILING lines (utils.h:159-203)
+            double p[12] = {0};
+            const int n = cf_forest_profile(forest, p, 12);
+            const char* names[12] = {"evaluations", "seconds", "response kernel ms", "response kernel launches",
+                    "responses", "feature generation ms", "scores ms", "partition ms", "levels", "replays", "nodes",
+                    "node histograms ms"};
+            for (int i = 0; i < n && i < 12; i++) {
+                std::ostringstream m;
+                m << "PROFILING('" << names[i] << "'): " << p[i];
+                cli::info(m.str());
+            }
+        }
 
         // RandomTreeExport::writeJSON: <outputFolder>/tree<N>.json.gz
         if (!outputFolder.empty()) {
