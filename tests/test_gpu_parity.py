@@ -707,7 +707,7 @@ def test_reference_testdata_ensemble(ctx):
 
 
 def test_reference_testdata_export_import(ctx, tmp_path):
-    """import_export_test.cpp's testExportImport (:104-167) on training1 with colour 50,205,50 ignored: the
+    """import_export_test.cpp's testExportImport (:103-166) on training1 with colour 50,205,50 ignored: the
     configuration, label priors and every tree read back from tree<N>.json.gz equal what was written."""
     lm = cf.LabelMap()
     train = cf.Dataset.load([os.path.join(REFDATA, "training1_colors.png")], useCIELab=True,
