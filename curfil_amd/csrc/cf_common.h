@@ -228,15 +228,11 @@ __device__ __forceinline__ double depthResponseFromCorners(const int2* d) {
 // so the split test `r <= t` of SplitFunction::split (random_tree.h:128) can be
 // evaluated in float without changing any decision.
 __device__ __forceinline__ float roundUpToFloat(double r) {
-    float f = static_cast<float>(r);
-    if (static_cast<double>(f) < r) {
-        uint32_t u = __float_as_uint(f);
-        if (f == 0.0f) u = 1u;                 // +0 or -0 -> smallest positive denormal
-        else if (u & 0x80000000u) u -= 1u;     // negative: toward zero
-        else u += 1u;                          // positive: away from zero
-        f = __uint_as_float(u);
-    }
-    return f;
+    const float f = static_cast<float>(r);
+    const uint32_t u = __float_as_uint(f);
+    // the next float up: +0 or -0 -> smallest positive denormal, negative -> toward zero, positive -> away
+    const uint32_t next = (f == 0.0f) ? 1u : ((u & 0x80000000u) ? u - 1u : u + 1u);
+    return (static_cast<double>(f) < r) ? __uint_as_float(next) : f;
 }
 
 // Node split record used by partition and prediction.

@@ -184,7 +184,9 @@ void allowLds(const void* kernel, size_t bytes);
 
 constexpr int kBinGroup = 16;            // features per bin row (16 x 8-bit bins = one uint4 per sample)
 constexpr int kMaxBinThresholds = 255;   // bins 0..T must fit in 8 bits
-// a block of <= 1024 consecutive entries of the image-major permutation, all of one (tree, image)
+// Samples per workgroup of pass 1 (one per lane of a 256-lane workgroup).
+constexpr int kBinBlockSamples = 256;
+// a block of <= kBinBlockSamples consecutive entries of the image-major permutation, all of one (tree, image)
 struct BinBlock {
     int tree;
     int begin;   // first position in the permutation

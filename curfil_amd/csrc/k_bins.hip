@@ -1,12 +1,13 @@
 // Two-pass split evaluation of a whole level (batched training):
 //
 //   k_eval_bins   pass 1, image-major: the level's samples ordered by
-//                 (tree, image) are cut into blocks of <= 1024 within each
-//                 (tree, image) segment; one workgroup per
-//                 (block, group of 16 features) computes every response's
-//                 interval among the feature's thresholds (the bin used by
-//                 the interval histograms) and stores the 16 bins of a sample
-//                 as one 16-byte row, bins[group][level index].
+//                 (tree, image, row) are cut into blocks of <= 256 within
+//                 each (tree, image) segment; one workgroup per
+//                 (block, group of 16 features), one sample per lane,
+//                 computes every response's interval among the feature's
+//                 thresholds (the bin used by the interval histograms) and
+//                 stores the 16 bins of a sample as one 16-byte row,
+//                 bins[group][level index].
 //   k_hist_bins   pass 2, node-major: one workgroup per (node chunk, group)
 //                 streams the chunk's bin rows and labels into LDS interval
 //                 histograms and adds them to the node's global bins.
@@ -19,6 +20,14 @@
 // groups of a batch before the next, and features are visited in an
 // order grouped by the integral planes they read (depth, then colour channel
 // pairs), so an XCD's L2 holds the planes in use.
+//
+// Pass 1 is bound by the vector memory path (L1 and L1 -> L2 requests), not
+// by arithmetic or HBM.  Fewer gathers in flight per CU run faster: two
+// workgroups of four waves per CU, each lane with its one sample's 8 corner
+// loads in flight, beat every configuration with more (more samples per
+// lane, more waves, larger blocks), which thrash the 32 KB vector L1 before
+// a box's second corner on the same line is read.  The launch reserves LDS
+// the kernel does not use to hold the CU at two workgroups.
 #include <hipcub/hipcub.hpp>
 
 #include <map>
@@ -31,7 +40,9 @@ namespace cf {
 namespace {
 
 constexpr int kBinThreads = 256;
-constexpr int kBinPerThread = 4;
+constexpr int kBinPerThread = 1;
+static_assert(kBinThreads * kBinPerThread == kBinBlockSamples, "one pass of a workgroup covers one block");
+constexpr int kBinWorkgroupsPerCu = 2;
 
 struct BinSample {
     double rinv;   // depthReciprocal(depth in metres)
@@ -40,13 +51,29 @@ struct BinSample {
 };
 
 // #{t : th[t] < q} over a row padded with +inf (never below q), T for NaN.
+// The thresholds hold no NaN and no -0 (k_bin_tables maps NaN to +inf and -0
+// to +0, which leaves every comparison unchanged), and q + 0 turns a -0
+// response into +0.  Then for finite q, th < q exactly when th - q is
+// negative: with denormals kept, x - y is zero only for x == y and then +0,
+// it is -0 only for -0 - +0, and overflow keeps the sign.  So the sign bit of
+// th[t] - q counts the threshold, with no compare-and-select.  An infinite q
+// (a response beyond the float range) takes the comparisons, since
+// inf - inf is NaN.
 template<int TM>
 __device__ __forceinline__ int binOfT(float q, const float (&th)[TM], int T) {
-    int b = 0;
+    const float qc = q + 0.0f;
+    uint32_t b = 0;
 #pragma unroll
-    for (int t = 0; t < TM; t++) b += (th[t] < q) ? 1 : 0;
-    return (q != q) ? T : b;
+    for (int t = 0; t < TM; t++) b += __float_as_uint(th[t] - qc) >> 31;
+    if (__builtin_isinf(q)) {
+        b = 0;
+#pragma unroll
+        for (int t = 0; t < TM; t++) b += (th[t] < q) ? 1u : 0u;
+    }
+    return (q != q) ? T : static_cast<int>(b);
 }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Shifts a sample's 16-byte bin row right by one byte and inserts b as the
 // last byte: after 16 inserts the first inserted bin is byte 0.
@@ -172,15 +199,20 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
 #pragma unroll
         for (int k = 0; k < kBinPerThread; k++) {
             if (outIdx[k] >= 0) {
+                // rows are read once, by pass 2: streaming stores keep them from evicting image planes from L2
                 uint4* o = (rs[k].lab & (1 << 12)) ? outFlip : out;
-                o[outIdx[k]] = make_uint4(row[k][0], row[k][1], row[k][2], row[k][3]);
+                const u32x4 v = {row[k][0], row[k][1], row[k][2], row[k][3]};
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(o + outIdx[k]));
             }
         }
     }
 }
 
 // Feature records and thresholds in evaluation order, thresholds padded to
-// TM per feature with +inf: gFeat[tree][pos], gThr[tree][pos][TM].
+// TM per feature with +inf: gFeat[tree][pos], gThr[tree][pos][TM].  A NaN
+// threshold becomes +inf and -0 becomes +0, for binOfT's sign-bit count;
+// neither changes any comparison th < q (NaN < q and +inf < q are false for
+// every q, and -0 and +0 compare equal).
 __global__ void __launch_bounds__(256) k_bin_tables(const FeatRec* __restrict__ feat, const float* __restrict__ thrFT,
         const uint16_t* __restrict__ evalOrder, int total, int F, int T, int TM, FeatRec* __restrict__ gFeat,
         float* __restrict__ gThr) {
@@ -190,7 +222,8 @@ __global__ void __launch_bounds__(256) k_bin_tables(const FeatRec* __restrict__ 
     const size_t tf = static_cast<size_t>(tree) * F + evalOrder[i];
     gFeat[i] = feat[tf];
     for (int t = 0; t < TM; t++) {
-        gThr[static_cast<size_t>(i) * TM + t] = t < T ? thrFT[tf * T + t] : __builtin_inff();
+        const float v = t < T ? thrFT[tf * T + t] : __builtin_inff();
+        gThr[static_cast<size_t>(i) * TM + t] = (v != v) ? __builtin_inff() : v + 0.0f;
     }
 }
 
@@ -384,14 +417,37 @@ void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t*
     CF_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n, 0, bits, s));
 }
 
+// Dynamic LDS that k_eval_bins reserves (and does not use) so that exactly
+// kBinWorkgroupsPerCu workgroups fit a CU: more than 1 / (w + 1) and at most
+// 1 / w of the CU's LDS; 64 KB where that lies inside (160 KB CUs), else the
+// middle of the interval.
+static size_t evalBinsLdsBytes() {
+    static std::mutex mu;
+    static std::map<int, size_t> bytes;
+    int dev = 0;
+    CF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = bytes.find(dev);
+    if (it != bytes.end()) return it->second;
+    int perCu = 0;
+    CF_HIP(hipDeviceGetAttribute(&perCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    const size_t lo = static_cast<size_t>(perCu) / (kBinWorkgroupsPerCu + 1) + 1;
+    const size_t hi = static_cast<size_t>(perCu) / kBinWorkgroupsPerCu;
+    size_t b = 64 * 1024;
+    if (b < lo || b > hi) b = ((lo + hi) / 2) & ~static_cast<size_t>(255);
+    return bytes[dev] = b;
+}
+
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
         const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
         const FeatTables& tabs, int N, uint4* bins, size_t flipOffset) {
     if (numWork == 0) return;
     const int TM = binThresholdWidth(tabs.T);
     const dim3 grid(static_cast<unsigned>(numWork)), block(kBinThreads);
-#define CF_EVAL_BINS(TMV)                                                                                     \
-    hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, 0, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
+    const size_t lds = evalBinsLdsBytes();
+#define CF_EVAL_BINS(TMV)                                                                                        \
+    allowLds(reinterpret_cast<const void*>(&k_eval_bins<TMV>), lds);                                            \
+    hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
             blocks, work, g0, gFeat, gThr, tabs.F, tabs.T, N, bins, flipOffset)
     switch (TM) {
         case 8: CF_EVAL_BINS(8); break;

@@ -357,7 +357,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             dSortTemp.resize(std::max<size_t>(sortBytes, 1));
             radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), M, bits);
             launchLevelLabels(s, level[cur].get(), N, dLab8.get());
-            // (tree, image) segments of the sorted order -> blocks of <= 1024 samples within a segment
+            // (tree, image) segments of the sorted order -> blocks of <= kBinBlockSamples samples within a segment
             const int numKeys = static_cast<int>(keyRange);
             dStarts.resize(numKeys + 1);
             CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, (numKeys + 1) * sizeof(int), s));
@@ -376,8 +376,9 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     while (k2 < numKeys && starts[k2] < 0) k2++;
                     const int b = starts[k], e = starts[k2];
                     const int first = static_cast<int>(blocks.size());
-                    for (int p = b; p < e; p += 1024) {
-                        blocks.push_back(BinBlock{k / images->count, p, std::min(1024, e - p), k % images->count});
+                    for (int p = b; p < e; p += kBinBlockSamples) {
+                        blocks.push_back(BinBlock{k / images->count, p, std::min(kBinBlockSamples, e - p),
+                                k % images->count});
                     }
                     segBlocks.emplace_back(first, static_cast<int>(blocks.size()) - first);
                     k = k2;

@@ -38,3 +38,52 @@ def test_round_up_key_preserves_double_vs_float_comparison():
     q = np.array([_round_up_to_float(x) for x in r], np.float32)
     for t in thr:
         assert np.array_equal(r <= float(t), q <= t)
+
+
+def _specials():
+    f32 = np.float32
+    tiny = np.finfo(f32).tiny
+    den = np.nextafter(f32(0), f32(1))
+    big = np.finfo(f32).max
+    v = [0.0, -0.0, den, -den, 2 * den, tiny, -tiny, np.nextafter(tiny, f32(0)), 1.0, -1.0, big, -big,
+         np.nextafter(big, f32(0)), 1e-30, -1e-30, 0.5, 3.0, -3.0]
+    v = np.array(v, f32)
+    with np.errstate(over="ignore"):
+        return np.concatenate([v, np.nextafter(v, f32(np.inf)), np.nextafter(v, f32(-np.inf))]).astype(f32)
+
+
+def test_sign_bit_bin_count_equals_comparisons():
+    # k_eval_bins counts #{t : th[t] < q} as the sum of the sign bits of th[t] - (q + 0) over
+    # thresholds stored with NaN -> +inf and -0 -> +0 (k_bin_tables), for finite q (NaN and
+    # infinite q take other paths).  Host float32 arithmetic keeps denormals, as the kernels
+    # are compiled to.
+    rng = np.random.default_rng(7)
+    sp = _specials()
+    thr_raw = np.concatenate([sp, np.array([np.nan, -np.nan, np.inf, -np.inf], np.float32),
+                              rng.normal(0, 50, 400).astype(np.float32),
+                              (rng.normal(0, 1, 100) * 1e-40).astype(np.float32)])
+    with np.errstate(invalid="ignore", over="ignore"):
+        thr = np.where(np.isnan(thr_raw), np.float32(np.inf), thr_raw + np.float32(0)).astype(np.float32)
+        assert not np.any(np.signbit(thr) & (thr == 0))
+        q = np.concatenate([sp, rng.normal(0, 50, 3000).astype(np.float32),
+                            thr[np.isfinite(thr)], (rng.normal(0, 1, 300) * 1e-40).astype(np.float32)])
+        q = q[np.isfinite(q)].astype(np.float32)
+        qc = (q + np.float32(0)).astype(np.float32)
+        diff = (thr[None, :] - qc[:, None]).astype(np.float32)
+        sign = (diff.view(np.uint32) >> 31).astype(np.int64)
+        # the canonical thresholds compare as the stored ones do
+        assert np.array_equal(thr[None, :] < q[:, None], thr_raw[None, :] < q[:, None])
+        assert np.array_equal(sign.astype(bool), thr[None, :] < q[:, None])
+        assert np.array_equal(sign.sum(1), (thr[None, :] < q[:, None]).sum(1))
+
+
+def test_branch_free_next_float_up():
+    # roundUpToFloat's next float above f: +-0 -> smallest denormal, negative -> toward zero,
+    # positive -> away from zero, i.e. nextafter(f, +inf) for every f below +inf
+    rng = np.random.default_rng(3)
+    f = np.concatenate([_specials(), np.float32([-np.inf]), rng.normal(0, 1e3, 5000).astype(np.float32)])
+    f = f[f < np.inf]
+    u = f.view(np.uint32).astype(np.uint64)
+    nxt = np.where(f == 0, 1, np.where(u & 0x80000000, u - 1, u + 1)).astype(np.uint32).view(np.float32)
+    ref = np.nextafter(f, np.float32(np.inf))
+    assert np.array_equal(nxt.view(np.uint32), ref.view(np.uint32))
