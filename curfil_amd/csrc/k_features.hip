@@ -12,8 +12,9 @@
 // order, and how many draws a candidate consumes depends on which responses
 // are NaN; k_fg_walk keeps that order exactly: one workgroup per (tree, level)
 // owns the sample-index generator (MT19937 state twisted in parallel in LDS),
-// evaluates the next 8 candidates at the next 256 queued draws in parallel
-// and lets one lane walk them with the reference's retry rules.
+// evaluates the next 16 candidates, each in a band of 128 of the next 512
+// queued draws, in parallel, and lets one wave walk them with the reference's
+// retry rules.
 #include "cf_internal.h"
 #include "cf_rng.h"
 
@@ -99,13 +100,16 @@ __device__ __forceinline__ int featureSortKey(const FeatRec& f) {
     return k;
 }
 
-constexpr int kWalkCand = 8;    // candidates evaluated per round
-constexpr int kWalkDraws = 256; // queued draws per round (one per thread)
+constexpr int kWalkCand = 16;   // candidates evaluated per round
+constexpr int kWalkDraws = 512; // queued draws a round can reach
+constexpr int kWalkBand = 128;  // draws evaluated per candidate per round
+constexpr int kWalkWords = kWalkDraws / 64;
+static_assert(kDrawQueue >= kWalkDraws + 624, "the draw ring holds a round's window and one refill");
 
 // What the walk did with candidate j of a round: window positions
-// [begin, end) were consumed with st = 0 (still open at the window's end),
-// 1 (accepted into output slot `slot`) or 2 (rejected); t0 thresholds had
-// been found before the round.
+// [begin, end) were consumed with st = 0 (still open at its band's end),
+// 1 (accepted into output slot `slot`) or 2 (rejected, or not reached); t0
+// thresholds had been found before the round.
 struct WalkStep {
     int st, begin, end, t0, slot;
 };
@@ -114,13 +118,15 @@ struct WalkShared {
     uint32_t mt[624];
     int dq[kDrawQueue];
     float resp[kWalkCand][kWalkDraws];
-    unsigned long long finite[kWalkCand][kWalkDraws / 64];   // bit: response is not NaN
+    unsigned long long finite[kWalkCand][kWalkWords];   // bit: response is not NaN (set inside the band only)
     FeatRec cand[kWalkCand];
     WalkStep step[kWalkCand];
     int waveCounts[4];
     // walk state after a round: next draw, next candidate, features generated,
-    // thresholds found and tries left for a candidate that ran past the round
+    // thresholds found and tries left for a candidate that ran past the round;
+    // band spacing of the next round and the draws / candidates finished so far
     int head, k, generated, t, tries, err;
+    int spacing, drawsDone, candsDone;
 };
 
 // One MT19937 twist of the LDS state, in three dependency-free phases.
@@ -177,15 +183,19 @@ __device__ int refillDraws(WalkShared& sh, int tail, const UniformIntParams& up)
     return tail;
 }
 
-// Per round: the responses (as float) of the next kWalkCand candidates at the
-// next kWalkDraws queued draws, one draw per thread; then lane 0 walks the
-// candidates in order through those draws with the reference's rules
+// Per round: candidate j of the next kWalkCand is evaluated at the draws
+// [j S, j S + kWalkBand) of the next kWalkDraws queued draws (S: about 7/8 of
+// the draws a finished candidate has used, at least min(T, 10), the fewest a
+// candidate can use), 2 candidates x 2 draws per lane and wave; then wave 0
+// walks the candidates in order with the reference's rules
 // (random_tree_image.cpp:648-700: a NaN response costs one of 10 tries, a
 // finite one is a threshold and resets the tries; T thresholds accept the
 // candidate, 10 NaN in a row reject it).  A candidate's first draw is the one
-// after its predecessor's last, so every candidate of the round is evaluated
-// at every draw of the window; a candidate still open at the end of the
-// window carries its state into the next round, which starts with it.
+// after its predecessor's last.  The round ends at the first candidate whose
+// first draw lies outside its band or which runs to its band's end; that
+// candidate carries its state into the next round, which starts with it.
+// The bands only decide how far a round gets, never which draws a candidate
+// uses.
 __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ jobs, const DevImage* __restrict__ imgs,
         const FeatRec* __restrict__ cand, int numCand, int F, int T, FeatRec* __restrict__ outFeat,
         float* __restrict__ outThr, int* __restrict__ outKey, int* __restrict__ err) {
@@ -193,7 +203,9 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
     const FeatGenJob job = jobs[blockIdx.x];
     const FeatRec* myCand = cand + static_cast<size_t>(blockIdx.x) * numCand;
     const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     const UniformIntParams up = uniformIntParams(0, job.n - 1);
+    const int minUse = min(T, 10);
     if (tid == 0) {
         uint32_t v = static_cast<uint32_t>(job.seed);
         sh.mt[0] = v;
@@ -207,74 +219,104 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
         sh.t = 0;
         sh.tries = 10;
         sh.err = 0;
+        sh.spacing = minUse;
+        sh.drawsDone = 0;
+        sh.candsDone = 0;
     }
     __syncthreads();
+    constexpr int kPerWave = kWalkCand / 4;          // candidates per wave: wave, wave + 4, ...
+    constexpr int kReps = kWalkBand / 64;            // draws per lane and candidate
+    constexpr int kSlots = kPerWave * kReps;
     int tail = 0;
     for (;;) {
-        const int head = sh.head, k0 = sh.k;
+        const int head = sh.head, k0 = sh.k, S = sh.spacing;
         const bool done = sh.generated >= F || sh.err != 0;
         __syncthreads();
         if (done) break;
         while (tail - head < kWalkDraws) tail = refillDraws(sh, tail, up);
+        for (int i = tid; i < kWalkCand * kWalkWords; i += 256) (&sh.finite[0][0])[i] = 0ull;
+        if (tid < kWalkCand && k0 + tid < numCand) sh.cand[tid] = myCand[k0 + tid];
+        __syncthreads();
         {
-            // all candidates' corner gathers are issued before any response is formed
-            const int d = sh.dq[(head + tid) % kDrawQueue];
-            const cf_sample smp = job.samples[d];
-            const DevImage im = imgs[smp.image];
-            const bool valid = smp.depth_mm > 0;
-            const double rinv = valid ? depthReciprocal(depthToFloat(smp.depth_mm)) : 0.0;
-            const size_t plane = static_cast<size_t>(im.W) * im.H;
-            bool ok[kWalkCand];
-            bool isDepth[kWalkCand];
-            int2 cv[kWalkCand][8];   // depth corners, or colour corners' bits in .x
+            // all slots' corner gathers are issued before any response is formed
+            bool ok[kSlots], isDepth[kSlots], active[kSlots];
+            int2 cv[kSlots][8];   // depth corners, or colour corners' bits in .x
 #pragma unroll
-            for (int j = 0; j < kWalkCand; j++) {
-                ok[j] = false;
-                isDepth[j] = false;
-                if (k0 + j < numCand) {
-                    const Feat ff = loadFeat(myCand + k0 + j);
-                    int idx[8];
-                    isDepth[j] = ff.type == T_DEPTH;
-                    ok[j] = valid && responseCorners(ff, smp.x, smp.y, im.W, im.H, rinv, false, idx);
-                    if (ok[j]) {
-                        if (isDepth[j]) {
+            for (int q = 0; q < kPerWave; q++) {
+                const int c = wave + 4 * q;
+                const bool candOk = k0 + c < numCand;
+                Feat ff{};
+                if (candOk) ff = loadFeat(&sh.cand[c]);
 #pragma unroll
-                            for (int c = 0; c < 8; c++) cv[j][c] = planeAt(im.dep, idx[c]);
-                        } else {
-                            const float* colA = im.col + static_cast<size_t>(ff.c1) * plane;
-                            const float* colB = im.col + static_cast<size_t>(ff.c2) * plane;
+                for (int r = 0; r < kReps; r++) {
+                    const int i = q * kReps + r;
+                    const int dpos = c * S + 64 * r + lane;
+                    active[i] = candOk && dpos < kWalkDraws;
+                    ok[i] = false;
+                    isDepth[i] = ff.type == T_DEPTH;
+                    if (active[i]) {
+                        const int d = sh.dq[(head + dpos) % kDrawQueue];
+                        const cf_sample smp = job.samples[d];
+                        const DevImage im = imgs[smp.image];
+                        const bool valid = smp.depth_mm > 0;
+                        const double rinv = valid ? depthReciprocal(depthToFloat(smp.depth_mm)) : 0.0;
+                        int idx[8];
+                        ok[i] = valid && responseCorners(ff, smp.x, smp.y, im.W, im.H, rinv, false, idx);
+                        if (ok[i]) {
+                            if (isDepth[i]) {
 #pragma unroll
-                            for (int c = 0; c < 4; c++) cv[j][c].x = __float_as_int(planeAt(colA, idx[c]));
+                                for (int e = 0; e < 8; e++) cv[i][e] = planeAt(im.dep, idx[e]);
+                            } else {
+                                const size_t plane = static_cast<size_t>(im.W) * im.H;
+                                const float* colA = im.col + static_cast<size_t>(ff.c1) * plane;
+                                const float* colB = im.col + static_cast<size_t>(ff.c2) * plane;
 #pragma unroll
-                            for (int c = 4; c < 8; c++) cv[j][c].x = __float_as_int(planeAt(colB, idx[c]));
+                                for (int e = 0; e < 4; e++) cv[i][e].x = __float_as_int(planeAt(colA, idx[e]));
+#pragma unroll
+                                for (int e = 4; e < 8; e++) cv[i][e].x = __float_as_int(planeAt(colB, idx[e]));
+                            }
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int j = 0; j < kWalkCand; j++) {
-                float v = __builtin_nanf("");
-                if (ok[j]) {
-                    if (isDepth[j]) {
-                        v = static_cast<float>(depthResponseFromCorners(cv[j]));
-                    } else {
-                        float c8[8];
+            for (int q = 0; q < kPerWave; q++) {
+                const int c = wave + 4 * q;
 #pragma unroll
-                        for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[j][c].x);
-                        v = static_cast<float>(colorResponseFromCorners(c8));
+                for (int r = 0; r < kReps; r++) {
+                    const int i = q * kReps + r;
+                    const int start = c * S + 64 * r;   // draw of lane 0
+                    const int dpos = start + lane;
+                    float v = __builtin_nanf("");
+                    if (ok[i]) {
+                        if (isDepth[i]) {
+                            v = static_cast<float>(depthResponseFromCorners(cv[i]));
+                        } else {
+                            float c8[8];
+#pragma unroll
+                            for (int e = 0; e < 8; e++) c8[e] = __int_as_float(cv[i][e].x);
+                            v = static_cast<float>(colorResponseFromCorners(c8));
+                        }
+                    }
+                    if (active[i]) sh.resp[c][dpos] = v;
+                    // the wave's 64 draws start at `start`, which need not be word aligned
+                    const unsigned long long fm = __ballot(active[i] && !__builtin_isnan(v));
+                    if (lane == 0 && fm != 0ull) {
+                        const int w = start >> 6, off = start & 63;
+                        atomicOr(&sh.finite[c][w], fm << off);
+                        if (off != 0 && w + 1 < kWalkWords) atomicOr(&sh.finite[c][w + 1], fm >> (64 - off));
                     }
                 }
-                sh.resp[j][tid] = v;
-                const unsigned long long fm = __ballot(!__builtin_isnan(v));
-                if ((tid & 63) == 0) sh.finite[j][tid >> 6] = fm;
             }
-            if (tid < kWalkCand && k0 + tid < numCand) sh.cand[tid] = myCand[k0 + tid];
         }
         __syncthreads();
         if (tid < 64) {
-            // wave 0 walks the finite-response bits with wave-uniform values (scalar
-            // unit): a NaN run costs one try per draw, a finite response is the next
-            // threshold and resets the tries
+            // wave 0 walks the finite-response bits a 64-draw word at a time: lane q
+            // stands for draw q of the word and tests whether the walk stops there
+            // (the T-th threshold, or the NaN that uses up the tries: a NaN run
+            // costs one try per draw, a finite response is the next threshold and
+            // resets the tries to 10); the first such lane ends the candidate
+            const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));   // bits [0, lane)
             int p = 0;
             int k = k0;
             int gen = __builtin_amdgcn_readfirstlane(sh.generated);
@@ -282,6 +324,9 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
             int tries = __builtin_amdgcn_readfirstlane(sh.tries);
             if (tid < kWalkCand) sh.step[tid].st = 2;   // not walked: nothing to copy
             while (k < k0 + kWalkCand && gen < F) {
+                const int j = k - k0;
+                const int lo = j * S, hi = min(lo + kWalkBand, kWalkDraws);
+                if (p < lo || p >= hi) break;   // its first draw was not evaluated: the next round starts here
                 // `--maxLoops <= 0` with maxLoops = 3F, checked right after sampling candidate k
                 if (k + 1 >= 3 * F) {
                     if (tid == 0) {
@@ -290,36 +335,34 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
                     }
                     break;
                 }
-                const int j = k - k0;
                 const int begin = p, t0 = t, slot = job.outOffset + gen;
                 int st = 0;
-                while (st == 0 && p < kWalkDraws) {
-                    const unsigned long long word = sh.finite[j][p >> 6];
-                    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(word));
-                    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(word >> 32));
-                    unsigned long long m = ((static_cast<unsigned long long>(hi) << 32) | lo) >> (p & 63);
-                    int rem = 64 - (p & 63);
-                    while (rem > 0) {
-                        const int z = m ? min(__builtin_ctzll(m), rem) : rem;   // NaN draws before the next finite
-                        if (z >= tries) {
-                            p += tries;
-                            st = 2;
-                            break;
-                        }
-                        tries -= z;
-                        p += z;
-                        rem -= z;
-                        if (rem == 0) break;
-                        m >>= z;
-                        t++;               // the finite draw at p
-                        tries = 10;
-                        p++;
-                        rem--;
-                        m >>= 1;
-                        if (t == T) {
-                            st = 1;
-                            break;
-                        }
+                while (st == 0 && p < hi) {
+                    const int w = p >> 6, o = p & 63;
+                    const int e = min(64, hi - (w << 6));   // lanes [o, e) lie in the band
+                    const unsigned long long word = sh.finite[j][w];
+                    const uint32_t lo32 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(word));
+                    const uint32_t hi32 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(word >> 32));
+                    const unsigned long long inBand = (~0ull << o) & (e == 64 ? ~0ull : ((1ull << e) - 1ull));
+                    const unsigned long long m = ((static_cast<unsigned long long>(hi32) << 32) | lo32) & inBand;
+                    // finite draws in [p, q) and the NaN run that ends at q: it starts after the last of
+                    // them and is held to 10 tries, or, with none, starts at p with the tries carried in
+                    const unsigned long long prevF = m & below;
+                    const bool fin = (m >> lane) & 1ull;
+                    const int run = prevF ? lane - (63 - __clzll(prevF)) : lane - o + 1;
+                    const bool accept = fin && t + __popcll(prevF) + 1 == T;
+                    const bool reject = lane >= o && lane < e && !fin && run == (prevF ? 10 : tries);
+                    const unsigned long long stop = __ballot(accept || reject);
+                    if (stop) {
+                        const int q = __builtin_ctzll(stop);
+                        st = ((m >> q) & 1ull) ? 1 : 2;
+                        p = (w << 6) + q + 1;
+                    } else {
+                        // no stop before e: its thresholds count, and the NaN draws after the last
+                        // finite one (fewer than the tries left) carry on
+                        t += __popcll(m);
+                        tries = m ? 10 - (e - 1 - (63 - __clzll(m))) : tries - (e - o);
+                        p = (w << 6) + e;
                     }
                 }
                 if (tid == 0) {
@@ -336,7 +379,7 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
                         outKey[slot] = featureSortKey(fr);
                     }
                 }
-                if (st == 0) break;   // the window ran out inside candidate k
+                if (st == 0) break;   // the band ran out inside candidate k
                 if (st == 1) gen++;
                 k++;
                 t = 0;
@@ -348,24 +391,40 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
                 sh.generated = gen;
                 sh.t = t;
                 sh.tries = tries;
+                // draws and candidates finished this round (an open candidate's draws count with it later)
+                int used = p;
+                if (k < k0 + kWalkCand) {
+                    const WalkStep& open = sh.step[k - k0];
+                    if (open.st == 0) used = open.begin;
+                }
+                const int drawsDone = sh.drawsDone + used, candsDone = sh.candsDone + (k - k0);
+                sh.drawsDone = drawsDone;
+                sh.candsDone = candsDone;
+                sh.spacing = candsDone > 0 ? max(minUse, (7 * drawsDone) / (8 * candsDone)) : minUse;
             }
         }
         __syncthreads();
         // thresholds of the accepted and still open candidates: the finite
         // draws of [begin, end) in order, after the t0 found earlier
 #pragma unroll
-        for (int j = 0; j < kWalkCand; j++) {
-            const WalkStep ws = sh.step[j];
-            if (ws.st == 2 || tid < ws.begin || tid >= ws.end) continue;
-            if (!((sh.finite[j][tid >> 6] >> (tid & 63)) & 1ull)) continue;
-            int before = 0;   // finite draws in [begin, tid)
-            for (int w = ws.begin >> 6; w <= (tid >> 6); w++) {
-                unsigned long long m = sh.finite[j][w];
-                if (w == (ws.begin >> 6)) m &= ~0ull << (ws.begin & 63);
-                if (w == (tid >> 6)) m &= (1ull << (tid & 63)) - 1ull;
-                before += __popcll(m);
+        for (int q = 0; q < kPerWave; q++) {
+            const int c = wave + 4 * q;
+            const WalkStep ws = sh.step[c];
+            if (ws.st == 2) continue;
+#pragma unroll
+            for (int r = 0; r < kReps; r++) {
+                const int dpos = c * S + 64 * r + lane;
+                if (dpos < ws.begin || dpos >= ws.end) continue;
+                if (!((sh.finite[c][dpos >> 6] >> (dpos & 63)) & 1ull)) continue;
+                int before = 0;   // finite draws in [begin, dpos)
+                for (int w = ws.begin >> 6; w <= (dpos >> 6); w++) {
+                    unsigned long long m = sh.finite[c][w];
+                    if (w == (ws.begin >> 6)) m &= ~0ull << (ws.begin & 63);
+                    if (w == (dpos >> 6)) m &= (1ull << (dpos & 63)) - 1ull;
+                    before += __popcll(m);
+                }
+                outThr[static_cast<size_t>(ws.slot) * T + ws.t0 + before] = sh.resp[c][dpos];
             }
-            outThr[static_cast<size_t>(ws.slot) * T + ws.t0 + before] = sh.resp[j][tid];
         }
         __syncthreads();
     }

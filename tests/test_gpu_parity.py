@@ -240,8 +240,12 @@ def test_golden_feature_hash_on_device(ctx):
     assert boost_feature_hash(feats, thr) == GOLDEN_FEATURE_HASH
 
 
-@pytest.mark.parametrize("seed,n,F,T", [(17, 4000, 100, 10), (3, 1, 20, 4), (123456, 50, 300, 32), (99, 2500, 2000, 20)])
-def test_feature_generation_matches_oracle(ctx, seed, n, F, T):
+# boxRadius 127 on 160x120 frames: most responses NaN, so candidates are rejected and NaN runs carry
+# across 64-draw words and walk rounds; T = 64 spans several words per candidate
+@pytest.mark.parametrize("seed,n,F,T,br,rs", [(17, 4000, 100, 10, 60, 10), (3, 1, 20, 4, 60, 10),
+                                              (123456, 50, 300, 32, 60, 10), (99, 2500, 2000, 20, 60, 10),
+                                              (5, 3000, 400, 40, 127, 20), (11, 800, 200, 64, 127, 4)])
+def test_feature_generation_matches_oracle(ctx, seed, n, F, T, br, rs):
     rgb, depth, labels = config0_dataset()
     im = _images_u8(ctx, rgb, depth, labels)
     ds = orc.Dataset.from_u8(rgb, depth, labels)
@@ -251,8 +255,8 @@ def test_feature_generation_matches_oracle(ctx, seed, n, F, T):
     if len(s) == 0 or s["depth_mm"].min() <= 0:
         s = s[:0]
         s = random_samples(rng, 1, 4, 160, 120, np.full_like(depth, 1000, dtype=np.int32))
-    gf, gt = cf.generate_features(ctx, im, s, seed, F, T, 60, 10)
-    of, ot = ds.generate_features(s, seed, F, T, 60, 10)
+    gf, gt = cf.generate_features(ctx, im, s, seed, F, T, br, rs)
+    of, ot = ds.generate_features(s, seed, F, T, br, rs)
     assert np.array_equal(gf, of)
     assert _same_float_bits(gt, ot)
 
