@@ -206,6 +206,14 @@ size_t radixSortTempBytes(int n, int bits);
 void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
         const uint32_t* valsIn, uint32_t* valsOut, int n, int bits);
 void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int yBits, int* starts);
+// Work list of pass 1 for ngb feature groups, built on the device: segment i (its blocks
+// segFirst[i] .. + segCount[i]) belongs to XCD i % 8; XCD x takes its segments in order, all groups
+// of a segment (group major, block minor) before the next, and workgroup w of the launch runs on
+// XCD w % 8, so work[pos * 8 + x] is XCD x's item pos, (-1, 0) past its end.  xcdPrefix[i] counts
+// the blocks of XCD x's segments before segment i; xcdBlocks[x] all of them; maxLen is the longest
+// XCD list in items.
+void launchBuildWork(hipStream_t s, const int* segFirst, const int* segCount, const int* xcdPrefix,
+        const int* xcdBlocks, int numSegs, int ngb, int maxLen, int2* work);
 // bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order;
 // permutation entries with kFlipEntry are evaluated flipped into bins[flipOffset + g * N + index]
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,

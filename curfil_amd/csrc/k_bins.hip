@@ -330,6 +330,33 @@ __global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict_
     }
 }
 
+// One thread per (XCD list position, XCD): the segment holding the position by binary search over
+// the XCD's block prefix counts, then group and block within the segment.
+__global__ void __launch_bounds__(256) k_build_work(const int* __restrict__ segFirst, const int* __restrict__ segCount,
+        const int* __restrict__ xcdPrefix, const int* __restrict__ xcdBlocks, int numSegs, int ngb, int maxLen,
+        int2* __restrict__ work) {
+    constexpr int kXcds = 8;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<size_t>(maxLen) * kXcds) return;
+    const int x = static_cast<int>(i % kXcds);
+    const long long pos = static_cast<long long>(i / kXcds);
+    int2 item = make_int2(-1, 0);
+    if (x < numSegs && pos < static_cast<long long>(xcdBlocks[x]) * ngb) {
+        // segments x, x + 8, ...: the last m with xcdPrefix[x + 8 m] * ngb <= pos
+        int lo = 0, hi = (numSegs - x + kXcds - 1) / kXcds;   // m in [lo, hi)
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (static_cast<long long>(xcdPrefix[x + kXcds * mid]) * ngb <= pos) lo = mid;
+            else hi = mid;
+        }
+        const int seg = x + kXcds * lo;
+        const long long local = pos - static_cast<long long>(xcdPrefix[seg]) * ngb;
+        const int nb = segCount[seg];
+        item = make_int2(segFirst[seg] + static_cast<int>(local % nb), static_cast<int>(local / nb));
+    }
+    work[i] = item;
+}
+
 // starts[segment] = first position of segment (key >> yBits) in the sorted keys (absent segments left alone)
 __global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__ keys, int n, int yBits,
         int* __restrict__ starts) {
@@ -392,6 +419,14 @@ void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, 
     if (n == 0) return;
     hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals,
             flipN, absentKey, yBits);
+}
+
+void launchBuildWork(hipStream_t s, const int* segFirst, const int* segCount, const int* xcdPrefix,
+        const int* xcdBlocks, int numSegs, int ngb, int maxLen, int2* work) {
+    const size_t total = static_cast<size_t>(maxLen) * 8;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_build_work, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, segFirst,
+            segCount, xcdPrefix, xcdBlocks, numSegs, ngb, maxLen, work);
 }
 
 void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int yBits, int* starts) {

@@ -194,6 +194,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<uint8_t> dSortTemp, dLab8;
     DevBuf<BinBlock> dBlocks;
     DevBuf<int2> dWork;
+    DevBuf<int> dSegInfo;
     DevBuf<FeatRec> dGFeat;
     DevBuf<float> dGThr;
     DevBuf<int> dStarts;
@@ -386,25 +387,22 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             }
             dBlocks.resize(blocks.size());
             CF_HIP(hipMemcpyAsync(dBlocks.get(), blocks.data(), blocks.size() * sizeof(BinBlock), hipMemcpyHostToDevice, s));
-            // work list of a batch of ngb groups: segment i goes to XCD i % 8; each XCD takes its segments in
-            // order, all groups of a segment (group major, block minor) before the next; workgroup w runs
-            // on XCD w % 8, so the lists are interleaved
+            // work list of a batch of ngb groups (built on the device): segment i goes to XCD i % 8; each XCD
+            // takes its segments in order, all groups of a segment (group major, block minor) before the next;
+            // workgroup w runs on XCD w % 8, so the lists are interleaved
             constexpr int kXcds = 8;
-            auto buildWork = [&](int ngb, std::vector<int2>& work) {
-                std::vector<size_t> len(kXcds, 0);
-                for (size_t i = 0; i < segBlocks.size(); i++) len[i % kXcds] += static_cast<size_t>(segBlocks[i].second) * ngb;
-                const size_t maxLen = *std::max_element(len.begin(), len.end());
-                work.assign(maxLen * kXcds, make_int2(-1, 0));
-                std::vector<size_t> pos(kXcds, 0);
-                for (size_t i = 0; i < segBlocks.size(); i++) {
-                    const int x = static_cast<int>(i % kXcds);
-                    for (int g = 0; g < ngb; g++) {
-                        for (int b = 0; b < segBlocks[i].second; b++) {
-                            work[pos[x]++ * kXcds + x] = make_int2(segBlocks[i].first + b, g);
-                        }
-                    }
-                }
-            };
+            const int numSegs = static_cast<int>(segBlocks.size());
+            std::vector<int> segInfo(3 * static_cast<size_t>(numSegs) + kXcds, 0);   // first, count, XCD prefix; XCD totals
+            int* xcdBlocks = segInfo.data() + 3 * static_cast<size_t>(numSegs);
+            for (int i = 0; i < numSegs; i++) {
+                segInfo[i] = segBlocks[i].first;
+                segInfo[numSegs + i] = segBlocks[i].second;
+                segInfo[2 * static_cast<size_t>(numSegs) + i] = xcdBlocks[i % kXcds];
+                xcdBlocks[i % kXcds] += segBlocks[i].second;
+            }
+            const int maxXcdBlocks = *std::max_element(xcdBlocks, xcdBlocks + kXcds);
+            dSegInfo.resize(segInfo.size());
+            CF_HIP(hipMemcpyAsync(dSegInfo.get(), segInfo.data(), segInfo.size() * sizeof(int), hipMemcpyHostToDevice, s));
             dEvalOrder.resize(static_cast<size_t>(numActive) * F);
             launchFeatureEvalOrder(s, tabs, numActive, dEvalOrder.get());
             dGFeat.resize(static_cast<size_t>(numActive) * F);
@@ -432,19 +430,20 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             const int ng = static_cast<int>(std::max<size_t>(1, std::min<size_t>(groupsTotal, binsBudget / rowBytes)));
             const size_t flipOffset = anyBoth ? static_cast<size_t>(ng) * N : 0;
             dBins.resize(static_cast<size_t>(ng) * N * (anyBoth ? 2 : 1));
-            std::vector<int2> work;
-            int workGroups = -1;
+            int workGroups = -1, numWork = 0;
             for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
                 const int ngb = std::min(ng, groupsTotal - g0);
                 if (ngb != workGroups) {
-                    buildWork(ngb, work);
                     workGroups = ngb;
-                    dWork.resize(work.size());
-                    CF_HIP(hipMemcpyAsync(dWork.get(), work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+                    const int maxLen = maxXcdBlocks * ngb;
+                    numWork = maxLen * kXcds;
+                    dWork.resize(static_cast<size_t>(numWork));
+                    launchBuildWork(s, dSegInfo.get(), dSegInfo.get() + numSegs, dSegInfo.get() + 2 * numSegs,
+                            dSegInfo.get() + 3 * numSegs, numSegs, ngb, maxLen, dWork.get());
                 }
                 timer.begin();
-                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(), dWork.get(),
-                        static_cast<int>(work.size()), g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get(), flipOffset);
+                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(), dWork.get(), numWork,
+                        g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get(), flipOffset);
                 timer.end(stats.evalKernelMs);
                 stats.evalLaunches += 1;
                 const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
