@@ -21,13 +21,15 @@
 // order grouped by the integral planes they read (depth, then colour channel
 // pairs), so an XCD's L2 holds the planes in use.
 //
-// Pass 1 is bound by the vector memory path (L1 and L1 -> L2 requests), not
-// by arithmetic or HBM.  Fewer gathers in flight per CU run faster: two
+// Pass 1 is bound by the vector memory path (L1 -> L2 requests times their
+// latency, about 70 in flight per CU), not by arithmetic or HBM.  Two
 // workgroups of four waves per CU, each lane with its one sample's 8 corner
-// loads in flight, beat every configuration with more (more samples per
-// lane, more waves, larger blocks), which thrash the 32 KB vector L1 before
-// a box's second corner on the same line is read.  The launch reserves LDS
-// the kernel does not use to hold the CU at two workgroups.
+// loads in flight, beat every configuration with more samples in flight
+// (more samples per lane, more waves, larger blocks): with 256-sample blocks
+// the workgroups an XCD runs at once cover a few feature groups of a
+// segment rather than many, the planes they read fit the XCD's L2 (1.3%
+// misses instead of 3.8%) and requests return sooner.  The launch reserves
+// LDS the kernel does not use to hold the CU at two workgroups.
 #include <hipcub/hipcub.hpp>
 
 #include <map>
