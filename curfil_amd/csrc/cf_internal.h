@@ -228,13 +228,16 @@ void launchBinTables(hipStream_t s, const FeatTables& tabs, int trees, const uin
 // (samples still set to Both also count their flipped row at flipOffset)
 void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,
         int g0, int ng, const FeatTables& tabs, int C, int N, uint32_t* G, size_t flipOffset = 0);
+// Scores and per-feature summaries of the slots' histograms.  With allScores, also every score:
+// allScores[T][F] of slot onlySlot alone, or allScores[slot][T][F] of all slots (onlySlot < 0).
 void launchScores(hipStream_t s, const uint32_t* G, const uint32_t* nodeHist, const int* slotTree, int numSlots,
         const FeatTables& tabs, int C, FeatSummary* summ, double* allScores, int onlySlot, int fOffset = 0,
         int Fb = -1, const uint16_t* evalOrder = nullptr);
 void launchScoresFromCounters(hipStream_t s, const uint32_t* counters, const uint32_t* hist, int F, int T, int C,
         double* scores);
 void launchReduce(hipStream_t s, const FeatSummary* summ, int numSlots, int F, NodeBest* best);
-void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, NodeBest* best);
+// The reference's sequential argmax of scores[node][T][F] into best[node], for numNodes nodes.
+void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, NodeBest* best, int numNodes = 1);
 void launchCountersFromBins(hipStream_t s, const uint32_t* G, const FeatTables& tabs, int C, uint32_t* counters);
 void launchResponses(hipStream_t s, const DevImage* imgs, const cf_sample* samples, int n, const FeatRec* feats,
         int F, double* out);

@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(256) k_scores(const uint32_t* __restrict__ G, 
 #pragma unroll
         for (int c = 0; c < CM; c++) right[c] = A[c] - run[c];
         const double sc = nigScore<CM>(C, run, right, A);
-        if (allScores) allScores[static_cast<size_t>(t) * F + f] = sc;
+        if (allScores) allScores[(static_cast<size_t>(onlySlot >= 0 ? 0 : slot) * T + t) * F + f] = sc;
         if (sc > best) {
             second = best;
             best = sc;
@@ -394,11 +394,14 @@ __device__ __forceinline__ bool scoreBetter(double best, double score) {
 // reference's predicate.  A score that does not beat the best cannot beat any
 // later (larger) best: fl(score - best) does not grow as best grows.  The
 // result is the sequential scan's.
-__global__ void __launch_bounds__(64) k_seq_argmax(const double* __restrict__ scores, int T, int F,
-        NodeBest* __restrict__ out) {
+__global__ void __launch_bounds__(64) k_seq_argmax(const double* __restrict__ allScores, int T, int F,
+        NodeBest* __restrict__ allOut) {
     constexpr int kWin = 8;   // windows of 64 scores read per round
     const int lane = threadIdx.x;
     const long long n = static_cast<long long>(T) * F;
+    // one node per workgroup: scores[T][F] and its result
+    const double* scores = allScores + static_cast<size_t>(blockIdx.x) * n;
+    NodeBest* out = allOut + blockIdx.x;
     double best = -__builtin_inf();
     long long pos = 0;
     for (long long base = 0; base < n; base += 64 * kWin) {
@@ -529,8 +532,9 @@ void launchReduce(hipStream_t s, const FeatSummary* summ, int numSlots, int F, N
     hipLaunchKernelGGL(k_reduce, dim3(numSlots), dim3(64), 0, s, summ, numSlots, F, best);
 }
 
-void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, NodeBest* best) {
-    hipLaunchKernelGGL(k_seq_argmax, dim3(1), dim3(64), 0, s, scores, T, F, best);
+void launchSequentialArgmax(hipStream_t s, const double* scores, int T, int F, NodeBest* best, int numNodes) {
+    if (numNodes <= 0) return;
+    hipLaunchKernelGGL(k_seq_argmax, dim3(static_cast<unsigned>(numNodes)), dim3(64), 0, s, scores, T, F, best);
 }
 
 void launchCountersFromBins(hipStream_t s, const uint32_t* G, const FeatTables& tabs, int C, uint32_t* counters) {

@@ -195,6 +195,10 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<BinBlock> dBlocks;
     DevBuf<int2> dWork;
     DevBuf<int> dSegInfo;
+    DevBuf<uint32_t> dRepHist;
+    DevBuf<int> dRepTree;
+    DevBuf<FeatSummary> dRepSumm;
+    DevBuf<NodeBest> dRepBest;
     DevBuf<FeatRec> dGFeat;
     DevBuf<float> dGThr;
     DevBuf<int> dStarts;
@@ -309,24 +313,48 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 stats.replays++;
             }
         };
-        // the literal sequential argmax of one node from its direct-path histograms
-        auto replayNode = [&](int i) {
-            const LevelNode& ln = nodes[i];
-            std::vector<Chunk> bc(chunks.begin() + ln.chunkBegin, chunks.begin() + ln.chunkEnd);
-            for (auto& c : bc) c.slot = 0;
-            dChunks.resize(bc.size());
-            dG.resize(perSlot);
-            dAllScores.resize(static_cast<size_t>(T) * F);
-            CF_HIP(hipMemcpyAsync(dChunks.get(), bc.data(), bc.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
-            CF_HIP(hipMemsetAsync(dG.get(), 0, perSlot * sizeof(uint32_t), s));
-            launchEvalHistograms(s, images->set(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()), tabs, C,
-                    dG.get());
-            launchScores(s, dG.get(), dNodeHist.get() + static_cast<size_t>(i) * C, dSlotTree.get() + i, 1, tabs, C,
-                    dSumm.get() + static_cast<size_t>(i) * F, dAllScores.get(), 0);
-            launchSequentialArgmax(s, dAllScores.get(), T, F, dBest.get() + i);
-            CF_HIP(hipMemcpyAsync(&best[i], dBest.get() + i, sizeof(NodeBest), hipMemcpyDeviceToHost, s));
-            CF_HIP(hipStreamSynchronize(s));
-            stats.replays++;
+        // the literal sequential argmax of flagged nodes from their direct-path histograms, as many nodes
+        // per launch as the histogram budget holds
+        auto replayNodes = [&](const std::vector<int>& flagged) {
+            for (size_t a = 0; a < flagged.size();) {
+                const int ns = static_cast<int>(std::min<size_t>(flagged.size() - a, maxSlots));
+                std::vector<Chunk> bc;
+                std::vector<uint32_t> nh(static_cast<size_t>(ns) * C);
+                std::vector<int> st(ns);
+                for (int k = 0; k < ns; k++) {
+                    const LevelNode& ln = nodes[flagged[a + k]];
+                    for (int c = ln.chunkBegin; c < ln.chunkEnd; c++) {
+                        Chunk ch = chunks[c];
+                        ch.slot = k;
+                        bc.push_back(ch);
+                    }
+                    std::memcpy(&nh[static_cast<size_t>(k) * C],
+                            &out[active[ln.active].job].hist[static_cast<size_t>(ln.node) * C], C * sizeof(uint32_t));
+                    st[k] = ln.active;
+                }
+                dChunks.resize(bc.size());
+                dG.resize(static_cast<size_t>(ns) * perSlot);
+                dAllScores.resize(static_cast<size_t>(ns) * T * F);
+                dRepHist.resize(nh.size());
+                dRepTree.resize(ns);
+                dRepSumm.resize(static_cast<size_t>(ns) * F);
+                dRepBest.resize(ns);
+                CF_HIP(hipMemcpyAsync(dChunks.get(), bc.data(), bc.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
+                CF_HIP(hipMemcpyAsync(dRepHist.get(), nh.data(), nh.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+                CF_HIP(hipMemcpyAsync(dRepTree.get(), st.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+                CF_HIP(hipMemsetAsync(dG.get(), 0, static_cast<size_t>(ns) * perSlot * sizeof(uint32_t), s));
+                launchEvalHistograms(s, images->set(), level[cur].get(), dChunks.get(), static_cast<int>(bc.size()),
+                        tabs, C, dG.get());
+                launchScores(s, dG.get(), dRepHist.get(), dRepTree.get(), ns, tabs, C, dRepSumm.get(), dAllScores.get(),
+                        -1);
+                launchSequentialArgmax(s, dAllScores.get(), T, F, dRepBest.get(), ns);
+                std::vector<NodeBest> rb(ns);
+                CF_HIP(hipMemcpyAsync(rb.data(), dRepBest.get(), ns * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
+                CF_HIP(hipStreamSynchronize(s));
+                for (int k = 0; k < ns; k++) best[flagged[a + k]] = rb[k];
+                stats.replays += ns;
+                a += ns;
+            }
         };
         auto evalTwoPass = [&]() {
             int N = 0;
@@ -474,9 +502,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             timer.end(stats.scoreMs);
             CF_HIP(hipMemcpyAsync(best.data(), dBest.get(), numNodes * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
+            std::vector<int> flagged;
             for (int i = 0; i < numNodes; i++) {
-                if (best[i].flag) replayNode(i);
+                if (best[i].flag) flagged.push_back(i);
             }
+            replayNodes(flagged);
         };
         if (mode == CF_MODE_PLAIN) {
             for (int i = 0; i < numNodes; i++) evalSlots(i, i + 1);
