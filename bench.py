@@ -12,8 +12,9 @@ images and the sampled root pixels; `value` is feature-threshold
 evaluations per second of the batched level-wise trainer.  Also reported:
 the plain per-node path on the same first tree (depth 12), dense prediction
 of all 800 frames (batched, Mpix/s) with single-frame latency percentiles,
-the roofline of the split-evaluation kernel, and the CPU oracle timed on a
-bounded sample of the same workload.
+the roofline of the split-evaluation kernel, the CPU oracle timed on a
+bounded sample of the same workload, and the file paths around the hot path
+(PNG datasets, tree files, the curfil_predict tool) on a few of the frames.
 """
 import argparse
 import json
@@ -54,6 +55,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window per serving rate")
+    ap.add_argument("--io-frames", type=int, default=32, help="frames written as PNG files for the I/O leg")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy a device buffer of this many MiB 3 times (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -133,11 +135,12 @@ def main():
                                        max(1, sum(p["eval_launches"] for p in profiles)), 4),
                 "bytes_per_response": BYTES_PER_RESPONSE}
 
-    plain_vs_batched = predict = predict10 = serving = None
+    plain_vs_batched = predict = predict10 = serving = io = None
     if not args.no_extras:
         plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
         predict10, forest10 = dense_prediction_leg(ctx)
         serving = serving_leg(ctx, forest10, args.serve_seconds)
+        io = io_leg(forest, cfg, rgb, depth, labels, args.io_frames)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -169,6 +172,7 @@ def main():
         "predict": predict,
         "predict_forest10": predict10,
         "serving": serving,
+        "io": io,
     }
     print(json.dumps(out), flush=True)
 
@@ -186,6 +190,67 @@ def measured_traffic(N, W, H):
     if t.get("workload", {}).get("frames") != N or t["workload"].get("width") != W or t["workload"].get("height") != H:
         return None
     return t.get("bytes_per_launch")
+
+
+IO_PALETTE = [(0, 0, 0), (128, 0, 0), (0, 128, 0), (128, 128, 0), (0, 0, 128)]
+
+
+def io_leg(forest, cfg, rgb, depth, labels, frames):
+    """The file paths around the hot path on the first `frames` frames: the frames written as
+    *_colors.png / *_depth.png / *_ground_truth.png, loaded back as a dataset (loadImages), the
+    trained forest exported to tree<N>.json.gz and imported again, and curfil_predict run on the
+    PNG folder (forest import, PNG decode, integrals, classification, prediction PNGs, metrics)."""
+    import shutil
+    import tempfile
+    import numpy as np
+    import curfil_amd as cf
+
+    root = tempfile.mkdtemp(prefix="curfil_io_")
+    try:
+        test, pred, trees = (os.path.join(root, d) for d in ("test", "pred", "trees"))
+        os.makedirs(test)
+        os.makedirs(pred)
+        pal = np.array(IO_PALETTE, np.uint8)
+        t0 = time.perf_counter()
+        for i in range(frames):
+            cf.write_png(os.path.join(test, f"{i:04d}_colors.png"), rgb[i])
+            cf.write_png(os.path.join(test, f"{i:04d}_depth.png"), depth[i])
+            cf.write_png(os.path.join(test, f"{i:04d}_ground_truth.png"), pal[labels[i]])
+        write_s = time.perf_counter() - t0
+        png_bytes = sum(os.path.getsize(os.path.join(test, f)) for f in os.listdir(test))
+        t0 = time.perf_counter()
+        ds = cf.load_images(test)
+        load_s = time.perf_counter() - t0
+        assert ds.count == frames
+        ds.close()
+        t0 = time.perf_counter()
+        paths = forest.export(trees, cfg, palette=pal)
+        export_s = time.perf_counter() - t0
+        tree_bytes = sum(os.path.getsize(p) for p in paths)
+        t0 = time.perf_counter()
+        back = cf.RandomForestImage.from_files(paths)
+        import_s = time.perf_counter() - t0
+        nodes = sum(len(back.tree(t)[0]) for t in range(back.num_trees))
+        t0 = time.perf_counter()
+        rc = cf.run_tool("predict", [pred, test] + paths)
+        predict_s = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"curfil_predict exited with {rc}")
+        out = {"frames": frames, "png_bytes": png_bytes,
+               "png_write_frames_per_s": frames / write_s,
+               "load_images_frames_per_s": frames / load_s,
+               "load_images_mb_per_s": png_bytes / load_s / 1e6,
+               "tree_files": len(paths), "tree_file_bytes": tree_bytes, "tree_nodes": nodes,
+               "export_s": export_s, "import_s": import_s,
+               "curfil_predict_frames_per_s": frames / predict_s,
+               "note": "CPU threads of this host for PNG coding and JSON; curfil_predict end to end, prediction "
+                       "PNGs written"}
+        log(f"io: load {out['load_images_frames_per_s']:.0f} frames/s ({out['load_images_mb_per_s']:.0f} MB/s of "
+            f"PNG), export {export_s:.2f}s import {import_s:.2f}s ({nodes} nodes), curfil_predict "
+            f"{out['curfil_predict_frames_per_s']:.0f} frames/s")
+        return out
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
 
 
 def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):

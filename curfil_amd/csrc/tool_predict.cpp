@@ -8,13 +8,20 @@
 // and prediction images are written.  Probability images (writeProbabilityImages)
 // are 16-bit PNGs holding round(p * 65535) per label: the reference writes
 // p * INT_MAX through a 16-bit depth image, which keeps only the low 16 bits.
-// Prediction runs on the GPU; mode 'cpu' is refused.
+// Prediction runs on the GPU; mode 'cpu' is refused.  Images are loaded,
+// classified and scored in file order; the PNG files of each image are
+// encoded by up to numThreads writers (0: the host's hardware threads, at
+// most 16) while the next images are classified, and images are decoded
+// 2 x numThreads at a time.
 #include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <filesystem>
+#include <future>
 #include <sstream>
+#include <thread>
 #include <vector>
 
 #include "cli.h"
@@ -118,71 +125,121 @@ extern "C" int cf_tool_predict(int argc, char** argv) {
         std::vector<double> total(static_cast<size_t>(C) * C, 0.0);
         std::vector<double> accuracies, accuraciesWithoutVoid;
         double predictMs = 0;
-        for (size_t i = 0; i < files.size(); i++) {
-            const char* f = files[i].c_str();
+        const long long threadsOpt = o.integer("numThreads");
+        const size_t writers = threadsOpt > 0 ? static_cast<size_t>(threadsOpt)
+                                              : std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+        std::deque<std::future<void>> pending;   // PNG writes in flight, oldest first
+        // files are decoded a batch at a time on the host's threads (labels encoded in file order, as one
+        // at a time would); a batch of equal-sized images goes to the device as one image set
+        const size_t batch = 2 * writers;
+        for (size_t b0 = 0; b0 < files.size(); b0 += batch) {
+            const size_t nb = std::min(batch, files.size() - b0);
+            std::vector<const char*> names(nb);
+            for (size_t k = 0; k < nb; k++) names[k] = files[b0 + k].c_str();
             cf_dataset* ds = nullptr;
-            cli::check(nullptr, cf_dataset_load(&f, 1, useDepthImages, useDepthFilling, useCIELab, nullptr, map, 1, &ds));
-            int W = 0, H = 0;
-            cf_dataset_image_size(ds, 0, &W, &H);
-            const size_t px = static_cast<size_t>(W) * H;
-            std::vector<uint8_t> rgb(px * 3), gt(px), prediction(px);
-            std::vector<uint16_t> depth(px);
-            cf_dataset_get(ds, 0, rgb.data(), depth.data(), gt.data());
-            for (size_t p = 0; p < px; p++) {
-                if (gt[p] >= C) {
-                    std::ostringstream m;
-                    m << "illegal label in ground truth image '" << files[i] << "' at pixel (" << p % W << "," << p / W
-                      << "): " << static_cast<int>(gt[p]) << " (numClasses: " << C << ")";
-                    throw std::runtime_error(m.str());
-                }
+            cli::check(nullptr, cf_dataset_load(names.data(), static_cast<int>(nb), useDepthImages, useDepthFilling,
+                    useCIELab, nullptr, map, static_cast<int>(writers), &ds));
+            int W0 = 0, H0 = 0;
+            cf_dataset_image_size(ds, 0, &W0, &H0);
+            bool uniform = true;
+            for (size_t k = 1; k < nb; k++) {
+                int w = 0, h = 0;
+                cf_dataset_image_size(ds, static_cast<int>(k), &w, &h);
+                uniform = uniform && w == W0 && h == H0;
             }
-            cf_images* im = nullptr;
-            cli::check(ctx, cf_dataset_to_images(ds, ctx, &im));
-            std::vector<float> probs(px * C);
-            const auto t0 = std::chrono::steady_clock::now();
-            cli::check(ctx, cf_predict(ctx, forest, im, 0, probs.data(), prediction.data()));
-            predictMs += cli::seconds(t0) * 1e3;
-            cf_images_destroy(im);
-            cf_dataset_destroy(ds);
-
-            const std::string basepath = folderPrediction + "/" + std::filesystem::path(files[i]).stem().string();
-            if (writeImages) {
-                cli::check(nullptr, cf_png_write((basepath + ".png").c_str(), W, H, 3, 8, rgb.data()));
-                if (useDepthImages) cli::check(nullptr, cf_png_write((basepath + "_depth.png").c_str(), W, H, 1, 16, depth.data()));
-                std::vector<uint8_t> gtRgb(px * 3), prRgb(px * 3);
+            cf_images* batchImages = nullptr;
+            if (uniform) cli::check(ctx, cf_dataset_to_images(ds, ctx, &batchImages));
+            for (size_t k = 0; k < nb; k++) {
+                const size_t i = b0 + k;
+                int W = 0, H = 0;
+                cf_dataset_image_size(ds, static_cast<int>(k), &W, &H);
+                const size_t px = static_cast<size_t>(W) * H;
+                std::vector<uint8_t> rgb(px * 3), gt(px), prediction(px);
+                std::vector<uint16_t> depth(px);
+                cf_dataset_get(ds, static_cast<int>(k), rgb.data(), depth.data(), gt.data());
                 for (size_t p = 0; p < px; p++) {
-                    std::memcpy(&gtRgb[3 * p], palette + 3 * gt[p], 3);
-                    std::memcpy(&prRgb[3 * p], palette + 3 * prediction[p], 3);
-                }
-                cli::check(nullptr, cf_png_write((basepath + "_ground_truth.png").c_str(), W, H, 3, 8, gtRgb.data()));
-                cli::check(nullptr, cf_png_write((basepath + "_prediction.png").c_str(), W, H, 3, 8, prRgb.data()));
-                if (writeProbabilityImages) {
-                    std::vector<uint16_t> pi(px);
-                    for (int l = 0; l < C; l++) {
-                        bool skip = false;
-                        for (int g : ignored) skip = skip || g == l;
-                        if (skip) continue;
-                        for (size_t p = 0; p < px; p++) {
-                            pi[p] = static_cast<uint16_t>(std::lround(std::min(1.0f, std::max(0.0f, probs[l * px + p])) * 65535.0));
-                        }
-                        const std::string pf = basepath + "_label_" + std::to_string(l) + ".png";
-                        cli::check(nullptr, cf_png_write(pf.c_str(), W, H, 1, 16, pi.data()));
+                    if (gt[p] >= C) {
+                        std::ostringstream m;
+                        m << "illegal label in ground truth image '" << files[i] << "' at pixel (" << p % W << "," << p / W
+                          << "): " << static_cast<int>(gt[p]) << " (numClasses: " << C << ")";
+                        throw std::runtime_error(m.str());
                     }
                 }
+                // an image of a batch of mixed sizes goes to the device alone, at its own size
+                cf_images* im = batchImages;
+                cf_dataset* single = nullptr;
+                int index = static_cast<int>(k);
+                if (!uniform) {
+                    const char* f = files[i].c_str();
+                    cli::check(nullptr, cf_dataset_load(&f, 1, useDepthImages, useDepthFilling, useCIELab, nullptr, map, 1,
+                            &single));
+                    cli::check(ctx, cf_dataset_to_images(single, ctx, &im));
+                    index = 0;
+                }
+                std::vector<float> probs(px * C);
+                const auto t0 = std::chrono::steady_clock::now();
+                cli::check(ctx, cf_predict(ctx, forest, im, index, probs.data(), prediction.data()));
+                predictMs += cli::seconds(t0) * 1e3;
+                if (!uniform) {
+                    cf_images_destroy(im);
+                    cf_dataset_destroy(single);
+                }
+
+                const std::string basepath = folderPrediction + "/" + std::filesystem::path(files[i]).stem().string();
+                if (writeImages) {
+                    if (pending.size() >= writers) {
+                        pending.front().get();   // rethrows a failed write
+                        pending.pop_front();
+                    }
+                    pending.push_back(std::async(std::launch::async,
+                            [=, rgb = std::move(rgb), depth = std::move(depth), probs = std::move(probs)]() {
+                        cli::check(nullptr, cf_png_write((basepath + ".png").c_str(), W, H, 3, 8, rgb.data()));
+                        if (useDepthImages) {
+                            cli::check(nullptr, cf_png_write((basepath + "_depth.png").c_str(), W, H, 1, 16, depth.data()));
+                        }
+                        std::vector<uint8_t> gtRgb(px * 3), prRgb(px * 3);
+                        for (size_t p = 0; p < px; p++) {
+                            std::memcpy(&gtRgb[3 * p], palette + 3 * gt[p], 3);
+                            std::memcpy(&prRgb[3 * p], palette + 3 * prediction[p], 3);
+                        }
+                        cli::check(nullptr, cf_png_write((basepath + "_ground_truth.png").c_str(), W, H, 3, 8, gtRgb.data()));
+                        cli::check(nullptr, cf_png_write((basepath + "_prediction.png").c_str(), W, H, 3, 8, prRgb.data()));
+                        if (writeProbabilityImages) {
+                            std::vector<uint16_t> pi(px);
+                            for (int l = 0; l < C; l++) {
+                                bool skip = false;
+                                for (int g : ignored) skip = skip || g == l;
+                                if (skip) continue;
+                                for (size_t p = 0; p < px; p++) {
+                                    pi[p] = static_cast<uint16_t>(
+                                            std::lround(std::min(1.0f, std::max(0.0f, probs[l * px + p])) * 65535.0));
+                                }
+                                const std::string pf = basepath + "_label_" + std::to_string(l) + ".png";
+                                cli::check(nullptr, cf_png_write(pf.c_str(), W, H, 1, 16, pi.data()));
+                            }
+                        }
+                    }));
+                }
+                double acc = 0, accWithoutVoid = 0;
+                std::vector<double> confusion(static_cast<size_t>(C) * C, 0.0);
+                cli::check(nullptr, cf_pixel_accuracy(prediction.data(), gt.data(), px, 1, ignored.data(),
+                        static_cast<int>(ignored.size()), nullptr, C, &acc));
+                cli::check(nullptr, cf_pixel_accuracy(prediction.data(), gt.data(), px, 0, ignored.data(),
+                        static_cast<int>(ignored.size()), confusion.data(), C, &accWithoutVoid));
+                for (size_t q = 0; q < total.size(); q++) total[q] += confusion[q];
+                accuracies.push_back(acc);
+                accuraciesWithoutVoid.push_back(accWithoutVoid);
+                std::ostringstream m;
+                m << "prediction " << (i + 1) << "/" << files.size() << " (" << files[i] << "): pixel accuracy (without void): "
+                  << 100 * acc << " (" << 100 * accWithoutVoid << ")";
+                cli::info(m.str());
             }
-            double acc = 0, accWithoutVoid = 0;
-            std::vector<double> confusion(static_cast<size_t>(C) * C, 0.0);
-            cli::check(nullptr, cf_pixel_accuracy(prediction.data(), gt.data(), px, 1, ignored.data(),
-                    static_cast<int>(ignored.size()), nullptr, C, &acc));
-            cli::check(nullptr, cf_pixel_accuracy(prediction.data(), gt.data(), px, 0, ignored.data(),
-                    static_cast<int>(ignored.size()), confusion.data(), C, &accWithoutVoid));
-            for (size_t k = 0; k < total.size(); k++) total[k] += confusion[k];
-            accuracies.push_back(acc);
-            accuraciesWithoutVoid.push_back(accWithoutVoid);
-            std::ostringstream m;
-            m << "prediction " << (i + 1) << "/" << files.size() << " (" << files[i] << "): pixel accuracy (without void): "
-              << 100 * acc << " (" << 100 * accWithoutVoid << ")";
-            cli::info(m.str());
+            if (batchImages) cf_images_destroy(batchImages);
+            cf_dataset_destroy(ds);
+        }
+        while (!pending.empty()) {
+            pending.front().get();
+            pending.pop_front();
         }
         cli::info(std::to_string(predictMs / static_cast<double>(files.size())) + " ms/image");
 

@@ -536,6 +536,11 @@ def test_command_line_tools_end_to_end(ctx, config0, tmp_path, capfd):
         cf.write_png(test / f"t{i}_colors.png", trgb[i])
         cf.write_png(test / f"t{i}_depth.png", tdepth[i])
         cf.write_png(test / f"t{i}_ground_truth.png", pal[tlabels[i]])
+    # a smaller third test image: curfil_predict classifies a folder of mixed sizes image by image
+    srgb, sdepth, slabels = cf.synth_dataset(1, 96, 80, sites=6, classes=5, zero_frac=0.05, seed0=123)
+    cf.write_png(test / "t2_colors.png", srgb[0])
+    cf.write_png(test / "t2_depth.png", sdepth[0])
+    cf.write_png(test / "t2_ground_truth.png", pal[slabels[0]])
     args = [train, 2, 1000, 100, 32, 7, 60, 10, 10, out, "--useCIELab=false", "--ignoreColor", "0,0,0"]
     assert cf.run_tool("train", args) == 0
     paths = [out / "tree0.json.gz", out / "tree1.json.gz"]
@@ -556,15 +561,15 @@ def test_command_line_tools_end_to_end(ctx, config0, tmp_path, capfd):
     lm = cf.LabelMap()
     for c in back.palette:
         lm.encode(c)
-    tset = cf.Dataset.load(cf.list_image_filenames(str(test)), label_map=lm)
-    tim = tset.to_images(ctx)
     accs = []
-    for i in range(2):
-        lab, _ = back.predict(tim, i)
+    for i in range(3):
+        tset = cf.Dataset.load([test / f"t{i}_colors.png"], label_map=lm)
+        lab, _ = back.predict(tset.to_images(ctx), 0)
         written = cf.read_png(pred / f"t{i}_colors_prediction.png")
         assert np.array_equal(written, back.palette[lab])
-        gt = tset.get(i)[2]
+        gt = tset.get(0)[2]
         accs.append(cf.calculatePixelAccuracy(lab, gt, True, ign))
+    assert cf.read_png(pred / "t2_colors_prediction.png").shape == (80, 96, 3)
     m = re.search(r"pixel accuracy: ([0-9.]+)", log)
     assert m and abs(float(m.group(1)) - 100 * np.mean(accs)) < 1e-4, log[-2000:]
     assert "confusion matrix" in log
