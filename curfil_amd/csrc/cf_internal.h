@@ -194,12 +194,12 @@ struct BinBlock {
     int image;   // the image of every sample in the block
 };
 void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, uint16_t* evalOrder);
-// keys ((tree * images + image) << yBits | y) and values (level index) of one tree's level segment;
+// keys ((image * keyMul + keyBase) << yBits | y) and values (level index) of one tree's level segment;
 // with flipN > 0 also the flipped evaluation of every Both sample at flipN + index (value |
 // kFlipEntry), and segment absentKey for the other samples there
 constexpr uint32_t kFlipEntry = 0x80000000u;
-void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
-        uint32_t* vals, int flipN, uint32_t absentKey, int yBits);
+void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t keyMul,
+        uint32_t* keys, uint32_t* vals, int flipN, uint32_t absentKey, int yBits);
 // label per level index, bit 7 set for samples still set to Both (counted in both orientations)
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels);
 size_t radixSortTempBytes(int n, int bits);

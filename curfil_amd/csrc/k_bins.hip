@@ -1,8 +1,8 @@
 // Two-pass split evaluation of a whole level (batched training):
 //
 //   k_eval_bins   pass 1, image-major: the level's samples ordered by
-//                 (tree, image, row) are cut into blocks of <= 256 within
-//                 each (tree, image) segment; one workgroup per
+//                 (image, tree, row) are cut into blocks of <= 256 within
+//                 each (image, tree) segment; one workgroup per
 //                 (block, group of 16 features), one sample per lane,
 //                 computes every response's interval among the feature's
 //                 thresholds (the bin used by the interval histograms) and
@@ -16,10 +16,12 @@
 // path (k_eval_hist), whose single kernel gathers the integrals in node order:
 // at deep levels a node's samples come from every image and the gathers miss
 // in L2.  In pass 1 a workgroup's samples come from one image, the
-// workgroups of one XCD sweep one (tree, image) segment for all feature
-// groups of a batch before the next, and features are visited in an
-// order grouped by the integral planes they read (depth, then colour channel
-// pairs), so an XCD's L2 holds the planes in use.
+// workgroups of one XCD sweep one image (the segments of all trees, tree by
+// tree within each feature group) for all feature groups of a batch before
+// the next, and every tree visits its features in an order grouped by the
+// integral planes they read (depth, then colour channel pairs), so the
+// trees' workgroups of one feature group read the same planes of one image
+// and an XCD's L2 holds them.
 //
 // Pass 1 is bound by the vector memory path (L1 -> L2 requests times their
 // latency, about 70 in flight per CU), not by arithmetic or HBM.  Two
@@ -316,18 +318,19 @@ __global__ void __launch_bounds__(64) k_feature_eval_order(const FeatRec* __rest
 // Radix-sort keys (tree * images + image) and values (level index) of one
 // tree's level segment.
 __global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict__ samples, int begin, int n,
-        uint32_t keyBase, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int flipN, uint32_t absentKey,
-        int yBits) {
+        uint32_t keyBase, uint32_t keyMul, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int flipN,
+        uint32_t absentKey, int yBits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const cf_sample x = samples[begin + i];
     // (tree, image) segment, then the sample's row: the samples of a row sit in one wave, where
     // corner reads of a feature that fall on one cache line are merged
     const uint32_t row = yBits > 0 ? x.y : 0u;
-    keys[begin + i] = ((keyBase + x.image) << yBits) | row;
+    const uint32_t seg = static_cast<uint32_t>(x.image) * keyMul + keyBase;
+    keys[begin + i] = (seg << yBits) | row;
     vals[begin + i] = static_cast<uint32_t>(begin + i);
     if (flipN > 0) {
-        keys[flipN + begin + i] = x.flip == 2 ? (((keyBase + x.image) << yBits) | row) : (absentKey << yBits);
+        keys[flipN + begin + i] = x.flip == 2 ? ((seg << yBits) | row) : (absentKey << yBits);
         vals[flipN + begin + i] = static_cast<uint32_t>(begin + i) | kFlipEntry;
     }
 }
@@ -416,11 +419,11 @@ void launchFeatureEvalOrder(hipStream_t s, const FeatTables& tabs, int trees, ui
     hipLaunchKernelGGL(k_feature_eval_order, dim3(trees), dim3(64), 0, s, tabs.feat.get(), tabs.F, evalOrder);
 }
 
-void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t* keys,
-        uint32_t* vals, int flipN, uint32_t absentKey, int yBits) {
+void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, uint32_t keyBase, uint32_t keyMul,
+        uint32_t* keys, uint32_t* vals, int flipN, uint32_t absentKey, int yBits) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keys, vals,
-            flipN, absentKey, yBits);
+    hipLaunchKernelGGL(k_level_keys, dim3((n + 255) / 256), dim3(256), 0, s, samples, begin, n, keyBase, keyMul, keys,
+            vals, flipN, absentKey, yBits);
 }
 
 void launchBuildWork(hipStream_t s, const int* segFirst, const int* segCount, const int* xcdPrefix,

@@ -377,16 +377,18 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             const char* rowOrder = std::getenv("CURFIL_ROW_ORDER");
             if (bits + yBits > 32 || (rowOrder && std::strcmp(rowOrder, "0") == 0)) yBits = 0;
             bits += yBits;
+            // segments (image, tree) in image-major order: the trees' segments of one image are adjacent
             for (int a = 0; a < numActive; a++) {
-                launchLevelKeys(s, level[cur].get(), active[a].begin, active[a].n,
-                        static_cast<uint32_t>(a) * static_cast<uint32_t>(images->count), dKeys.get(), dVals.get(),
-                        anyBoth ? N : 0, static_cast<uint32_t>(keyRange), yBits);
+                launchLevelKeys(s, level[cur].get(), active[a].begin, active[a].n, static_cast<uint32_t>(a),
+                        static_cast<uint32_t>(numActive), dKeys.get(), dVals.get(), anyBoth ? N : 0,
+                        static_cast<uint32_t>(keyRange), yBits);
             }
             const size_t sortBytes = radixSortTempBytes(M, bits);
             dSortTemp.resize(std::max<size_t>(sortBytes, 1));
             radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), M, bits);
             launchLevelLabels(s, level[cur].get(), N, dLab8.get());
-            // (tree, image) segments of the sorted order -> blocks of <= kBinBlockSamples samples within a segment
+            // (image, tree) segments of the sorted order -> blocks of <= kBinBlockSamples samples within a segment;
+            // the segments of one image form a group of consecutive blocks
             const int numKeys = static_cast<int>(keyRange);
             dStarts.resize(numKeys + 1);
             CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, (numKeys + 1) * sizeof(int), s));
@@ -396,7 +398,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             CF_HIP(hipStreamSynchronize(s));
             if (starts[numKeys] < 0) starts[numKeys] = M;   // the end of the last segment
             std::vector<BinBlock> blocks;
-            std::vector<std::pair<int, int>> segBlocks;   // per segment: first block, block count
+            std::vector<std::pair<int, int>> segBlocks;   // per image group: first block, block count
             {
                 int k = 0;
                 while (k < numKeys && starts[k] < 0) k++;
@@ -406,18 +408,24 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     const int b = starts[k], e = starts[k2];
                     const int first = static_cast<int>(blocks.size());
                     for (int p = b; p < e; p += kBinBlockSamples) {
-                        blocks.push_back(BinBlock{k / images->count, p, std::min(kBinBlockSamples, e - p),
-                                k % images->count});
+                        blocks.push_back(BinBlock{k % numActive, p, std::min(kBinBlockSamples, e - p), k / numActive});
                     }
-                    segBlocks.emplace_back(first, static_cast<int>(blocks.size()) - first);
+                    const int image = k / numActive;
+                    if (!segBlocks.empty() && blocks[segBlocks.back().first].image == image) {
+                        segBlocks.back().second += static_cast<int>(blocks.size()) - first;
+                    } else {
+                        segBlocks.emplace_back(first, static_cast<int>(blocks.size()) - first);
+                    }
                     k = k2;
                 }
             }
             dBlocks.resize(blocks.size());
             CF_HIP(hipMemcpyAsync(dBlocks.get(), blocks.data(), blocks.size() * sizeof(BinBlock), hipMemcpyHostToDevice, s));
-            // work list of a batch of ngb groups (built on the device): segment i goes to XCD i % 8; each XCD
-            // takes its segments in order, all groups of a segment (group major, block minor) before the next;
-            // workgroup w runs on XCD w % 8, so the lists are interleaved
+            // work list of a batch of ngb feature groups (built on the device): image group i goes to XCD i % 8;
+            // each XCD takes its image groups in order, all feature groups of an image group (feature group
+            // major, then tree, then block) before the next, so the trees' workgroups of one feature group read
+            // the same planes of one image from the XCD's L2; workgroup w runs on XCD w % 8, so the lists are
+            // interleaved
             constexpr int kXcds = 8;
             const int numSegs = static_cast<int>(segBlocks.size());
             std::vector<int> segInfo(3 * static_cast<size_t>(numSegs) + kXcds, 0);   // first, count, XCD prefix; XCD totals
