@@ -97,9 +97,9 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
         int g0, const FeatRec* __restrict__ gFeat, const float* __restrict__ gThr, int F, int T, int N,
         uint4* __restrict__ bins, size_t flipOffset) {
     // work[blockIdx] = (sample block, feature group within the batch); the
-    // host interleaves the XCDs' lists (workgroup w runs on XCD w % 8) so each
-    // XCD sweeps one (tree, image) segment for every group of the batch
-    // before the next segment.  Padding entries have a negative block.
+    // XCDs' lists are interleaved (workgroup w runs on XCD w % 8) so each XCD
+    // sweeps one image (the segments of all trees) for every group of the
+    // batch before the next image.  Padding entries have a negative block.
     const int2 wk = work[blockIdx.x];
     if (wk.x < 0) return;
     const int g = wk.y;
