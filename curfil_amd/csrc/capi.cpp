@@ -629,7 +629,7 @@ int cf_forest_grow(cf_ctx* ctx, cf_images* im, cf_forest* fo, const cf_train_con
         require(!fo->roots.empty(), "forest has no sampled roots");
         std::vector<TreeJob> jobs;
         for (size_t t = 0; t < fo->roots.size(); t++) {
-            jobs.push_back(TreeJob{fo->treeIds[t], HostRandomSource(fo->rsSeeds[t]), fo->roots[t]});
+            jobs.push_back(TreeJob{fo->treeIds[t], HostRandomSource(fo->rsSeeds[t]), &fo->roots[t]});
         }
         fo->stats = TrainStats();
         fo->trees.clear();
@@ -680,11 +680,10 @@ int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* im, const cf_train_config
         fo->C = C;
         fo->prior.assign(C, 1u);
         std::vector<TreeJob> jobs;
-        TreeJob j{treeId, HostRandomSource(rsSeed), std::vector<cf_sample>(roots, roots + n)};
-        fo->roots.push_back(j.roots);
+        fo->roots.emplace_back(roots, roots + n);
         fo->treeIds.push_back(treeId);
         fo->rsSeeds.push_back(rsSeed);
-        jobs.push_back(std::move(j));
+        jobs.push_back(TreeJob{treeId, HostRandomSource(rsSeed), &fo->roots.back()});
         trainTrees(ctx, im, *cfg, C, jobs, mode, fo->trees, fo->stats);
         *out = fo.release();
     });

@@ -32,7 +32,7 @@ struct TrainStats {
 struct TreeJob {
     int treeId;
     HostRandomSource rs;
-    std::vector<cf_sample> roots;
+    const std::vector<cf_sample>* roots;   // the tree's root samples, held by the forest (not copied)
 };
 
 // Level-wise training of several trees on one device (RandomTreeTrain::train,

@@ -121,8 +121,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     size_t total = 0;
     bool anyBoth = false;
     for (auto& j : jobs) {
-        total += j.roots.size();
-        for (const cf_sample& x : j.roots) anyBoth = anyBoth || x.flip == 2;
+        total += j.roots->size();
+        for (const cf_sample& x : *j.roots) anyBoth = anyBoth || x.flip == 2;
     }
     // a Both sample splits at most once into a NoFlip sample and a Flip copy, which
     // never split again: a level holds at most twice the root samples
@@ -143,26 +143,26 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             root.level = 1;
             root.parent = -1;
             root.left = root.right = -1;
-            root.num_samples = static_cast<int32_t>(jobs[j].roots.size());
+            root.num_samples = static_cast<int32_t>(jobs[j].roots->size());
             root.is_leaf = 1;
             tb.nodes.push_back(root);
             tb.hist.assign(C, 0u);
-            for (const cf_sample& x : jobs[j].roots) {
+            for (const cf_sample& x : *jobs[j].roots) {
                 if (x.label >= C) throw Error(CF_ERR_ARG, "sample label >= number of classes");
                 if (x.image >= static_cast<uint32_t>(images->count)) throw Error(CF_ERR_ARG, "sample image out of range");
                 if (x.x >= images->W || x.y >= images->H) throw Error(CF_ERR_ARG, "sample position outside the image");
                 tb.hist[x.label] += 1;
                 if (x.flip == 2) tb.hist[x.label] += 1;
             }
-            if (jobs[j].roots.empty()) continue;
+            if (jobs[j].roots->empty()) continue;
             ActiveTree a;
             a.job = static_cast<int>(j);
             a.idNode = jobs[j].treeId;
             a.nodes.push_back(0);
-            a.counts.push_back(static_cast<int>(jobs[j].roots.size()));
+            a.counts.push_back(static_cast<int>(jobs[j].roots->size()));
             a.begin = static_cast<int>(all.size());
-            a.n = static_cast<int>(jobs[j].roots.size());
-            all.insert(all.end(), jobs[j].roots.begin(), jobs[j].roots.end());
+            a.n = static_cast<int>(jobs[j].roots->size());
+            all.insert(all.end(), jobs[j].roots->begin(), jobs[j].roots->end());
             active.push_back(std::move(a));
         }
         if (!all.empty()) {
