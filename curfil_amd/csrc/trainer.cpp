@@ -134,8 +134,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
 
     std::vector<ActiveTree> active;
     {
-        std::vector<cf_sample> all;
-        all.reserve(total);
+        size_t offset = 0;   // each tree's roots are copied straight from the forest into level 0
         for (size_t j = 0; j < jobs.size(); j++) {
             TreeBuild& tb = out[j];
             cf_tree_node root{};
@@ -160,13 +159,12 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             a.idNode = jobs[j].treeId;
             a.nodes.push_back(0);
             a.counts.push_back(static_cast<int>(jobs[j].roots->size()));
-            a.begin = static_cast<int>(all.size());
+            a.begin = static_cast<int>(offset);
             a.n = static_cast<int>(jobs[j].roots->size());
-            all.insert(all.end(), jobs[j].roots->begin(), jobs[j].roots->end());
+            CF_HIP(hipMemcpyAsync(level[0].get() + offset, jobs[j].roots->data(), jobs[j].roots->size() * sizeof(cf_sample),
+                    hipMemcpyHostToDevice, s));
+            offset += jobs[j].roots->size();
             active.push_back(std::move(a));
-        }
-        if (!all.empty()) {
-            CF_HIP(hipMemcpyAsync(level[0].get(), all.data(), all.size() * sizeof(cf_sample), hipMemcpyHostToDevice, s));
         }
     }
     int cur = 0;
