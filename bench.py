@@ -1,20 +1,23 @@
 """Benchmark of the CURFIL hot path on one MI355X.
 
-Workload (the plain full-frame configuration): 800 synthetic RGB-D frames of
-640x480 (30 Voronoi sites, 4 classes + void, 8% missing depth), 3 trees to
-depth 18, 4000 class-uniform samples per image and tree (void ignored),
-2000 candidate features (offsets <= 120 px, regions <= 20 px) x 20
-thresholds, minSampleCount 50.
+Headline workload (batched level-wise training): 800 synthetic RGB-D frames
+of 640x480 (30 Voronoi sites, 4 classes + void, 8% missing depth), 5 trees
+trained together to depth 20, 4000 class-uniform samples per image and tree
+(void ignored), 4000 candidate features (offsets <= 120 px, regions <= 20
+px) x 32 thresholds, minSampleCount 50.
 
-A step is one full training of the 3-tree forest (feature generation,
-split evaluation and partition of every level) from the resident integral
-images and the sampled root pixels; `value` is feature-threshold
-evaluations per second of the batched level-wise trainer.  Also reported:
-the plain per-node path on the same first tree (depth 12), dense prediction
-of all 800 frames (batched, Mpix/s) with single-frame latency percentiles,
-the roofline of the split-evaluation kernel, the CPU oracle timed on a
-bounded sample of the same workload, and the file paths around the hot path
-(PNG datasets, tree files, the curfil_predict tool) on a few of the frames.
+A step is one full training of the 5-tree forest (feature generation, split
+evaluation and partition of every level) from the resident integral images
+and the sampled root pixels; `value` is feature-threshold evaluations per
+second of the batched level-wise trainer.  Also reported: the plain
+full-frame workload (3 trees to depth 18, F = 2000 x T = 20) on the same
+frames, the plain per-node path against the batched one on the headline
+configuration (tree 0, depth 12), dense prediction (the trained forest, and
+10 complete trees of depth 20 on 64-frame batches) with its roofline, the
+serving loop under Poisson arrivals, the roofline of the split-evaluation
+kernel, the CPU oracle timed on bounded samples of the same workloads, and
+the file paths around the hot path (PNG datasets, tree files, the
+curfil_predict tool) on a few of the frames.
 """
 import argparse
 import json
@@ -51,7 +54,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=800)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=200000)
+    ap.add_argument("--cpu-sample", type=int, default=120000)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window per serving rate")
@@ -78,9 +81,12 @@ def main():
     import curfil_amd as cf
 
     W, H, N = 640, 480, args.frames
-    cfg_kw = dict(randomSeed=4711, samplesPerImage=12000, featureCount=2000, minSampleCount=50, maxDepth=18,
-                  boxRadius=120, regionSize=20, thresholds=20, ignored_labels=(0,))
-    trees = 3
+    base_kw = dict(randomSeed=4711, minSampleCount=50, boxRadius=120, regionSize=20, ignored_labels=(0,))
+    # the batched workload: 5 trees to depth 20, F = 4000 x T = 32 (samplesPerImage counts all trees)
+    trees = 5
+    cfg_kw = dict(base_kw, samplesPerImage=4000 * trees, featureCount=4000, thresholds=32, maxDepth=20)
+    # the plain full-frame workload: 3 trees to depth 18, F = 2000 x T = 20
+    full_kw = dict(base_kw, samplesPerImage=4000 * 3, featureCount=2000, thresholds=20, maxDepth=18)
     ctx = cf.Context(0)
 
     t0 = time.time()
@@ -123,20 +129,11 @@ def main():
     value = evals * len(step_times) / total_time
     nodes = [len(forest.tree(t)[0]) for t in range(trees)]
 
-    # roofline of the response kernel (pass 1 of the level evaluation: the integral gathers), from its
-    # HIP-event time on the library's stream
-    eval_s = sum(p["eval_kernel_ms"] for p in profiles) / 1e3
-    responses = sum(p["responses"] for p in profiles)
-    achieved = responses * BYTES_PER_RESPONSE / eval_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(N, W, H),
-                "kernel": "k_eval_bins", "launches": int(sum(p["eval_launches"] for p in profiles)),
-                "avg_launch_ms": round(sum(p["eval_kernel_ms"] for p in profiles) /
-                                       max(1, sum(p["eval_launches"] for p in profiles)), 4),
-                "bytes_per_response": BYTES_PER_RESPONSE}
+    roofline = eval_roofline(profiles, N, W, H, cfg_kw["thresholds"])
 
-    plain_vs_batched = predict = predict10 = serving = io = None
+    full_frame = plain_vs_batched = predict = predict10 = serving = io = None
     if not args.no_extras:
+        full_frame = full_frame_leg(ctx, images, full_kw, N, W, H)
         plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
         predict10, forest10 = dense_prediction_leg(ctx)
         serving = serving_leg(ctx, forest10, args.serve_seconds)
@@ -144,7 +141,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(forest, rgb, depth, labels, cfg_kw, args.cpu_sample)
+        cpu = cpu_baseline(forest, rgb, depth, labels, cfg_kw, args.cpu_sample,
+                           forest10 if not args.no_extras else None)
 
     out = {
         "metric": "feature-threshold evals/s (train)",
@@ -159,15 +157,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"plain full-frame training: {N} frames {W}x{H}, {trees} trees depth 18, "
-                               f"4000 samples/image/tree, F=2000 x T=20, boxRadius 120, regionSize 20, "
-                               f"minSampleCount 50, 4 classes + void",
+        "config": {"workload": f"batched level-wise training: {N} frames {W}x{H}, {trees} trees depth 20 "
+                               f"trained together, 4000 samples/image/tree, F=4000 x T=32, boxRadius 120, "
+                               f"regionSize 20, minSampleCount 50, 4 classes + void, u32 histograms, f64 "
+                               f"responses against f32 thresholds",
                    "mode": "batched level-wise", "evaluations_per_step": evals, "nodes_per_tree": nodes,
-                   "root_samples_per_tree": roots[0]},
+                   "root_samples_per_tree": roots[0], "levels": int(prof["levels"])},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phase_ms": {k: round(prof[k], 1) for k in ("featgen_ms", "eval_kernel_ms", "hist_ms", "score_ms",
                                                     "partition_ms")},
+        "full_frame_training": full_frame,
         "plain_vs_batched": plain_vs_batched,
         "predict": predict,
         "predict_forest10": predict10,
@@ -177,17 +177,57 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(N, W, H):
-    """HBM bytes per k_eval_bins launch from the committed rocprofv3 counter
-    passes (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
-    counter's reading for a copy kernel of known size), for this workload only."""
-    path = os.path.join(ROOT, "profiles", "k_eval_bins_traffic.json")
+def eval_roofline(profiles, N, W, H, T):
+    """Roofline of the response kernel (pass 1 of the level evaluation: the integral gathers), from its
+    HIP-event time on the library's stream: 48 B of corner reads per (sample, feature) response."""
+    eval_s = sum(p["eval_kernel_ms"] for p in profiles) / 1e3
+    responses = sum(p["responses"] for p in profiles)
+    launches = sum(p["eval_launches"] for p in profiles)
+    achieved = responses * BYTES_PER_RESPONSE / eval_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_eval_bins", N, W, H, T),
+            "kernel": "k_eval_bins", "launches": int(launches),
+            "avg_launch_ms": round(sum(p["eval_kernel_ms"] for p in profiles) / max(1, launches), 4),
+            "responses_per_launch": responses / max(1, launches),
+            "bytes_per_response": BYTES_PER_RESPONSE}
+
+
+def full_frame_leg(ctx, images, full_kw, N, W, H):
+    """The plain full-frame workload on the same frames (3 trees to depth 18, F = 2000 x T = 20), trained
+    batched: one warm-up and one timed step."""
+    import curfil_amd as cf
+    cfg = cf.TrainingConfiguration(**full_kw)
+    f = cf.RandomForestImage.sample(ctx, images, cfg, 3, num_labels=5)
+    f.grow(cfg, cf.MODE_BATCHED)
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    f.grow(cfg, cf.MODE_BATCHED)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    p = f.profile()
+    out = {"workload": f"{N} frames {W}x{H}, 3 trees depth 18, 4000 samples/image/tree, F=2000 x T=20, "
+                       f"minSampleCount 50", "seconds": dt, "evals_per_s": p["evaluations"] / dt,
+           "evaluations": p["evaluations"], "nodes_per_tree": [len(f.tree(t)[0]) for t in range(3)],
+           "roofline": eval_roofline([p], N, W, H, 20)}
+    log(f"full-frame workload (3 trees depth 18, F=2000 T=20): {dt:.2f}s, {out['evals_per_s']:.3e} evals/s, "
+        f"k_eval_bins {out['roofline']['avg_launch_ms']:.2f} ms per launch, frac {out['roofline']['frac']}")
+    return out
+
+
+def measured_traffic(kernel, N, W, H, T):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter passes (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by the same counter's reading for a copy kernel of known size),
+    for this workload only."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("workload", {}).get("frames") != N or t["workload"].get("width") != W or t["workload"].get("height") != H:
+    w = t.get("workload", {})
+    if (w.get("frames"), w.get("width"), w.get("height"), w.get("thresholds")) != (N, W, H, T):
         return None
     return t.get("bytes_per_launch")
 
@@ -455,14 +495,46 @@ def serving_leg(ctx, forest, seconds):
     return results
 
 
-def cpu_baseline(forest, rgb, depth, labels, cfg_kw, sample):
-    """The oracle (scalar C++ port of the reference CPU path) evaluating the
-    root node of tree 0 on its first `sample` root samples: all 2000 x 20
-    feature-threshold pairs, one thread."""
+def cpu_threads():
+    """Host threads for the multi-thread CPU legs: the host's processors, at most 16 (the host-CPU share a
+    single GPU gets on the benchmark machine)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class _OracleThreads:
+    """The oracle's thread count (ORC_THREADS, read per call) for the duration of a block."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        self.old = os.environ.get("ORC_THREADS")
+        os.environ["ORC_THREADS"] = str(self.n)
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            os.environ.pop("ORC_THREADS", None)
+        else:
+            os.environ["ORC_THREADS"] = self.old
+
+
+def cpu_baseline(forest, rgb, depth, labels, cfg_kw, sample, forest10=None):
+    """The oracle (C++ restatement of the reference CPU path) on bounded samples of the workloads:
+
+    * the headline: the root node of tree 0 on its first root samples, all F x T feature-threshold pairs,
+      on 1 thread (`sample` / 4 samples) and split over the host threads (`sample` samples), per-thread
+      counters summed afterwards as the reference's TBB loop does (random_tree_image.cpp:339-371);
+    * the CPU-runnable workload end to end (4 frames 160x120: sampling, 1 tree to depth 8, prediction of
+      all 4 frames);
+    * dense prediction with the 10-tree depth-20 forest on 4 of the 64 frames, rows split over the host
+      threads (random_forest_image.cpp:310)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
+    import curfil_amd as cf
     import oracle as orc
 
+    threads = cpu_threads()
     roots = forest.root_samples(0)[:sample].copy()
     used = np.unique(roots["image"])
     remap = np.zeros(int(used.max()) + 1, np.uint32)
@@ -471,26 +543,60 @@ def cpu_baseline(forest, rgb, depth, labels, cfg_kw, sample):
     ds = orc.Dataset.from_u8(rgb[used], depth[used], labels[used])
     F, T = cfg_kw["featureCount"], cfg_kw["thresholds"]
     feats, thr = ds.generate_features(roots, 12345, F, T, cfg_kw["boxRadius"], cfg_kw["regionSize"])
-    t0 = time.perf_counter()
-    ds.node_counters(roots, feats, thr, 5)
-    dt = time.perf_counter() - t0
-    ev = len(roots) * F * T
-    log(f"cpu baseline: {ev:.3e} evals in {dt:.2f}s")
-    # the same sample split over host threads, as the reference's TBB loop over samples does
-    # (random_tree_image.cpp:339-371): per-thread counters, summed afterwards
-    import concurrent.futures
-    threads = max(1, min(16, os.cpu_count() or 1))
-    parts = np.array_split(roots, threads)
-    t0 = time.perf_counter()
-    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
-        list(ex.map(lambda r: ds.node_counters(r, feats, thr, 5), parts))
-    dtm = time.perf_counter() - t0
-    log(f"cpu baseline, {threads} threads: {ev:.3e} evals in {dtm:.2f}s")
-    return {"value": ev / dt, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"root node of tree 0: first {len(roots)} root samples x {F} features x {T} thresholds "
-                      f"({ev:.2e} evaluations), 1 thread, {cpu_model()}",
-            "multi_thread": {"value": ev / dtm, "cores": threads,
-                             "note": "same sample split over host threads (ctypes calls run in parallel)"}}
+    one = roots[:max(1, len(roots) // 4)]
+    with _OracleThreads(1):
+        t0 = time.perf_counter()
+        ds.node_counters(one, feats, thr, 5)
+        dt1 = time.perf_counter() - t0
+    ev1 = len(one) * F * T
+    log(f"cpu baseline, 1 thread: {ev1:.3e} evals in {dt1:.2f}s")
+    with _OracleThreads(threads):
+        t0 = time.perf_counter()
+        ds.node_counters(roots, feats, thr, 5)
+        dtm = time.perf_counter() - t0
+    evm = len(roots) * F * T
+    log(f"cpu baseline, {threads} threads: {evm:.3e} evals in {dtm:.2f}s")
+    out = {"value": evm / dtm, "unit": "evals/s", "cores": threads, "kind": "port",
+           "sample": f"root node of tree 0 of the batched workload: first {len(roots)} root samples x {F} "
+                     f"features x {T} thresholds ({evm:.2e} evaluations), {threads} threads; {cpu_model()}, "
+                     f"{os.cpu_count()} processors visible",
+           "single_thread": {"value": ev1 / dt1, "cores": 1,
+                             "sample": f"first {len(one)} root samples x {F} x {T} ({ev1:.2e} evaluations)"}}
+
+    # the CPU-runnable workload end to end
+    from helpers import CONFIG0
+    r0, d0, l0 = cf.synth_dataset(4, 160, 120, sites=12, classes=5, zero_frac=0.05, seed0=4711)
+    with _OracleThreads(threads):
+        t0 = time.perf_counter()
+        ds0 = orc.Dataset.from_u8(r0, d0, l0)
+        of = ds0.train_forest(orc.config(**CONFIG0), 1, 6)
+        t_train = time.perf_counter() - t0
+        on, oh = of.tree(0)
+        post = [np.array([orc.normalize_histogram(h, of.prior(), 0.0, True) for h in oh])]
+        t1 = time.perf_counter()
+        for i in range(4):
+            ds0.predict(i, [on], post, 6)
+        t_pred = time.perf_counter() - t1
+    out["small_end_to_end"] = {"workload": "4 frames 160x120, 1 tree depth 8, F=100 x T=10, predict all 4 frames",
+                               "train_s": t_train, "predict_s": t_pred, "nodes": len(on), "cores": threads}
+    log(f"cpu baseline, small workload end to end: train {t_train:.2f}s ({len(on)} nodes), predict 4 frames "
+        f"{t_pred:.2f}s")
+
+    if forest10 is not None:
+        W, H = 640, 480
+        r3, d3, _ = cf.synth_dataset(4, W, H, sites=12, classes=5, zero_frac=0.05, seed0=5711)
+        ds3 = orc.Dataset.from_u8(r3, d3)
+        trees = [forest10.tree(t)[0] for t in range(forest10.num_trees)]
+        posts = [forest10.posteriors(t) for t in range(forest10.num_trees)]
+        with _OracleThreads(threads):
+            t0 = time.perf_counter()
+            for i in range(4):
+                ds3.predict(i, trees, posts, 5)
+            dt = time.perf_counter() - t0
+        out["dense_prediction"] = {"workload": "4 of the 64 frames 640x480, 10 complete trees of depth 20",
+                                   "mpix_per_s": 4 * W * H / dt / 1e6, "frames_per_s": 4 / dt, "cores": threads}
+        log(f"cpu baseline, dense prediction: {4 * W * H / dt / 1e6:.2f} Mpix/s ({4 / dt:.2f} frames/s)")
+    return out
 
 
 if __name__ == "__main__":

@@ -405,7 +405,8 @@ class RandomForestImage:
         self.ctx.check(lib.cf_forest_grow(self.ctx.h, im.h, self.h, ctypes.byref(c), mode))
 
     PROFILE_KEYS = ("evaluations", "seconds", "eval_kernel_ms", "eval_launches", "responses", "featgen_ms",
-                    "score_ms", "partition_ms", "levels", "replays", "nodes", "hist_ms")
+                    "score_ms", "partition_ms", "levels", "replays", "nodes", "hist_ms", "group_batches_max",
+                    "slot_batches_max")
 
     def profile(self):
         out = np.zeros(len(self.PROFILE_KEYS), np.float64)

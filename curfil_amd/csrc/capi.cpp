@@ -73,6 +73,15 @@ void checkFeature(int type, int c1, int c2) {
     if (type == T_COLOR && (c1 < 0 || c1 > 2 || c2 < 0 || c2 > 2)) throw Error(CF_ERR_ARG, "invalid colour channel");
 }
 
+// Depth in millimetres above 50 m is refused, as the reference's depth loader
+// refuses it (image.cpp:430); the reciprocal coordinate normalisation is proven
+// equal to the reference's division for depths up to 50000 mm.
+void checkDepthMm(const uint16_t* depth, size_t n) {
+    for (size_t p = 0; p < n; p++) {
+        if (depth[p] > 50000) throw Error(CF_ERR_ARG, "illegal depth value (> 50000 mm)");
+    }
+}
+
 void checkFeaturesSoA(const int8_t* soa, int F) {
     for (int f = 0; f < F; f++) {
         checkFeature(soa[static_cast<size_t>(P_TYPE) * F + f], static_cast<uint8_t>(soa[static_cast<size_t>(P_C1) * F + f]),
@@ -84,6 +93,13 @@ void uploadFeaturesSoA(cf_ctx* ctx, FeatTables& tabs, const int8_t* soa, const f
     tabs.soa.resize(static_cast<size_t>(F) * kFeatParams);
     tabs.thrTF.resize(static_cast<size_t>(F) * T);
     checkFeaturesSoA(soa, F);
+    // the reference's thresholds are responses of sampled pixels that are never NaN
+    // (random_tree_image.cpp:679-690); a NaN threshold has no rank among the others
+    if (thrTF) {
+        for (size_t k = 0; k < static_cast<size_t>(F) * T; k++) {
+            if (std::isnan(thrTF[k])) throw Error(CF_ERR_ARG, "NaN threshold");
+        }
+    }
     CF_HIP(hipMemcpyAsync(tabs.soa.get(), soa, static_cast<size_t>(F) * kFeatParams, hipMemcpyHostToDevice, ctx->stream));
     if (thrTF) {
         CF_HIP(hipMemcpyAsync(tabs.thrTF.get(), thrTF, static_cast<size_t>(F) * T * sizeof(float), hipMemcpyHostToDevice,
@@ -212,6 +228,8 @@ int cf_images_create(cf_ctx* ctx, int count, int width, int height, cf_images** 
         require(out != nullptr, "null output");
         require(count > 0 && width > 0 && height > 0, "image set dimensions must be positive");
         require(width <= 32767 && height <= 32767, "image too large");
+        require(static_cast<uint64_t>(width) * height * sizeof(int2) < (1ull << 32),
+                "image too large (an integral plane must stay below 4 GiB)");
         CF_HIP(hipSetDevice(ctx->device));
         auto im = std::make_unique<cf_images>();
         im->ctx = ctx;
@@ -257,6 +275,7 @@ int cf_images_set_rgbd_u8(cf_images* im, int index, const uint8_t* rgb, const ui
         require(rgb && depth, "null input");
         cf_ctx* ctx = im->ctx;
         const size_t px = static_cast<size_t>(im->W) * im->H;
+        checkDepthMm(depth, px);
         ctx->scratch0.resize(px * 3 + px * 2 + 64);
         ctx->scratch1.resize(px * 5 * sizeof(uint32_t));
         uint8_t* dRgb = ctx->scratch0.get();
@@ -283,6 +302,7 @@ int cf_images_set_rgbd_u8_padded(cf_images* im, int index, const uint8_t* rgb, c
         cf_ctx* ctx = im->ctx;
         const size_t px = static_cast<size_t>(width) * height;
         const size_t slot = static_cast<size_t>(im->W) * im->H;
+        checkDepthMm(depth, px);
         ctx->scratch0.resize(px * 3 + px * 2 + 64);
         ctx->scratch1.resize(px * 5 * sizeof(uint32_t));
         ctx->scratch2.resize(px * 3 * sizeof(float) + px * sizeof(int2) + 256);
@@ -663,7 +683,7 @@ int cf_forest_profile(const cf_forest* fo, double* out, int n) {
     const TrainStats& st = fo->stats;
     const double v[] = {st.evaluations, st.seconds, st.evalKernelMs, st.evalLaunches, st.responses, st.featGenMs,
             st.scoreMs, st.partitionMs, static_cast<double>(st.levels), static_cast<double>(st.replays), st.nodes,
-            st.histMs};
+            st.histMs, static_cast<double>(st.groupBatchesMax), static_cast<double>(st.slotBatchesMax)};
     const int k = static_cast<int>(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < k; i++) out[i] = v[i];
     return k;

@@ -26,6 +26,8 @@ struct TrainStats {
     double evalKernelMs = 0, evalLaunches = 0;
     double featGenMs = 0, scoreMs = 0, partitionMs = 0;
     double histMs = 0;   // two-pass evaluation: node histograms from the bin rows
+    // two-pass evaluation: most feature-group batches and node-slot batches of one level
+    int groupBatchesMax = 0, slotBatchesMax = 0;
 };
 
 // One tree to grow: its id, random source state and root samples.

@@ -483,13 +483,19 @@ __global__ void __launch_bounds__(256) k_threshold_ranks(const float* __restrict
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const float* th = thrFT + static_cast<size_t>(i) * T;
+    // order: by (value, index) with NaN after every number, so `order` is a permutation
+    // whatever the table holds (the callers refuse NaN thresholds)
     for (int t = 0; t < T; t++) {
         const float v = th[t];
+        const bool vNan = v != v;
         int r = 0, pos = 0;
         for (int k = 0; k < T; k++) {
             const float u = th[k];
+            const bool uNan = u != u;
+            const bool below = vNan ? !uNan : u < v;
+            const bool same = vNan ? uNan : u == v;
             r += (u < v) ? 1 : 0;
-            pos += (u < v || (u == v && k < t)) ? 1 : 0;
+            pos += (below || (same && k < t)) ? 1 : 0;
         }
         rank[static_cast<size_t>(i) * T + t] = static_cast<uint16_t>(r);
         order[static_cast<size_t>(i) * T + pos] = static_cast<uint16_t>(t);

@@ -371,6 +371,12 @@ int cf_server_submit(cf_server* sv, const uint8_t* rgb, const uint16_t* depth, i
     if (!sv || !rgb || !depth || !id || width <= 0 || height <= 0 || width > 32767 || height > 32767) {
         return CF_ERR_ARG;
     }
+    // an integral plane is addressed with 32-bit byte offsets
+    if (static_cast<uint64_t>(width) * height * sizeof(int2) >= (1ull << 32)) return CF_ERR_ARG;
+    // depth above 50 m is refused, as the reference's depth loader does (image.cpp:430)
+    for (size_t p = 0, px = static_cast<size_t>(width) * height; p < px; p++) {
+        if (depth[p] > 50000) return CF_ERR_ARG;
+    }
     // a frame that does not fit the device budget on its own could never be admitted
     if (slotBytes(width, height, sv->fo->C) > sv->budget) return CF_ERR_NOMEM;
     try {

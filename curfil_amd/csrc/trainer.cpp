@@ -81,19 +81,24 @@ private:
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
-size_t evalBudgetBytes() {
-    const char* e = std::getenv("CURFIL_EVAL_BUDGET_MB");
-    size_t mb = e ? static_cast<size_t>(std::atoll(e)) : 2048;
+// Device memory budgets of a batched level.  CURFIL_*_BUDGET_MB sets them in MiB
+// (at least 16); CURFIL_*_BUDGET_KB in KiB without that floor, so that small
+// workloads can be made to run the evaluation in several feature-group and
+// node-slot batches, as the full-size workloads do.
+size_t budgetBytes(const char* mbVar, const char* kbVar, size_t defaultMb) {
+    if (const char* kb = std::getenv(kbVar)) {
+        const long long v = std::atoll(kb);
+        return static_cast<size_t>(std::max(1LL, v)) << 10;
+    }
+    const char* e = std::getenv(mbVar);
+    size_t mb = e ? static_cast<size_t>(std::atoll(e)) : defaultMb;
     if (mb < 16) mb = 16;
     return mb << 20;
 }
 
-size_t binsBudgetBytes() {
-    const char* e = std::getenv("CURFIL_BINS_BUDGET_MB");
-    size_t mb = e ? static_cast<size_t>(std::atoll(e)) : 4096;
-    if (mb < 16) mb = 16;
-    return mb << 20;
-}
+size_t evalBudgetBytes() { return budgetBytes("CURFIL_EVAL_BUDGET_MB", "CURFIL_EVAL_BUDGET_KB", 2048); }
+
+size_t binsBudgetBytes() { return budgetBytes("CURFIL_BINS_BUDGET_MB", "CURFIL_BINS_BUDGET_KB", 4096); }
 
 bool isPure(const uint32_t* h, int C) {
     int nz = 0;
@@ -465,6 +470,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             const size_t flipOffset = anyBoth ? static_cast<size_t>(ng) * N : 0;
             dBins.resize(static_cast<size_t>(ng) * N * (anyBoth ? 2 : 1));
             int workGroups = -1, numWork = 0;
+            stats.groupBatchesMax = std::max(stats.groupBatchesMax, (groupsTotal + ng - 1) / ng);
             for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
                 const int ngb = std::min(ng, groupsTotal - g0);
                 if (ngb != workGroups) {
@@ -482,6 +488,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 stats.evalLaunches += 1;
                 const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
                 const int maxSlotsB = static_cast<int>(std::max<size_t>(1, budget / (perSlotB * sizeof(uint32_t))));
+                stats.slotBatchesMax = std::max(stats.slotBatchesMax, (numNodes + maxSlotsB - 1) / maxSlotsB);
                 for (int s0 = 0; s0 < numNodes;) {
                     const int s1 = std::min(numNodes, s0 + maxSlotsB);
                     const int ns = s1 - s0;

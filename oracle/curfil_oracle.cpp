@@ -31,11 +31,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace orc {
@@ -402,12 +404,10 @@ static int generateFeatures(const Dataset& ds, const std::vector<Sample>& sample
 // FeatureEvaluationCPU::operator(): random_tree_image.cpp:68-157 (counter index
 // [label][feature][threshold][lr], lr = !(r <= t)).
 // ---------------------------------------------------------------------------
-static void nodeCounters(const Dataset& ds, const std::vector<Sample>& samples, const int8_t* feats,
-        const float* thr, int F, int T, int C, uint32_t* counters) {
-    std::fill(counters, counters + static_cast<size_t>(C) * F * T * 2, 0u);
-    std::vector<Feature> ff(F);
-    for (int i = 0; i < F; i++) ff[i] = featureFromSoA(feats, F, i);
-    for (const Sample& s : samples) {
+static void nodeCountersRange(const Dataset& ds, const Sample* samples, size_t n, const std::vector<Feature>& ff,
+        const float* thr, int F, int T, uint32_t* counters) {
+    for (size_t k = 0; k < n; k++) {
+        const Sample& s = samples[k];
         const Image& im = ds.images[s.image];
         for (int f = 0; f < F; f++) {
             double v1 = 0, v2 = 0;
@@ -425,6 +425,43 @@ static void nodeCounters(const Dataset& ds, const std::vector<Sample>& samples, 
                 }
             }
         }
+    }
+}
+
+// Host threads for the counters (ORC_THREADS, default: the hardware threads, at most 16).
+static int oracleThreads() {
+    if (const char* e = std::getenv("ORC_THREADS")) return std::max(1, std::atoi(e));
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::min(16u, std::max(1u, hw)));
+}
+
+// The samples are split into chunks of at least 100 (the reference's grain,
+// random_tree_image.cpp:343), each thread counts into its own counters and the
+// counters are summed afterwards (random_tree_image.cpp:362-371).  Integer
+// sums: the result does not depend on the split.
+static void nodeCounters(const Dataset& ds, const std::vector<Sample>& samples, const int8_t* feats,
+        const float* thr, int F, int T, int C, uint32_t* counters) {
+    const size_t total = static_cast<size_t>(C) * F * T * 2;
+    std::fill(counters, counters + total, 0u);
+    std::vector<Feature> ff(F);
+    for (int i = 0; i < F; i++) ff[i] = featureFromSoA(feats, F, i);
+    const size_t n = samples.size();
+    const size_t threads = static_cast<size_t>(oracleThreads());
+    const size_t grain = std::max<size_t>(100, (n + threads - 1) / threads);
+    const size_t parts = std::max<size_t>(1, (n + grain - 1) / grain);
+    if (parts == 1 || static_cast<size_t>(F) * n < 200000) {
+        nodeCountersRange(ds, samples.data(), n, ff, thr, F, T, counters);
+        return;
+    }
+    std::vector<std::vector<uint32_t>> local(parts, std::vector<uint32_t>(total, 0u));
+    std::vector<std::thread> pool;
+    for (size_t p = 0; p < parts; p++) {
+        const size_t b = p * grain, e = std::min(n, b + grain);
+        pool.emplace_back([&, p, b, e] { nodeCountersRange(ds, samples.data() + b, e - b, ff, thr, F, T, local[p].data()); });
+    }
+    for (auto& t : pool) t.join();
+    for (size_t p = 0; p < parts; p++) {
+        for (size_t k = 0; k < total; k++) counters[k] += local[p][k];
     }
 }
 
@@ -1079,40 +1116,52 @@ void orc_predict(void* p, int img, int numTrees, const int* numNodes, const Node
         int C, float* probs /*[C][H][W]*/, uint8_t* labels /*[H][W]*/) {
     const Image& im = static_cast<Dataset*>(p)->images[img];
     const int W = im.W, H = im.H;
-    for (int y = 0; y < H; y++) {
-        for (int x = 0; x < W; x++) {
-            std::vector<float> acc(C, 0.0f);
-            const int32_t d = pixelDepth(im, x, y);
-            size_t base = 0;
-            for (int t = 0; t < numTrees; t++) {
-                const NodeOut* tn = nodes + base;
-                int i = 0;
-                while (!tn[i].isLeaf) {
-                    Feature f;
-                    const int8_t* q = tn[i].feature;
-                    f.type = q[F_TYPE]; f.o1x = q[F_O1X]; f.o1y = q[F_O1Y]; f.o2x = q[F_O2X]; f.o2y = q[F_O2Y];
-                    f.r1x = q[F_R1X]; f.r1y = q[F_R1Y]; f.r2x = q[F_R2X]; f.r2y = q[F_R2Y];
-                    f.c1 = static_cast<uint8_t>(q[F_C1]); f.c2 = static_cast<uint8_t>(q[F_C2]);
-                    const double r = response(im, f, x, y, d, false);
-                    i = (r <= tn[i].threshold) ? tn[i].left : tn[i].right;
+    // rows are independent: split over host threads as the reference splits them over TBB tasks
+    // (random_forest_image.cpp:310)
+    auto rows = [&](int y0, int y1) {
+        for (int y = y0; y < y1; y++) {
+            for (int x = 0; x < W; x++) {
+                std::vector<float> acc(C, 0.0f);
+                const int32_t d = pixelDepth(im, x, y);
+                size_t base = 0;
+                for (int t = 0; t < numTrees; t++) {
+                    const NodeOut* tn = nodes + base;
+                    int i = 0;
+                    while (!tn[i].isLeaf) {
+                        Feature f;
+                        const int8_t* q = tn[i].feature;
+                        f.type = q[F_TYPE]; f.o1x = q[F_O1X]; f.o1y = q[F_O1Y]; f.o2x = q[F_O2X]; f.o2y = q[F_O2Y];
+                        f.r1x = q[F_R1X]; f.r1y = q[F_R1Y]; f.r2x = q[F_R2X]; f.r2y = q[F_R2Y];
+                        f.c1 = static_cast<uint8_t>(q[F_C1]); f.c2 = static_cast<uint8_t>(q[F_C2]);
+                        const double r = response(im, f, x, y, d, false);
+                        i = (r <= tn[i].threshold) ? tn[i].left : tn[i].right;
+                    }
+                    const double* h = post + (base + i) * C;
+                    for (int c = 0; c < C; c++) acc[c] += h[c];   // float += double
+                    base += numNodes[t];
                 }
-                const double* h = post + (base + i) * C;
-                for (int c = 0; c < C; c++) acc[c] += h[c];   // float += double
-                base += numNodes[t];
+                double sum = 0.0f;
+                for (int c = 0; c < C; c++) sum += acc[c];
+                float bestProb = -1.0f;
+                uint8_t lab = 0;
+                for (int c = 0; c < C; c++) {
+                    acc[c] /= sum;
+                    const float prob = acc[c];
+                    if (prob > bestProb) { lab = static_cast<uint8_t>(c); bestProb = prob; }
+                }
+                for (int c = 0; c < C; c++) probs[(static_cast<size_t>(c) * H + y) * W + x] = acc[c];
+                labels[static_cast<size_t>(y) * W + x] = lab;
             }
-            double sum = 0.0f;
-            for (int c = 0; c < C; c++) sum += acc[c];
-            float bestProb = -1.0f;
-            uint8_t lab = 0;
-            for (int c = 0; c < C; c++) {
-                acc[c] /= sum;
-                const float prob = acc[c];
-                if (prob > bestProb) { lab = static_cast<uint8_t>(c); bestProb = prob; }
-            }
-            for (int c = 0; c < C; c++) probs[(static_cast<size_t>(c) * H + y) * W + x] = acc[c];
-            labels[static_cast<size_t>(y) * W + x] = lab;
         }
+    };
+    const int threads = std::min(oracleThreads(), H);
+    const int per = (H + threads - 1) / threads;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) {
+        const int y0 = t * per, y1 = std::min(H, y0 + per);
+        if (y0 < y1) pool.emplace_back(rows, y0, y1);
     }
+    for (auto& t : pool) t.join();
 }
 
 // -- golden hash of feature_generation_test.cpp:310-387 ----------------------

@@ -101,3 +101,80 @@ def config0_dataset():
 
 CONFIG0 = dict(randomSeed=4711, samplesPerImage=1000, featureCount=100, minSampleCount=32, maxDepth=8,
                boxRadius=60, regionSize=10, thresholds=10, ignored_labels=(0,))
+
+
+def forced_level_case():
+    """Images and root samples whose tree is forced whatever features are drawn.
+
+    Image 0 (group B): colour 50 everywhere, depth 1000 mm; image 1 (group D):
+    colour 200, depth 1000 mm; image 2 (group A): depth 0 (missing) everywhere.
+    Samples sit in the centre, where every box of boxRadius 20 / regionSize 10
+    stays inside the 160x120 frame.  A colour feature answers v * (area1 -
+    area2) on B and D (the same value for every sample of a group), a depth
+    feature 0; a sample on missing depth answers NaN and always goes right
+    (random_tree_image.h:485-623).  Labels: B 20 x 1 and 20 x 2, D 20 x 4,
+    A 20 x 3.
+
+    The root's best split is B | A + D (normalised information gain 2/3,
+    against 0.577 for D | A + B and B + D | A).  At level 2 the B node (40
+    samples, impure) cannot be split: every threshold is one of the level's
+    responses, so all its samples fall on one side, the side is empty and the
+    two ids are rolled back; the A + D node (40 samples) then splits
+    perfectly into D (left) and A (right), whose ids follow the rolled-back
+    counter: 3 and 4.  (The reference reaches its rollback branch,
+    random_tree.h:1366-1394, only after throwing "no samples in left node"
+    for such a node; this build keeps the node as a leaf and rolls the ids
+    back as that branch intends.  With maxDepth 2 or minSampleCount above 40
+    no node has an empty side and the reference's rule applies unchanged.)
+    """
+    W, H = 160, 120
+    rgb = np.zeros((3, H, W, 3), np.uint8)
+    rgb[0] = 50
+    rgb[1] = 200
+    rgb[2] = 120
+    depth = np.zeros((3, H, W), np.uint16)
+    depth[0] = 1000
+    depth[1] = 1000
+    rng = np.random.default_rng(5)
+
+    def group(img, labels):
+        s = np.zeros(len(labels), orc.SAMPLE_DTYPE)
+        s["image"] = img
+        s["x"] = rng.integers(40, 120, len(labels))
+        s["y"] = rng.integers(35, 85, len(labels))
+        s["depth_mm"] = depth[img, s["y"], s["x"]]
+        s["label"] = labels
+        return s
+
+    roots = np.concatenate([group(0, [1] * 20 + [2] * 20), group(1, [4] * 20), group(2, [3] * 20)])
+    return rgb, depth, roots
+
+
+FORCED_TREE = {
+    # index: (id, level, parent, left, right, num_samples, is_leaf, histogram over labels 0..4)
+    0: (0, 1, -1, 1, 2, 80, 0, [0, 20, 20, 20, 20]),
+    1: (1, 2, 0, -1, -1, 40, 1, [0, 20, 20, 0, 0]),
+    2: (2, 2, 0, 3, 4, 40, 0, [0, 0, 0, 20, 20]),
+    3: (3, 3, 2, -1, -1, 20, 1, [0, 0, 0, 0, 20]),
+    4: (4, 3, 2, -1, -1, 20, 1, [0, 0, 0, 20, 0]),
+}
+
+
+def forced_expected(max_depth, min_samples):
+    """The forced tree's nodes for a maxDepth / minSampleCount: the root is level 1 and a level equal
+    to maxDepth is not evaluated (random_tree.h:1256-1259); a child grows iff it is impure and holds at
+    least minSampleCount samples (random_tree.h:1182-1191); the root is always evaluated."""
+    if max_depth <= 1:
+        keep = [0]
+    elif max_depth == 2 or min_samples > 40:
+        keep = [0, 1, 2]
+    else:
+        keep = [0, 1, 2, 3, 4]
+    out = []
+    for i in keep:
+        node = list(FORCED_TREE[i])
+        if node[3] not in keep:
+            node[3] = node[4] = -1
+            node[6] = 1
+        out.append(node)
+    return out

@@ -116,6 +116,31 @@ def test_invalid_features_rejected_on_host(ctx):
             cf.node_counters(ctx, im, samples, bad, thr, 2)
 
 
+def test_invalid_thresholds_depths_and_sizes_rejected_on_host(ctx):
+    """A NaN threshold (the reference never draws one: sampled thresholds are finite responses), depth
+    above 50000 mm (image.cpp:430) and images whose integral planes would pass 4 GiB are refused on the
+    host with CF_ERR_ARG before any kernel runs."""
+    from test_oracle_pins import complex_case
+    color, depth, feats, thr, samples = complex_case()
+    im = _images_f32(ctx, color, depth)
+    bad = thr.copy()
+    bad[1, 2] = np.nan
+    with pytest.raises(cf.CurfilError, match="NaN threshold"):
+        cf.node_counters(ctx, im, samples, feats, bad, 2)
+    with pytest.raises(cf.CurfilError, match="NaN threshold"):
+        cf.node_best_split(ctx, im, samples, feats, bad, 2, np.array([2, 2], np.uint32))
+    W, H = 16, 12
+    im8 = cf.Images(ctx, 1, W, H)
+    dep = np.full((H, W), 1000, np.uint16)
+    dep[3, 4] = 50001
+    with pytest.raises(cf.CurfilError, match="50000"):
+        im8.set_rgbd_u8(0, np.zeros((H, W, 3), np.uint8), dep)
+    with pytest.raises(cf.CurfilError, match="50000"):
+        im8.set_rgbd_u8_padded(0, np.zeros((H - 2, W - 2, 3), np.uint8), dep[:H - 2, :W - 2].copy())
+    with pytest.raises(cf.CurfilError, match="4 GiB"):
+        cf.Images(ctx, 1, 30000, 20000)
+
+
 def test_known_answer_depth_simple(ctx):
     W, H = 16, 20
     depth = np.zeros((1, H, W), np.int32)
