@@ -59,6 +59,8 @@ def main():
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window per serving rate")
     ap.add_argument("--io-frames", type=int, default=32, help="frames written as PNG files for the I/O leg")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="only the dense-prediction leg (10 trees of depth 20, 64-frame batches); for kernel profiles")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy a device buffer of this many MiB 3 times (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -79,6 +81,12 @@ def main():
 
     import numpy as np
     import curfil_amd as cf
+
+    if args.dense_only:
+        ctx = cf.Context(0)
+        predict10, _ = dense_prediction_leg(ctx)
+        print(json.dumps({"predict_forest10": predict10}), flush=True)
+        return
 
     W, H, N = 640, 480, args.frames
     base_kw = dict(randomSeed=4711, minSampleCount=50, boxRadius=120, regionSize=20, ignored_labels=(0,))

@@ -113,6 +113,12 @@ def _load():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    lib.cf_eval_shape_info.argtypes = [I, P, P, P, P]
+    lib.cf_eval_shape_info.restype = I
+    lib.cf_eval_tuning.argtypes = [P, I]
+    lib.cf_eval_tuning.restype = I
+    lib.cf_eval_tuning_reset.argtypes = []
+    lib.cf_eval_tuning_reset.restype = I
     lib.cf_last_error.argtypes = [P]
     lib.cf_last_error.restype = ctypes.c_char_p
     lib.cf_server_error.argtypes = [P]
@@ -406,7 +412,7 @@ class RandomForestImage:
 
     PROFILE_KEYS = ("evaluations", "seconds", "eval_kernel_ms", "eval_launches", "responses", "featgen_ms",
                     "score_ms", "partition_ms", "levels", "replays", "nodes", "hist_ms", "group_batches_max",
-                    "slot_batches_max")
+                    "slot_batches_max", "eval_shape")
 
     def profile(self):
         out = np.zeros(len(self.PROFILE_KEYS), np.float64)
@@ -924,6 +930,38 @@ def synth_forest(ctx, images, depth_mm, trees=10, depth=20, classes=5, seed=4711
         hist[n_split:] = np.maximum(np.rint(e / e.sum(axis=1, keepdims=True) * (1 << 20)), 0).astype(np.uint32)
         out.append((nodes, hist))
     return RandomForestImage.from_trees(ctx, out, classes)
+
+
+def eval_shape_info(shape):
+    """(lanes, samples per lane, features per step, workgroups per CU) of a pass-1 launch shape, or None."""
+    v = [ctypes.c_int() for _ in range(4)]
+    if lib.cf_eval_shape_info(shape, *[ctypes.byref(x) for x in v]) != CF_OK:
+        return None
+    return tuple(x.value for x in v)
+
+
+def eval_shapes():
+    out = []
+    while eval_shape_info(len(out)) is not None:
+        out.append(eval_shape_info(len(out)))
+    return out
+
+
+def eval_tuning():
+    """The pass-1 tuner's table: one dict per (threshold width, level size class)."""
+    n = lib.cf_eval_tuning(None, 0)
+    buf = np.zeros(6 * max(n, 1), np.float64)
+    lib.cf_eval_tuning(_ptr(buf), len(buf))
+    rows = []
+    for k in range(n):
+        tm, cls, best, ns_best, ns_default, timed = buf[6 * k:6 * k + 6]
+        rows.append({"threshold_width": int(tm), "size_class": int(cls), "shape": int(best),
+                     "ns_per_response": ns_best, "default_ns_per_response": ns_default, "launches_timed": int(timed)})
+    return rows
+
+
+def eval_tuning_reset():
+    lib.cf_eval_tuning_reset()
 
 
 def synth_dataset(n, width, height, sites, classes, zero_frac, seed0=4711, threads=8):

@@ -20,10 +20,12 @@ SOURCES = [
     "k_features.hip",
     "k_eval.hip",
     "k_bins.hip",
+    "k_sort.hip",
     "k_partition.hip",
     "k_classify.hip",
     "capi.cpp",
     "trainer.cpp",
+    "tuner.cpp",
     "sampling.cpp",
     "synth.cpp",
     "server.cpp",

@@ -28,6 +28,8 @@ struct TrainStats {
     double histMs = 0;   // two-pass evaluation: node histograms from the bin rows
     // two-pass evaluation: most feature-group batches and node-slot batches of one level
     int groupBatchesMax = 0, slotBatchesMax = 0;
+    // two-pass evaluation: pass-1 launches per launch shape (see evalShape)
+    double shapeLaunches[16] = {};
 };
 
 // One tree to grow: its id, random source state and root samples.

@@ -184,9 +184,8 @@ void allowLds(const void* kernel, size_t bytes);
 
 constexpr int kBinGroup = 16;            // features per bin row (16 x 8-bit bins = one uint4 per sample)
 constexpr int kMaxBinThresholds = 255;   // bins 0..T must fit in 8 bits
-// Samples per workgroup of pass 1 (one per lane of a 256-lane workgroup).
-constexpr int kBinBlockSamples = 256;
-// a block of <= kBinBlockSamples consecutive entries of the image-major permutation, all of one (tree, image)
+// a block of consecutive entries of the image-major permutation (at most a pass-1 workgroup's samples),
+// all of one (tree, image)
 struct BinBlock {
     int tree;
     int begin;   // first position in the permutation
@@ -202,10 +201,14 @@ void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, 
         uint32_t* keys, uint32_t* vals, int flipN, uint32_t absentKey, int yBits);
 // label per level index, bit 7 set for samples still set to Both (counted in both orientations)
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels);
-size_t radixSortTempBytes(int n, int bits);
-void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
-        const uint32_t* valsIn, uint32_t* valsOut, int n, int bits);
-void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int yBits, int* starts);
+// Stable counting sort of n level entries by key (< levelSortBuckets(numSegs, yBits)): out = the values
+// in key order, starts[seg] = first position of segment seg (key >> yBits) or -1 if it is empty, and
+// starts[numSegs] = the number of entries with a key below numSegs << yBits.  count and cursor hold
+// levelSortBuckets entries, blockSum levelSortScanBlocks.
+size_t levelSortBuckets(int numSegs, int yBits);
+size_t levelSortScanBlocks(int numSegs, int yBits);
+void levelCountingSort(hipStream_t s, const uint32_t* keys, const uint32_t* vals, int n, int numSegs, int yBits,
+        uint32_t* count, uint32_t* cursor, uint32_t* blockSum, uint32_t* out, int* starts);
 // Work list of pass 1 for ngb feature groups, built on the device: segment i (its blocks
 // segFirst[i] .. + segCount[i]) belongs to XCD i % 8; XCD x takes its segments in order, all groups
 // of a segment (group major, block minor) before the next, and workgroup w of the launch runs on
@@ -216,9 +219,24 @@ void launchBuildWork(hipStream_t s, const int* segFirst, const int* segCount, co
         const int* xcdBlocks, int numSegs, int ngb, int maxLen, int2* work);
 // bins[g][level index] for the work items (block, g) of feature groups g0 + g of the evaluation order;
 // permutation entries with kFlipEntry are evaluated flipped into bins[flipOffset + g * N + index]
+// with launch shape `shape` (evalShape): blocks of shape.lanes * shape.perLane samples
 void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
         const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
-        const FeatTables& tabs, int N, uint4* bins, size_t flipOffset = 0);
+        const FeatTables& tabs, int N, uint4* bins, size_t flipOffset, int shape);
+// A launch shape of pass 1: lanes per workgroup, samples per lane (a workgroup's block holds
+// lanes * perLane samples), features whose corner gathers are issued together, workgroups per CU (held
+// by an LDS reservation).
+struct EvalShape {
+    int lanes, perLane, featPerStep, wgPerCu;
+};
+int numEvalShapes();
+const EvalShape& evalShape(int i);
+// Pass-1 shape tuner: the shape for a launch of threshold width TM at a level of `entries`
+// permutation entries.  *measure asks the caller to time the launch and report it with
+// evalTunerRecord.  CURFIL_EVAL_SHAPE=i pins shape i, CURFIL_TUNE=0 keeps shape 0,
+// CURFIL_TUNE_FILE keeps the choices between runs.
+int evalTunerChoose(int TM, size_t entries, bool* measure);
+void evalTunerRecord(int TM, size_t entries, int shape, double ms, double responses);
 // padded threshold width of the bin kernel for T thresholds
 int binThresholdWidth(int T);
 // features and +inf-padded thresholds in evaluation order: gFeat[trees * F], gThr[trees * F * binThresholdWidth(T)]

@@ -32,8 +32,6 @@
 // segment rather than many, the planes they read fit the XCD's L2 (1.3%
 // misses instead of 3.8%) and requests return sooner.  The launch reserves
 // LDS the kernel does not use to hold the CU at two workgroups.
-#include <hipcub/hipcub.hpp>
-
 #include <map>
 #include <mutex>
 
@@ -42,11 +40,6 @@
 namespace cf {
 
 namespace {
-
-constexpr int kBinThreads = 256;
-constexpr int kBinPerThread = 1;
-static_assert(kBinThreads * kBinPerThread == kBinBlockSamples, "one pass of a workgroup covers one block");
-constexpr int kBinWorkgroupsPerCu = 2;
 
 struct BinSample {
     double rinv;   // depthReciprocal(depth in metres)
@@ -88,14 +81,59 @@ __device__ __forceinline__ void pushBin(uint32_t (&w)[4], uint32_t b) {
     w[3] = (w[3] >> 8) | (b << 24);
 }
 
-}  // namespace
+// The 8 corners of one (sample, feature) response: depth corners as {sum, count}, colour corners'
+// float bits in .x.  Returns false (and issues no load) when the sample has no valid depth or a box
+// leaves the image.
+__device__ __forceinline__ bool gatherCorners(const Feat& fr, const BinSample& rs, const float* col,
+        const int2* dep, int W, int H, size_t plane, int2 (&cv)[8]) {
+    const int lab = rs.lab;
+    int idx[8];
+    const bool ok = lab >= 0 && ((lab >> 10) & 1) == 0 &&
+            responseCorners(fr, rs.xy & 0xffff, rs.xy >> 16, W, H, rs.rinv, ((lab >> 8) & 3) == 1, idx);
+    if (ok) {
+        if (fr.type == T_DEPTH) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) cv[c] = planeAt(dep, idx[c]);
+        } else {
+            const float* colA = col + static_cast<size_t>(fr.c1) * plane;
+            const float* colB = col + static_cast<size_t>(fr.c2) * plane;
+#pragma unroll
+            for (int c = 0; c < 4; c++) cv[c].x = __float_as_int(planeAt(colA, idx[c]));
+#pragma unroll
+            for (int c = 4; c < 8; c++) cv[c].x = __float_as_int(planeAt(colB, idx[c]));
+        }
+    }
+    return ok;
+}
 
 template<int TM>
-__global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restrict__ colBase,
+__device__ __forceinline__ uint32_t binFromCorners(const Feat& fr, bool ok, const int2 (&cv)[8],
+        const float (&th)[TM], int T) {
+    if (!ok) return static_cast<uint32_t>(T);
+    double r;
+    if (fr.type == T_DEPTH) {
+        r = depthResponseFromCorners(cv);
+    } else {
+        float c8[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[c].x);
+        r = colorResponseFromCorners(c8);
+    }
+    return static_cast<uint32_t>(binOfT<TM>(roundUpToFloat(r), th, T));
+}
+
+}  // namespace
+
+// Pass 1 with the launch shape <LANES lanes per workgroup, SPL samples per
+// lane, FPS features whose corner gathers are issued together>: a workgroup
+// covers one block of LANES * SPL samples.
+template<int TM, int LANES, int SPL, int FPS>
+__global__ void __launch_bounds__(LANES) k_eval_bins(const float* __restrict__ colBase,
         const int2* __restrict__ depBase, int W, int H, const cf_sample* __restrict__ samples,
         const uint32_t* __restrict__ perm, const BinBlock* __restrict__ blocks, const int2* __restrict__ work,
         int g0, const FeatRec* __restrict__ gFeat, const float* __restrict__ gThr, int F, int T, int N,
         uint4* __restrict__ bins, size_t flipOffset) {
+    static_assert(kBinGroup % FPS == 0, "features per step divide the group");
     // work[blockIdx] = (sample block, feature group within the batch); the
     // XCDs' lists are interleaved (workgroup w runs on XCD w % 8) so each XCD
     // sweeps one image (the segments of all trees) for every group of the
@@ -112,12 +150,12 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
     const int pos0 = (g0 + g) * kBinGroup;
     uint4* out = bins + static_cast<size_t>(g) * N;
     uint4* outFlip = bins + flipOffset + static_cast<size_t>(g) * N;
-    for (int base = 0; base < bb.count; base += kBinThreads * kBinPerThread) {
-        BinSample rs[kBinPerThread];
-        int outIdx[kBinPerThread];
+    for (int base = 0; base < bb.count; base += LANES * SPL) {
+        BinSample rs[SPL];
+        int outIdx[SPL];
 #pragma unroll
-        for (int k = 0; k < kBinPerThread; k++) {
-            const int i = base + k * kBinThreads + threadIdx.x;
+        for (int k = 0; k < SPL; k++) {
+            const int i = base + k * LANES + threadIdx.x;
             // empty lanes keep in-range fields (they issue no gathers)
             rs[k].lab = -1;
             rs[k].xy = 0;
@@ -137,71 +175,55 @@ __global__ void __launch_bounds__(kBinThreads) k_eval_bins(const float* __restri
                 rs[k].lab = ((flipEntry ? 1 : x.flip) << 8) | ((valid ? 0 : 1) << 10) | (flipEntry ? 1 << 12 : 0);
             }
         }
-        uint32_t row[kBinPerThread][4];
+        uint32_t row[SPL][4];
 #pragma unroll
-        for (int k = 0; k < kBinPerThread; k++) row[k][0] = row[k][1] = row[k][2] = row[k][3] = 0u;
+        for (int k = 0; k < SPL; k++) row[k][0] = row[k][1] = row[k][2] = row[k][3] = 0u;
 
-        for (int fi = 0; fi < kBinGroup; fi++) {
-            const int pos = pos0 + fi;
-            if (pos >= F) {   // uniform: the last group of the table may be partial
+        // thresholds are uniform (scalar loads): with few of them in flight they are loaded together with
+        // the features, before the gathers
+        constexpr bool kThFirst = FPS * TM <= 64;
+        for (int fi = 0; fi < kBinGroup; fi += FPS) {
+            Feat fr[FPS];
+            bool live[FPS];
+            int2 cv[FPS][SPL][8];
+            bool ok[FPS][SPL];
+            float th[kThFirst ? FPS : 1][TM];
+            // the gathers of FPS features x SPL samples are all issued before any response is formed
 #pragma unroll
-                for (int k = 0; k < kBinPerThread; k++) pushBin(row[k], 0u);
-                continue;
+            for (int j = 0; j < FPS; j++) {
+                const size_t tp = static_cast<size_t>(bb.tree) * F + pos0 + fi + j;
+                live[j] = pos0 + fi + j < F;   // uniform: the last group of the table may be partial
+                fr[j] = live[j] ? loadFeat(gFeat + tp) : Feat{};
+                if (kThFirst && live[j]) {
+#pragma unroll
+                    for (int t = 0; t < TM; t++) th[kThFirst ? j : 0][t] = gThr[tp * TM + t];
+                }
+#pragma unroll
+                for (int k = 0; k < SPL; k++) {
+                    ok[j][k] = live[j] && gatherCorners(fr[j], rs[k], col, dep, W, H, plane, cv[j][k]);
+                }
             }
-            const size_t tp = static_cast<size_t>(bb.tree) * F + pos;
-            const Feat fr = loadFeat(gFeat + tp);
-            float th[TM];
-            const float* thp = gThr + tp * TM;
 #pragma unroll
-            for (int t = 0; t < TM; t++) th[t] = thp[t];
-            if (fr.type == T_DEPTH) {
-                int2 cv[kBinPerThread][8];
-                bool ok[kBinPerThread];
+            for (int j = 0; j < FPS; j++) {
+                if (!live[j]) {
 #pragma unroll
-                for (int k = 0; k < kBinPerThread; k++) {
-                    const int lab = rs[k].lab;
-                    int idx[8];
-                    ok[k] = lab >= 0 && ((lab >> 10) & 1) == 0 &&
-                            responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
-                                    ((lab >> 8) & 3) == 1, idx);
-                    if (ok[k]) {   // lanes whose box leaves the image issue no gathers
+                    for (int k = 0; k < SPL; k++) pushBin(row[k], 0u);
+                    continue;
+                }
+                const int tj = kThFirst ? j : 0;
+                if (!kThFirst) {
+                    const size_t tp = static_cast<size_t>(bb.tree) * F + pos0 + fi + j;
 #pragma unroll
-                        for (int c = 0; c < 8; c++) cv[k][c] = planeAt(dep, idx[c]);
-                    }
+                    for (int t = 0; t < TM; t++) th[tj][t] = gThr[tp * TM + t];
                 }
 #pragma unroll
-                for (int k = 0; k < kBinPerThread; k++) {
-                    const int b = ok[k] ? binOfT<TM>(roundUpToFloat(depthResponseFromCorners(cv[k])), th, T) : T;
-                    pushBin(row[k], static_cast<uint32_t>(b));
-                }
-            } else {
-                float cv[kBinPerThread][8];
-                bool ok[kBinPerThread];
-#pragma unroll
-                for (int k = 0; k < kBinPerThread; k++) {
-                    const int lab = rs[k].lab;
-                    int idx[8];
-                    ok[k] = lab >= 0 && ((lab >> 10) & 1) == 0 &&
-                            responseCorners(fr, rs[k].xy & 0xffff, rs[k].xy >> 16, W, H, rs[k].rinv,
-                                    ((lab >> 8) & 3) == 1, idx);
-                    const float* colA = col + static_cast<size_t>(fr.c1) * plane;
-                    const float* colB = col + static_cast<size_t>(fr.c2) * plane;
-                    if (ok[k]) {
-#pragma unroll
-                        for (int c = 0; c < 4; c++) cv[k][c] = planeAt(colA, idx[c]);
-#pragma unroll
-                        for (int c = 4; c < 8; c++) cv[k][c] = planeAt(colB, idx[c]);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kBinPerThread; k++) {
-                    const int b = ok[k] ? binOfT<TM>(roundUpToFloat(colorResponseFromCorners(cv[k])), th, T) : T;
-                    pushBin(row[k], static_cast<uint32_t>(b));
+                for (int k = 0; k < SPL; k++) {
+                    pushBin(row[k], binFromCorners<TM>(fr[j], ok[j][k], cv[j][k], th[tj], T));
                 }
             }
         }
 #pragma unroll
-        for (int k = 0; k < kBinPerThread; k++) {
+        for (int k = 0; k < SPL; k++) {
             if (outIdx[k] >= 0) {
                 // rows are read once, by pass 2: streaming stores keep them from evicting image planes from L2
                 uint4* o = (rs[k].lab & (1 << 12)) ? outFlip : out;
@@ -315,8 +337,8 @@ __global__ void __launch_bounds__(64) k_feature_eval_order(const FeatRec* __rest
     }
 }
 
-// Radix-sort keys (tree * images + image) and values (level index) of one
-// tree's level segment.
+// Sort keys ((tree * images + image) << yBits | row) and values (level index) of one tree's level
+// segment.
 __global__ void __launch_bounds__(256) k_level_keys(const cf_sample* __restrict__ samples, int begin, int n,
         uint32_t keyBase, uint32_t keyMul, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int flipN,
         uint32_t absentKey, int yBits) {
@@ -360,15 +382,6 @@ __global__ void __launch_bounds__(256) k_build_work(const int* __restrict__ segF
         item = make_int2(segFirst[seg] + static_cast<int>(local % nb), static_cast<int>(local / nb));
     }
     work[i] = item;
-}
-
-// starts[segment] = first position of segment (key >> yBits) in the sorted keys (absent segments left alone)
-__global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__ keys, int n, int yBits,
-        int* __restrict__ starts) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t seg = keys[i] >> yBits;
-    if (i == 0 || seg != (keys[i - 1] >> yBits)) starts[seg] = i;
 }
 
 __global__ void __launch_bounds__(256) k_level_labels(const cf_sample* __restrict__ samples, int n,
@@ -434,61 +447,67 @@ void launchBuildWork(hipStream_t s, const int* segFirst, const int* segCount, co
             segCount, xcdPrefix, xcdBlocks, numSegs, ngb, maxLen, work);
 }
 
-void launchKeyStarts(hipStream_t s, const uint32_t* keys, int n, int yBits, int* starts) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_key_starts, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, yBits, starts);
-}
-
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_level_labels, dim3((n + 255) / 256), dim3(256), 0, s, samples, n, labels);
 }
 
-size_t radixSortTempBytes(int n, int bits) {
-    size_t bytes = 0;
-    CF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const uint32_t*>(nullptr),
-            static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-            n, 0, bits));
-    return bytes;
-}
+// Launch shapes of pass 1 the tuner chooses from.  Shape 0 is the default:
+// two workgroups of four waves per CU, one sample per lane, one feature's
+// gathers at a time (the fastest of a manual sweep on the full-frame
+// workload).  The others trade samples in flight per CU (waves, samples per
+// lane, workgroups per CU) against L2 working set, and issue two features'
+// gathers together.
+constexpr EvalShape kEvalShapes[] = {
+    {256, 1, 1, 2},
+    {256, 1, 2, 2},
+    {256, 1, 1, 3},
+    {128, 1, 1, 4},
+    {128, 2, 1, 2},
+    {256, 2, 1, 1},
+    {512, 1, 1, 1},
+    {128, 1, 2, 4},
+};
+constexpr int kNumEvalShapes = static_cast<int>(sizeof(kEvalShapes) / sizeof(kEvalShapes[0]));
+static_assert(kNumEvalShapes <= 16, "TrainStats counts launches of at most 16 shapes");
 
-void radixSortPairs(hipStream_t s, void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut,
-        const uint32_t* valsIn, uint32_t* valsOut, int n, int bits) {
-    CF_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n, 0, bits, s));
+int numEvalShapes() { return kNumEvalShapes; }
+
+const EvalShape& evalShape(int i) {
+    if (i < 0 || i >= kNumEvalShapes) throw Error(CF_ERR_ARG, "unknown pass-1 launch shape");
+    return kEvalShapes[i];
 }
 
 // Dynamic LDS that k_eval_bins reserves (and does not use) so that exactly
-// kBinWorkgroupsPerCu workgroups fit a CU: more than 1 / (w + 1) and at most
-// 1 / w of the CU's LDS; 64 KB where that lies inside (160 KB CUs), else the
-// middle of the interval.
-static size_t evalBinsLdsBytes() {
+// w workgroups fit a CU: more than 1 / (w + 1) and at most 1 / w of the CU's
+// LDS; 64 KB where that lies inside (w = 2 on 160 KB CUs), else the middle of
+// the interval.
+static size_t evalBinsLdsBytes(int w) {
     static std::mutex mu;
-    static std::map<int, size_t> bytes;
+    static std::map<std::pair<int, int>, size_t> bytes;
     int dev = 0;
     CF_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> g(mu);
-    auto it = bytes.find(dev);
+    auto it = bytes.find({dev, w});
     if (it != bytes.end()) return it->second;
     int perCu = 0;
     CF_HIP(hipDeviceGetAttribute(&perCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    const size_t lo = static_cast<size_t>(perCu) / (kBinWorkgroupsPerCu + 1) + 1;
-    const size_t hi = static_cast<size_t>(perCu) / kBinWorkgroupsPerCu;
+    const size_t lo = static_cast<size_t>(perCu) / (w + 1) + 1;
+    const size_t hi = static_cast<size_t>(perCu) / w;
     size_t b = 64 * 1024;
     if (b < lo || b > hi) b = ((lo + hi) / 2) & ~static_cast<size_t>(255);
-    return bytes[dev] = b;
+    return bytes[{dev, w}] = b;
 }
 
-void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
-        const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
-        const FeatTables& tabs, int N, uint4* bins, size_t flipOffset) {
-    if (numWork == 0) return;
-    const int TM = binThresholdWidth(tabs.T);
-    const dim3 grid(static_cast<unsigned>(numWork)), block(kBinThreads);
-    const size_t lds = evalBinsLdsBytes();
+template<int LANES, int SPL, int FPS>
+static void launchEvalBinsShape(hipStream_t s, int TM, size_t lds, const ImageSet& imgs, const cf_sample* samples,
+        const uint32_t* perm, const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat,
+        const float* gThr, const FeatTables& tabs, int N, uint4* bins, size_t flipOffset) {
+    const dim3 grid(static_cast<unsigned>(numWork)), block(LANES);
 #define CF_EVAL_BINS(TMV)                                                                                        \
-    allowLds(reinterpret_cast<const void*>(&k_eval_bins<TMV>), lds);                                            \
-    hipLaunchKernelGGL(k_eval_bins<TMV>, grid, block, lds, s, imgs.col, imgs.dep, imgs.W, imgs.H, samples, perm, \
-            blocks, work, g0, gFeat, gThr, tabs.F, tabs.T, N, bins, flipOffset)
+    allowLds(reinterpret_cast<const void*>(&k_eval_bins<TMV, LANES, SPL, FPS>), lds);                            \
+    hipLaunchKernelGGL((k_eval_bins<TMV, LANES, SPL, FPS>), grid, block, lds, s, imgs.col, imgs.dep, imgs.W,     \
+            imgs.H, samples, perm, blocks, work, g0, gFeat, gThr, tabs.F, tabs.T, N, bins, flipOffset)
     switch (TM) {
         case 8: CF_EVAL_BINS(8); break;
         case 12: CF_EVAL_BINS(12); break;
@@ -500,6 +519,30 @@ void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* sample
         default: CF_EVAL_BINS(kMaxBinThresholds); break;
     }
 #undef CF_EVAL_BINS
+}
+
+void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* samples, const uint32_t* perm,
+        const BinBlock* blocks, const int2* work, int numWork, int g0, const FeatRec* gFeat, const float* gThr,
+        const FeatTables& tabs, int N, uint4* bins, size_t flipOffset, int shape) {
+    if (numWork == 0) return;
+    const int TM = binThresholdWidth(tabs.T);
+    const EvalShape& sh = evalShape(shape);
+    const size_t lds = evalBinsLdsBytes(sh.wgPerCu);
+#define CF_SHAPE(L, S, F)                                                                                         \
+    if (sh.lanes == L && sh.perLane == S && sh.featPerStep == F) {                                               \
+        launchEvalBinsShape<L, S, F>(s, TM, lds, imgs, samples, perm, blocks, work, numWork, g0, gFeat, gThr, tabs, \
+                N, bins, flipOffset);                                                                            \
+        return;                                                                                                  \
+    }
+    CF_SHAPE(256, 1, 1)
+    CF_SHAPE(256, 1, 2)
+    CF_SHAPE(128, 1, 1)
+    CF_SHAPE(128, 2, 1)
+    CF_SHAPE(256, 2, 1)
+    CF_SHAPE(512, 1, 1)
+    CF_SHAPE(128, 1, 2)
+#undef CF_SHAPE
+    throw Error(CF_ERR_RUNTIME, "pass-1 launch shape without a kernel");
 }
 
 void launchHistBins(hipStream_t s, const uint4* bins, const uint8_t* labels, const Chunk* chunks, int numChunks,

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 
 #include "cf_host.h"
 
@@ -38,6 +39,33 @@ struct ActiveTree {
     std::vector<int> nodes;   // frontier: node indices into out[job].nodes
     std::vector<int> counts;  // samples per frontier node
     int begin = 0, n = 0;     // segment of the level buffer
+};
+
+// The pass-1 blocks of one block size at the current level, and the work list of a batch size.
+struct BlockSet {
+    int level = -1;        // level the blocks were built for
+    DevBuf<BinBlock> blocks;
+    DevBuf<int> segInfo;   // per image group: first block, block count, XCD prefix; then the XCD totals
+    int numSegs = 0, maxXcdBlocks = 0;
+    DevBuf<int2> work;
+    int workGroups = -1;   // feature groups the work list was built for
+    int numWork = 0;
+};
+
+// An event pair for timing launches the tuner measures.
+class EventPair {
+public:
+    EventPair() {
+        CF_HIP(hipEventCreate(&a));
+        CF_HIP(hipEventCreate(&b));
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    hipEvent_t a = nullptr, b = nullptr;
 };
 
 struct LevelNode {
@@ -193,11 +221,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<int2> dChunkCounts;
     DevBuf<uint32_t> dLeftHist;
     DevBuf<int2> dDst;
-    DevBuf<uint32_t> dKeys, dKeys2, dVals, dPerm;
-    DevBuf<uint8_t> dSortTemp, dLab8;
-    DevBuf<BinBlock> dBlocks;
-    DevBuf<int2> dWork;
-    DevBuf<int> dSegInfo;
+    DevBuf<uint32_t> dKeys, dVals, dPerm, dSortCount, dSortCursor, dSortBlocks;
+    DevBuf<uint8_t> dLab8;
+    std::map<int, BlockSet> blockSets;   // by pass-1 block size
+    EventPair tuneEvents;
+    const hipEvent_t tuneStart = tuneEvents.a, tuneEnd = tuneEvents.b;
     DevBuf<uint32_t> dRepHist;
     DevBuf<int> dRepTree;
     DevBuf<FeatSummary> dRepSumm;
@@ -362,47 +390,56 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         auto evalTwoPass = [&]() {
             int N = 0;
             for (int a = 0; a < numActive; a++) N = std::max(N, active[a].begin + active[a].n);
-            // image-major order of the level: (tree, image) keys, stable radix sort of level indices.  With
-            // Both samples a second entry per level index evaluates it flipped; the entries of samples that
-            // are not Both get the key one past the last (tree, image) and sort to the end.
+            // image-major order of the level: (tree, image) keys, stable counting sort of level indices.
+            // With Both samples a second entry per level index evaluates it flipped; the entries of samples
+            // that are not Both get the key one past the last (tree, image) and sort to the end.
             const int M = anyBoth ? 2 * N : N;
             dKeys.resize(M);
-            dKeys2.resize(M);
             dVals.resize(M);
             dPerm.resize(M);
             dLab8.resize(N);
             const uint64_t keyRange = static_cast<uint64_t>(numActive) * static_cast<uint64_t>(images->count);
-            int bits = 1;
-            while ((1ull << bits) <= keyRange) bits++;
-            // within a segment, samples ordered by row when the key still fits 32 bits (CURFIL_ROW_ORDER=0: off)
+            if (keyRange >= (1ull << 27)) throw Error(CF_ERR_UNSUPPORTED, "too many (tree, image) segments in one level");
+            // within a segment, samples ordered by row while the buckets stay below 2^26 (CURFIL_ROW_ORDER=0: off)
             int yBits = 0;
             while ((1 << yBits) < images->H) yBits++;
             const char* rowOrder = std::getenv("CURFIL_ROW_ORDER");
-            if (bits + yBits > 32 || (rowOrder && std::strcmp(rowOrder, "0") == 0)) yBits = 0;
-            bits += yBits;
+            if (((keyRange + 2) << yBits) > (1ull << 26) || (rowOrder && std::strcmp(rowOrder, "0") == 0)) yBits = 0;
             // segments (image, tree) in image-major order: the trees' segments of one image are adjacent
             for (int a = 0; a < numActive; a++) {
                 launchLevelKeys(s, level[cur].get(), active[a].begin, active[a].n, static_cast<uint32_t>(a),
                         static_cast<uint32_t>(numActive), dKeys.get(), dVals.get(), anyBoth ? N : 0,
                         static_cast<uint32_t>(keyRange), yBits);
             }
-            const size_t sortBytes = radixSortTempBytes(M, bits);
-            dSortTemp.resize(std::max<size_t>(sortBytes, 1));
-            radixSortPairs(s, dSortTemp.get(), sortBytes, dKeys.get(), dKeys2.get(), dVals.get(), dPerm.get(), M, bits);
+            const int numKeys = static_cast<int>(keyRange);
+            dSortCount.resize(levelSortBuckets(numKeys, yBits));
+            dSortCursor.resize(levelSortBuckets(numKeys, yBits));
+            dSortBlocks.resize(levelSortScanBlocks(numKeys, yBits));
+            dStarts.resize(numKeys + 1);
+            levelCountingSort(s, dKeys.get(), dVals.get(), M, numKeys, yBits, dSortCount.get(), dSortCursor.get(),
+                    dSortBlocks.get(), dPerm.get(), dStarts.get());
             launchLevelLabels(s, level[cur].get(), N, dLab8.get());
             // (image, tree) segments of the sorted order -> blocks of <= kBinBlockSamples samples within a segment;
             // the segments of one image form a group of consecutive blocks
-            const int numKeys = static_cast<int>(keyRange);
-            dStarts.resize(numKeys + 1);
-            CF_HIP(hipMemsetAsync(dStarts.get(), 0xff, (numKeys + 1) * sizeof(int), s));
-            launchKeyStarts(s, dKeys2.get(), M, yBits, dStarts.get());
             std::vector<int> starts(numKeys + 1);
             CF_HIP(hipMemcpyAsync(starts.data(), dStarts.get(), (numKeys + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
             if (starts[numKeys] < 0) starts[numKeys] = M;   // the end of the last segment
-            std::vector<BinBlock> blocks;
-            std::vector<std::pair<int, int>> segBlocks;   // per image group: first block, block count
-            {
+            // blocks of one pass-1 block size: consecutive runs of <= blockSamples entries within a (image, tree)
+            // segment; the segments of one image form a group of consecutive blocks.  Built on the host from the
+            // segment starts for each block size the tuner asks for; the work list of a batch of ngb feature
+            // groups is then built on the device: image group i goes to XCD i % 8; each XCD takes its image
+            // groups in order, all feature groups of an image group (feature group major, then tree, then block)
+            // before the next, so the trees' workgroups of one feature group read the same planes of one image
+            // from the XCD's L2; workgroup w runs on XCD w % 8, so the lists are interleaved
+            for (auto& kv : blockSets) kv.second.level = -1;
+            auto blocksFor = [&](int blockSamples) -> BlockSet& {
+                BlockSet& bs = blockSets[blockSamples];
+                if (bs.level == levelNr) return bs;
+                bs.level = levelNr;
+                bs.workGroups = -1;
+                std::vector<BinBlock> blocks;
+                std::vector<std::pair<int, int>> segBlocks;   // per image group: first block, block count
                 int k = 0;
                 while (k < numKeys && starts[k] < 0) k++;
                 while (k < numKeys) {
@@ -410,8 +447,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     while (k2 < numKeys && starts[k2] < 0) k2++;
                     const int b = starts[k], e = starts[k2];
                     const int first = static_cast<int>(blocks.size());
-                    for (int p = b; p < e; p += kBinBlockSamples) {
-                        blocks.push_back(BinBlock{k % numActive, p, std::min(kBinBlockSamples, e - p), k / numActive});
+                    for (int p = b; p < e; p += blockSamples) {
+                        blocks.push_back(BinBlock{k % numActive, p, std::min(blockSamples, e - p), k / numActive});
                     }
                     const int image = k / numActive;
                     if (!segBlocks.empty() && blocks[segBlocks.back().first].image == image) {
@@ -421,27 +458,39 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     }
                     k = k2;
                 }
-            }
-            dBlocks.resize(blocks.size());
-            CF_HIP(hipMemcpyAsync(dBlocks.get(), blocks.data(), blocks.size() * sizeof(BinBlock), hipMemcpyHostToDevice, s));
-            // work list of a batch of ngb feature groups (built on the device): image group i goes to XCD i % 8;
-            // each XCD takes its image groups in order, all feature groups of an image group (feature group
-            // major, then tree, then block) before the next, so the trees' workgroups of one feature group read
-            // the same planes of one image from the XCD's L2; workgroup w runs on XCD w % 8, so the lists are
-            // interleaved
-            constexpr int kXcds = 8;
-            const int numSegs = static_cast<int>(segBlocks.size());
-            std::vector<int> segInfo(3 * static_cast<size_t>(numSegs) + kXcds, 0);   // first, count, XCD prefix; XCD totals
-            int* xcdBlocks = segInfo.data() + 3 * static_cast<size_t>(numSegs);
-            for (int i = 0; i < numSegs; i++) {
-                segInfo[i] = segBlocks[i].first;
-                segInfo[numSegs + i] = segBlocks[i].second;
-                segInfo[2 * static_cast<size_t>(numSegs) + i] = xcdBlocks[i % kXcds];
-                xcdBlocks[i % kXcds] += segBlocks[i].second;
-            }
-            const int maxXcdBlocks = *std::max_element(xcdBlocks, xcdBlocks + kXcds);
-            dSegInfo.resize(segInfo.size());
-            CF_HIP(hipMemcpyAsync(dSegInfo.get(), segInfo.data(), segInfo.size() * sizeof(int), hipMemcpyHostToDevice, s));
+                bs.blocks.resize(std::max<size_t>(blocks.size(), 1));
+                if (!blocks.empty()) {
+                    CF_HIP(hipMemcpyAsync(bs.blocks.get(), blocks.data(), blocks.size() * sizeof(BinBlock),
+                            hipMemcpyHostToDevice, s));
+                }
+                constexpr int kXcds = 8;
+                bs.numSegs = static_cast<int>(segBlocks.size());
+                // first, count, XCD prefix per image group; XCD totals
+                std::vector<int> segInfo(3 * static_cast<size_t>(bs.numSegs) + kXcds, 0);
+                int* xcdBlocks = segInfo.data() + 3 * static_cast<size_t>(bs.numSegs);
+                for (int i = 0; i < bs.numSegs; i++) {
+                    segInfo[i] = segBlocks[i].first;
+                    segInfo[bs.numSegs + i] = segBlocks[i].second;
+                    segInfo[2 * static_cast<size_t>(bs.numSegs) + i] = xcdBlocks[i % kXcds];
+                    xcdBlocks[i % kXcds] += segBlocks[i].second;
+                }
+                bs.maxXcdBlocks = *std::max_element(xcdBlocks, xcdBlocks + kXcds);
+                bs.segInfo.resize(segInfo.size());
+                CF_HIP(hipMemcpyAsync(bs.segInfo.get(), segInfo.data(), segInfo.size() * sizeof(int),
+                        hipMemcpyHostToDevice, s));
+                return bs;
+            };
+            auto workFor = [&](BlockSet& bs, int ngb) {
+                if (bs.workGroups == ngb) return;
+                constexpr int kXcds = 8;
+                bs.workGroups = ngb;
+                const int maxLen = bs.maxXcdBlocks * ngb;
+                bs.numWork = maxLen * kXcds;
+                bs.work.resize(std::max<size_t>(static_cast<size_t>(bs.numWork), 1));
+                const int ns = bs.numSegs;
+                launchBuildWork(s, bs.segInfo.get(), bs.segInfo.get() + ns, bs.segInfo.get() + 2 * ns,
+                        bs.segInfo.get() + 3 * ns, ns, ngb, maxLen, bs.work.get());
+            };
             dEvalOrder.resize(static_cast<size_t>(numActive) * F);
             launchFeatureEvalOrder(s, tabs, numActive, dEvalOrder.get());
             dGFeat.resize(static_cast<size_t>(numActive) * F);
@@ -469,23 +518,31 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             const int ng = static_cast<int>(std::max<size_t>(1, std::min<size_t>(groupsTotal, binsBudget / rowBytes)));
             const size_t flipOffset = anyBoth ? static_cast<size_t>(ng) * N : 0;
             dBins.resize(static_cast<size_t>(ng) * N * (anyBoth ? 2 : 1));
-            int workGroups = -1, numWork = 0;
             stats.groupBatchesMax = std::max(stats.groupBatchesMax, (groupsTotal + ng - 1) / ng);
+            const int TM = binThresholdWidth(T);
             for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
                 const int ngb = std::min(ng, groupsTotal - g0);
-                if (ngb != workGroups) {
-                    workGroups = ngb;
-                    const int maxLen = maxXcdBlocks * ngb;
-                    numWork = maxLen * kXcds;
-                    dWork.resize(static_cast<size_t>(numWork));
-                    launchBuildWork(s, dSegInfo.get(), dSegInfo.get() + numSegs, dSegInfo.get() + 2 * numSegs,
-                            dSegInfo.get() + 3 * numSegs, numSegs, ngb, maxLen, dWork.get());
-                }
+                bool measure = false;
+                const int shape = evalTunerChoose(TM, static_cast<size_t>(M), &measure);
+                const EvalShape& es = evalShape(shape);
+                BlockSet& bs = blocksFor(es.lanes * es.perLane);
+                workFor(bs, ngb);
+                if (measure) CF_HIP(hipEventRecord(tuneStart, s));
                 timer.begin();
-                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), dBlocks.get(), dWork.get(), numWork,
-                        g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get(), flipOffset);
+                launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), bs.blocks.get(), bs.work.get(),
+                        bs.numWork, g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get(), flipOffset, shape);
                 timer.end(stats.evalKernelMs);
+                if (measure) {
+                    CF_HIP(hipEventRecord(tuneEnd, s));
+                    CF_HIP(hipEventSynchronize(tuneEnd));
+                    float ms = 0;
+                    CF_HIP(hipEventElapsedTime(&ms, tuneStart, tuneEnd));
+                    // responses of the launch: every entry of the permutation for each feature of the batch
+                    const int feats = std::min(F - g0 * kBinGroup, ngb * kBinGroup);
+                    evalTunerRecord(TM, static_cast<size_t>(M), shape, ms, static_cast<double>(M) * feats);
+                }
                 stats.evalLaunches += 1;
+                stats.shapeLaunches[shape] += 1;
                 const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
                 const int maxSlotsB = static_cast<int>(std::max<size_t>(1, budget / (perSlotB * sizeof(uint32_t))));
                 stats.slotBatchesMax = std::max(stats.slotBatchesMax, (numNodes + maxSlotsB - 1) / maxSlotsB);

@@ -193,10 +193,26 @@ int cf_ctx_set_profiling(cf_ctx* ctx, int enable);
  * launches, responses, feature-generation ms, score+argmax ms, partition ms,
  * levels, sequential-replay nodes, evaluated nodes, node-histogram ms (the
  * second pass of batched levels), the most feature-group batches and the most
- * node-slot batches of one batched level.  The response kernel is the one that
+ * node-slot batches of one batched level, the pass-1 launch shape used most
+ * (cf_eval_shape_info).  The response kernel is the one that
  * gathers the integrals: k_eval_hist (direct path) or k_eval_bins (batched
  * levels).  Returns the number of entries. */
 int cf_forest_profile(const cf_forest* forest, double* out, int n);
+
+/* Launch shapes of the batched levels' response kernel and their tuner (no
+ * counterpart in the reference, whose kernels have fixed shapes,
+ * random_tree_image_gpu.cu:2034-2065, 2095-2108).  A shape: lanes per
+ * workgroup, samples per lane, features whose gathers are issued together,
+ * workgroups per CU.  Every shape gives the same histograms; the tuner times
+ * them on the first launches of each (threshold width, level size class)
+ * and keeps the fastest (CURFIL_EVAL_SHAPE=i pins shape i, CURFIL_TUNE=0
+ * keeps shape 0, CURFIL_TUNE_FILE=path keeps the choices between runs). */
+int cf_eval_shape_info(int shape, int* lanes, int* per_lane, int* features_per_step, int* workgroups_per_cu);
+/* per tuned key, 6 values: threshold width, size class, chosen shape (-1 while
+ * open), ns per response of the chosen shape and of shape 0, launches timed;
+ * returns the number of keys */
+int cf_eval_tuning(double* out, int n);
+int cf_eval_tuning_reset(void);
 /* one tree from given root samples (RandomTreeImage::doTrain,
  * random_tree_image.cpp:714-730); rs_seed seeds the tree's RandomSource. */
 int cf_tree_train_from_samples(cf_ctx* ctx, cf_images* images, const cf_train_config* config, int tree_id,

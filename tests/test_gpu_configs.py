@@ -286,3 +286,45 @@ def test_onehot_forest_known_answers_on_device(ctx, row):
         assert gl[y, x] == (1 if votes[1, k] > votes[0, k] else 0)
         ol, op = ds.predict(img, trees, [forest.posteriors(t) for t in range(3)], 2)
         assert np.array_equal(gl, ol) and _same_float_bits(gp, op)
+
+
+# ------------------------------------------------------ pass-1 launch shapes (the tuner's candidates)
+@pytest.mark.parametrize("shape", range(len(cf.eval_shapes())))
+def test_every_launch_shape_trains_like_oracle(ctx, small, shape):
+    """Each pass-1 launch shape the tuner can choose, pinned, with several feature-group and node-slot
+    batches per level and horizontal flipping: the trees equal the oracle's."""
+    im, ds = small
+    params = dict(CONFIG0, maxDepth=8, horizontalFlipping=True)
+    with _Env(CURFIL_EVAL_SHAPE=shape, CURFIL_BINS_BUDGET_KB=160, CURFIL_EVAL_BUDGET_KB=96):
+        forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=2,
+                                            num_labels=6, mode=cf.MODE_BATCHED)
+    assert forest.profile()["eval_shape"] == shape
+    print(f"shape {shape} {cf.eval_shape_info(shape)}: nodes {[len(forest.tree(t)[0]) for t in range(2)]}")
+    _compare_forest(forest, _small_oracle_flip(ds), 2)
+
+
+_ORACLE_CACHE = {}
+
+
+def _small_oracle_flip(ds):
+    if "flip8" not in _ORACLE_CACHE:
+        params = dict(CONFIG0, maxDepth=8, horizontalFlipping=True)
+        _ORACLE_CACHE["flip8"] = ds.train_forest(orc.config(**params), 2, 6)
+    return _ORACLE_CACHE["flip8"]
+
+
+def test_tuner_chooses_a_shape_and_trees_stay_exact(ctx, full_frames, full_frame_oracle):
+    """With the tuner on (and allowed to time small levels), a full-frame training times every candidate
+    shape on its first pass-1 launches and keeps one; the trees equal the oracle's."""
+    im, _, _ = full_frames
+    params, of = full_frame_oracle
+    cf.eval_tuning_reset()
+    with _Env(CURFIL_BINS_BUDGET_KB=1024, CURFIL_TUNE=1, CURFIL_TUNE_SMALL=1):
+        forest = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params), tree_count=3,
+                                            num_labels=5, mode=cf.MODE_BATCHED)
+    table = cf.eval_tuning()
+    print(f"tuning table: {table}")
+    assert table and all(r["launches_timed"] >= 2 * len(cf.eval_shapes()) for r in table)
+    assert all(r["shape"] >= 0 for r in table)
+    cf.eval_tuning_reset()
+    _compare_forest(forest, of, 3)
