@@ -377,15 +377,22 @@ def dense_prediction_leg(ctx):
     forest.predict_batch(im, list(range(n)), probabilities=True)   # forest upload + warm
     log(f"forest of 10 trees x depth 20 built in {time.time() - t0:.1f}s")
     out = {"workload": "64 frames 640x480, 10 complete trees of depth 20 (random splits), 5 classes"}
+    # outputs to host: into page-locked arrays (DMA at the link's rate), and into ordinary numpy arrays
+    pl = cf.pinned_empty(ctx, (n, H, W), np.uint8)
+    pp = cf.pinned_empty(ctx, (n, 5, H, W), np.float32)
     for probs in (False, True):
-        ts = []
-        for _ in range(5):
-            t1 = time.perf_counter()
-            forest.predict_batch(im, list(range(n)), probabilities=probs)
-            ts.append(time.perf_counter() - t1)
-        ms = float(np.median(ts)) * 1e3
-        key = "with_posteriors" if probs else "labels_only"
-        out[key] = {"batch_ms": ms, "frames_per_s": n / ms * 1e3, "mpix_per_s": n * W * H / ms / 1e3}
+        for pinned in (True, False):
+            ts = []
+            for _ in range(5):
+                t1 = time.perf_counter()
+                if pinned:
+                    forest.predict_batch(im, list(range(n)), out=(pl, pp if probs else None))
+                else:
+                    forest.predict_batch(im, list(range(n)), probabilities=probs)
+                ts.append(time.perf_counter() - t1)
+            ms = float(np.median(ts)) * 1e3
+            key = ("with_posteriors" if probs else "labels_only") + ("" if pinned else "_pageable")
+            out[key] = {"batch_ms": ms, "frames_per_s": n / ms * 1e3, "mpix_per_s": n * W * H / ms / 1e3}
     dp = cf.DeviceBuffer(ctx, n * 5 * H * W * 4)
     dl = cf.DeviceBuffer(ctx, n * H * W)
     ts = []
@@ -404,8 +411,9 @@ def dense_prediction_leg(ctx):
         lat.append((time.perf_counter() - t1) * 1e3)
     out["single_frame_p50_ms"] = float(np.percentile(lat, 50))
     out["single_frame_p99_ms"] = float(np.percentile(lat, 99))
-    out["note"] = ("labels_only / with_posteriors: outputs copied to host memory; device_outputs: posteriors and "
-                   "labels written to device memory (caller-owned), no host copy")
+    out["note"] = ("labels_only / with_posteriors: outputs copied to page-locked host arrays (cf_host_alloc); "
+                   "*_pageable: to ordinary numpy arrays; device_outputs: posteriors and labels written to "
+                   "device memory (caller-owned), no host copy")
     log(f"dense prediction (10 trees depth 20): {out['device_outputs']['frames_per_s']:.0f} frames/s on device, "
         f"{out['labels_only']['frames_per_s']:.0f} frames/s labels to host, "
         f"{out['with_posteriors']['frames_per_s']:.0f} frames/s with posteriors; single frame p50 "

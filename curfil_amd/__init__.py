@@ -89,6 +89,7 @@ def _load():
         "cf_predict": [P, P, P, I, P, P], "cf_predict_batch": [P, P, P, P, I, P, P],
         "cf_predict_batch_device": [P, P, P, P, I, P, P],
         "cf_device_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_device_free": [P, P],
+        "cf_host_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_host_free": [P, P],
         "cf_device_to_host": [P, P, P, ctypes.c_size_t],
         "cf_synth_rgbd": [U32, U32, I, I, I, I, D, P, P, P],
         "cf_images_set_rgbd_u8_padded": [P, I, P, P, I, I],
@@ -533,11 +534,20 @@ class RandomForestImage:
         self.ctx.check(lib.cf_predict(self.ctx.h, self.h, images.h, index, _ptr(probs), _ptr(labels)))
         return labels, probs
 
-    def predict_batch(self, images, indices, probabilities=True):
+    def predict_batch(self, images, indices, probabilities=True, out=None):
+        """Labels [n][H][W] and (with `probabilities`) posteriors [n][C][H][W] of frames `indices`.  `out`
+        = (labels, probs) arrays to fill, e.g. from pinned_empty (page-locked: copied at the link's
+        rate); probs may be None."""
         idx = np.ascontiguousarray(indices, np.int32)
         C, H, W = self.num_classes, images.height, images.width
-        probs = np.zeros((len(idx), C, H, W), np.float32) if probabilities else None
-        labels = np.zeros((len(idx), H, W), np.uint8)
+        if out is not None:
+            labels, probs = out
+            assert labels.dtype == np.uint8 and labels.shape == (len(idx), H, W) and labels.flags.c_contiguous
+            if probs is not None:
+                assert probs.dtype == np.float32 and probs.shape == (len(idx), C, H, W) and probs.flags.c_contiguous
+        else:
+            probs = np.empty((len(idx), C, H, W), np.float32) if probabilities else None
+            labels = np.empty((len(idx), H, W), np.uint8)
         self.bind(images.ctx)
         self.ctx.check(lib.cf_predict_batch(self.ctx.h, self.h, images.h, _ptr(idx), len(idx), _ptr(probs),
                                             _ptr(labels)))
@@ -815,6 +825,32 @@ class DeviceBuffer:
 
     def __del__(self):
         self.close()
+
+
+class _PinnedBlock:
+    """Page-locked host memory (cf_host_alloc), freed when the last array over it goes away."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        p = ctypes.c_void_p()
+        ctx.check(lib.cf_host_alloc(ctx.h, max(1, nbytes), ctypes.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ctx.h:
+            lib.cf_host_free(self.ctx.h, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(ctx, shape, dtype):
+    """An uninitialised numpy array in page-locked host memory (device copies into it run at DMA rate)."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    block = _PinnedBlock(ctx, n)
+    buf = (ctypes.c_char * max(1, n)).from_address(block.ptr)
+    buf._block = block     # the ctypes buffer keeps the memory alive; numpy keeps the buffer
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
 
 class Server:

@@ -913,6 +913,21 @@ int cf_device_alloc(cf_ctx* ctx, size_t bytes, void** ptr) {
     });
 }
 
+int cf_host_alloc(cf_ctx* ctx, size_t bytes, void** ptr) {
+    return guarded(ctx, [&] {
+        require(ptr != nullptr && bytes > 0, "invalid allocation");
+        CF_HIP(hipSetDevice(ctx->device));
+        CF_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    });
+}
+
+int cf_host_free(cf_ctx* ctx, void* ptr) {
+    return guarded(ctx, [&] {
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        if (ptr) CF_HIP(hipHostFree(ptr));
+    });
+}
+
 int cf_device_free(cf_ctx* ctx, void* ptr) {
     return guarded(ctx, [&] {
         CF_HIP(hipStreamSynchronize(ctx->stream));

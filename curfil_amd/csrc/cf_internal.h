@@ -263,6 +263,9 @@ void launchResponsesPaired(hipStream_t s, const DevImage* imgs, const cf_sample*
         double* out);
 
 // Partition
+// split record of node i from best[i] and the feature table of tree nodeTree[i]
+void launchSplitRecords(hipStream_t s, const NodeBest* best, const int* nodeTree, int numNodes, const FeatTables& tabs,
+        SplitRec* out);
 void launchPartitionDirections(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,
         int numChunks, const SplitRec* splits, int C, uint8_t* dir, int2* chunkCounts, uint32_t* leftHist);
 void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk* chunks, int numChunks,

@@ -19,6 +19,22 @@
 
 namespace cf {
 
+// Split record of every node of the level from its argmax (feature of the sorted table, threshold index).
+__global__ void __launch_bounds__(256) k_split_records(const NodeBest* __restrict__ best,
+        const int* __restrict__ nodeTree, int numNodes, const FeatRec* __restrict__ feat,
+        const float* __restrict__ thrFT, int F, int T, SplitRec* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= numNodes) return;
+    const NodeBest b = best[i];
+    const size_t tf = static_cast<size_t>(nodeTree[i]) * F + b.feat;
+    SplitRec sr;
+    sr.feat = feat[tf];
+    sr.threshold = thrFT[tf * T + b.thresh];
+    sr.flags = 0;
+    sr.pad0 = sr.pad1 = 0;
+    out[i] = sr;
+}
+
 __global__ void __launch_bounds__(256) k_part_dir(const DevImage* __restrict__ imgs,
         const cf_sample* __restrict__ samples, const Chunk* __restrict__ chunks, const SplitRec* __restrict__ splits,
         int C, uint8_t* __restrict__ dir, int2* __restrict__ chunkCounts, uint32_t* __restrict__ leftHist) {
@@ -103,6 +119,13 @@ __global__ void __launch_bounds__(256) k_part_scatter(const cf_sample* __restric
         doneR += totR;
         __syncthreads();
     }
+}
+
+void launchSplitRecords(hipStream_t s, const NodeBest* best, const int* nodeTree, int numNodes, const FeatTables& tabs,
+        SplitRec* out) {
+    if (numNodes == 0) return;
+    hipLaunchKernelGGL(k_split_records, dim3((numNodes + 255) / 256), dim3(256), 0, s, best, nodeTree, numNodes,
+            tabs.feat.get(), tabs.thrFT.get(), tabs.F, tabs.T, out);
 }
 
 void launchPartitionDirections(hipStream_t s, const DevImage* imgs, const cf_sample* samples, const Chunk* chunks,

@@ -218,6 +218,8 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<NodeBest> dBest;
     DevBuf<double> dAllScores;
     DevBuf<SplitRec> dSplits;
+    DevBuf<NodeBest> dBestAll;
+    DevBuf<int> dNodeTree;
     DevBuf<int2> dChunkCounts;
     DevBuf<uint32_t> dLeftHist;
     DevBuf<int2> dDst;
@@ -598,25 +600,18 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     levelSamples * F / ((stats.evalKernelMs - evalMsBefore) * 1e-3));
         }
 
-        // ---- split records
-        std::vector<int8_t> soa(static_cast<size_t>(numActive) * kFeatParams * F);
-        std::vector<float> thrTF(static_cast<size_t>(numActive) * T * F);
-        CF_HIP(hipMemcpyAsync(soa.data(), tabs.soa.get(), soa.size(), hipMemcpyDeviceToHost, s));
-        CF_HIP(hipMemcpyAsync(thrTF.data(), tabs.thrTF.get(), thrTF.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-        CF_HIP(hipStreamSynchronize(s));
-        std::vector<SplitRec> splits(numNodes);
-        for (int i = 0; i < numNodes; i++) {
-            const int a = nodes[i].active, f = best[i].feat, t = best[i].thresh;
-            SplitRec sr{};
-            for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = soa[(static_cast<size_t>(a) * kFeatParams + p) * F + f];
-            sr.threshold = thrTF[(static_cast<size_t>(a) * T + t) * F + f];
-            splits[i] = sr;
-        }
+        // ---- split records, built on the device from the argmax and the level's feature tables
+        std::vector<int> nodeTree(numNodes);
+        for (int i = 0; i < numNodes; i++) nodeTree[i] = nodes[i].active;
         dSplits.resize(numNodes);
+        dBestAll.resize(numNodes);
+        dNodeTree.resize(numNodes);
         dChunkCounts.resize(chunks.size());
         dLeftHist.resize(static_cast<size_t>(numNodes) * C);
         dChunks.resize(chunks.size());
-        CF_HIP(hipMemcpyAsync(dSplits.get(), splits.data(), numNodes * sizeof(SplitRec), hipMemcpyHostToDevice, s));
+        CF_HIP(hipMemcpyAsync(dBestAll.get(), best.data(), numNodes * sizeof(NodeBest), hipMemcpyHostToDevice, s));
+        CF_HIP(hipMemcpyAsync(dNodeTree.get(), nodeTree.data(), numNodes * sizeof(int), hipMemcpyHostToDevice, s));
+        launchSplitRecords(s, dBestAll.get(), dNodeTree.get(), numNodes, tabs, dSplits.get());
         CF_HIP(hipMemcpyAsync(dChunks.get(), chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, s));
         CF_HIP(hipMemsetAsync(dLeftHist.get(), 0, static_cast<size_t>(numNodes) * C * sizeof(uint32_t), s));
         timer.begin();
@@ -625,9 +620,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         timer.end(stats.partitionMs);
         std::vector<int2> chunkCounts(chunks.size());   // left and right entries per chunk
         std::vector<uint32_t> leftHist(static_cast<size_t>(numNodes) * C);
+        std::vector<SplitRec> splits(numNodes);
         CF_HIP(hipMemcpyAsync(chunkCounts.data(), dChunkCounts.get(), chunks.size() * sizeof(int2), hipMemcpyDeviceToHost,
                 s));
         CF_HIP(hipMemcpyAsync(leftHist.data(), dLeftHist.get(), leftHist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        CF_HIP(hipMemcpyAsync(splits.data(), dSplits.get(), numNodes * sizeof(SplitRec), hipMemcpyDeviceToHost, s));
         CF_HIP(hipStreamSynchronize(s));
 
         // ---- tree growth on the host, next frontier and sample destinations
@@ -713,6 +710,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             timer.end(stats.partitionMs);
             cur = 1 - cur;
         }
+        // the host vectors handed to the copies above live until here
         CF_HIP(hipStreamSynchronize(s));
         active.swap(next);
     }

@@ -281,6 +281,11 @@ int cf_predict_batch_device(cf_ctx* ctx, cf_forest* forest, cf_images* images, c
  * for cf_predict_batch_device's outputs); no counterpart in the reference */
 int cf_device_alloc(cf_ctx* ctx, size_t bytes, void** ptr);
 int cf_device_free(cf_ctx* ctx, void* ptr);
+/* page-locked host memory (no counterpart in the reference): outputs of
+ * cf_predict / cf_predict_batch written there are copied by DMA at the link's
+ * rate instead of being staged through pageable memory */
+int cf_host_alloc(cf_ctx* ctx, size_t bytes, void** ptr);
+int cf_host_free(cf_ctx* ctx, void* ptr);
 int cf_device_to_host(cf_ctx* ctx, void* host, const void* device, size_t bytes);
 
 /* ---- serving loop ------------------------------------------------------------ */

@@ -23,8 +23,16 @@
 //     50-digit decimal recomputation (tests/test_oracle_pins.py).
 //   * argmax tie rule: constructed tie / near-tie cases.
 //   * normalizeHistogram: random_tree_test.cpp:46-154 known answers.
-//   * class-uniform sampling, level loop, prediction: invariants and the
-//     reference's own end-to-end behaviour only (parity unpinned beyond them).
+//   * level loop: forced-outcome cases (maxDepth, minSampleCount, node-id
+//     rollback) whose tree does not depend on the features drawn.
+//   * prediction: one-hot forests over the known-answer responses of
+//     random_tree_image_gpu_test.cu:653-766 (votes, probabilities, labels).
+//   * class-uniform merge order with several threads: parity unpinned (the
+//     reference's own order is nondeterministic there); single-thread order.
+//
+// Node counters and prediction run over host threads (ORC_THREADS, default the
+// hardware threads, at most 16), as the reference's TBB loops do; both are
+// integer sums or independent rows, so the result does not depend on it.
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).
 
