@@ -88,6 +88,7 @@ def _load():
         "cf_forest_import": [P, ctypes.POINTER(ctypes.c_char_p), I, P, P, ctypes.POINTER(P)],
         "cf_predict": [P, P, P, I, P, P], "cf_predict_batch": [P, P, P, P, I, P, P],
         "cf_predict_batch_device": [P, P, P, P, I, P, P],
+        "cf_predict_batch_device_noalloc": [P, P, P, P, I, P, P],
         "cf_device_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_device_free": [P, P],
         "cf_host_alloc": [P, ctypes.c_size_t, ctypes.POINTER(P)], "cf_host_free": [P, P],
         "cf_device_to_host": [P, P, P, ctypes.c_size_t],
@@ -561,6 +562,15 @@ class RandomForestImage:
         self.ctx.check(lib.cf_predict_batch_device(self.ctx.h, self.h, images.h, _ptr(idx), len(idx),
                                                    ctypes.c_void_p(probs_ptr or None),
                                                    ctypes.c_void_p(labels_ptr or None)))
+
+    def predict_batch_device_noalloc(self, images, indices, probs_ptr, labels_ptr):
+        """predict_batch_device without any allocation (the forest must already be on the device: predict
+        with it once first); at most 65536 frames per call."""
+        idx = np.ascontiguousarray(indices, np.int32)
+        ctx = images.ctx
+        ctx.check(lib.cf_predict_batch_device_noalloc(ctx.h, self.h, images.h, _ptr(idx), len(idx),
+                                                      ctypes.c_void_p(probs_ptr or None),
+                                                      ctypes.c_void_p(labels_ptr or None)))
 
     def close(self):
         if self.h:

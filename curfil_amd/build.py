@@ -23,6 +23,7 @@ SOURCES = [
     "k_sort.hip",
     "k_partition.hip",
     "k_classify.hip",
+    "predict_noalloc.hip",
     "capi.cpp",
     "trainer.cpp",
     "tuner.cpp",

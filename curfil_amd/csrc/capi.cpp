@@ -195,6 +195,12 @@ int cf_ctx_create(int device, cf_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return CF_ERR_HIP;
     if (device < 0 || device >= n) return CF_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return CF_ERR_HIP;
+    // the allocation-free prediction's device array: its address on this device, resolved at set-up; a
+    // failure is kept and reported by that entry point (every other call works without the array)
+    try {
+        resolvePredictScratch(device);
+    } catch (const Error&) {
+    }
     auto* c = new cf_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

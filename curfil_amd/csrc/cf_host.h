@@ -108,6 +108,8 @@ void writePng(const std::string& path, int width, int height, int channels, int 
 void bindForest(cf_forest* fo, cf_ctx* ctx);
 // packs a normalized forest into device node records (once; later calls return at once)
 void uploadForest(cf_forest* fo);
+// resolves the allocation-free prediction's device array on `device` (current), once per device
+void resolvePredictScratch(int device);
 
 // one tree as a JSON file, gzip-compressed when `path` ends in ".gz"
 // (RandomTreeExport::writeJSON, export.cpp:180-313); palette [C][3] or null

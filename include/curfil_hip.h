@@ -277,6 +277,17 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* forest, cf_images* images, const in
 int cf_predict_batch_device(cf_ctx* ctx, cf_forest* forest, cf_images* images, const int* indices, int count,
         float* d_probs, uint8_t* d_labels);
 
+/* cf_predict_batch_device with no allocation at all, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops); it has no
+ * counterpart in the reference.  The batch's frame indices are staged in a
+ * fixed device array of the library (65536 frames at most, 256 KiB per
+ * device, held until the process ends) instead of a buffer that grows.  The
+ * forest must already be resident on the device (any earlier prediction with
+ * it uploads it), or CF_ERR_ARG.  Calls that overlap on one device (other
+ * contexts, unsynchronised threads) race on that array. */
+int cf_predict_batch_device_noalloc(cf_ctx* ctx, cf_forest* forest, cf_images* images, const int* indices,
+        int count, float* d_probs, uint8_t* d_labels);
+
 /* device memory for callers without an allocator of their own (for example
  * for cf_predict_batch_device's outputs); no counterpart in the reference */
 int cf_device_alloc(cf_ctx* ctx, size_t bytes, void** ptr);
