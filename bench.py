@@ -138,6 +138,13 @@ def main():
     nodes = [len(forest.tree(t)[0]) for t in range(trees)]
 
     roofline = eval_roofline(profiles, N, W, H, cfg_kw["thresholds"])
+    shapes = cf.eval_shapes()
+    tuning = {"shapes": [dict(zip(("lanes", "samples_per_lane", "features_per_step", "workgroups_per_cu"), s))
+                         for s in shapes],
+              "table": cf.eval_tuning(), "shape_used": int(prof["eval_shape"]),
+              "note": "the tuner times every shape on the first pass-1 launches of each (threshold width, level "
+                      "size class) key (the warm-up step here) and keeps the fastest; shape 0 is the default"}
+    log(f"tuning: {tuning['table']} (shape used {tuning['shape_used']})")
 
     full_frame = plain_vs_batched = predict = predict10 = serving = io = None
     if not args.no_extras:
@@ -172,6 +179,7 @@ def main():
                    "mode": "batched level-wise", "evaluations_per_step": evals, "nodes_per_tree": nodes,
                    "root_samples_per_tree": roots[0], "levels": int(prof["levels"])},
         "roofline": roofline,
+        "tuning": tuning,
         "cpu_baseline": cpu,
         "phase_ms": {k: round(prof[k], 1) for k in ("featgen_ms", "eval_kernel_ms", "hist_ms", "score_ms",
                                                     "partition_ms")},
@@ -234,10 +242,11 @@ def measured_traffic(kernel, N, W, H, T):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    w = t.get("workload", {})
-    if (w.get("frames"), w.get("width"), w.get("height"), w.get("thresholds")) != (N, W, H, T):
-        return None
-    return t.get("bytes_per_launch")
+    for e in t.get("entries", [t]):
+        w = e.get("workload", {})
+        if (w.get("frames"), w.get("width"), w.get("height"), w.get("thresholds")) == (N, W, H, T):
+            return e.get("bytes_per_launch")
+    return None
 
 
 IO_PALETTE = [(0, 0, 0), (128, 0, 0), (0, 128, 0), (128, 128, 0), (0, 0, 128)]

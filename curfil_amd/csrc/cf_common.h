@@ -106,6 +106,17 @@ __device__ __forceinline__ double boxColor(const float* __restrict__ c, int W, i
 // averageRegionDepth (random_tree_image.h:268-316).  Sums use wrapping
 // 32-bit arithmetic: the box sum is exact whenever it fits in int32, even if
 // single integral entries wrapped (they are stored modulo 2^32).
+// (double)sum / 1000.0, correctly rounded, without a division: the product with
+// RN(1/1000) corrected once by the exact residual fma(-q, 1000, a).  Checked
+// against the division for every 32-bit integer sum (tests/test_numerics.py);
+// box sums are 32-bit integers.
+__host__ __device__ __forceinline__ double depthSumToMetres(int sum) {
+    constexpr double kInv1000 = 1.0 / 1000.0;
+    const double a = static_cast<double>(sum);
+    const double q = a * kInv1000;
+    return fma(fma(-q, 1000.0, a), kInv1000, q);
+}
+
 __device__ __forceinline__ double boxDepth(const int2* __restrict__ dep, int W, int H, int px, int py, int ox, int oy,
         int rx, int ry) {
     const int w = rx > 1 ? rx : 1;
@@ -122,8 +133,7 @@ __device__ __forceinline__ double boxDepth(const int2* __restrict__ dep, int W, 
     if (numValid == 0) return cf_nan();
     const int sum = static_cast<int>((static_cast<uint32_t>(lr.x) - static_cast<uint32_t>(ur.x)) +
             (static_cast<uint32_t>(ul.x) - static_cast<uint32_t>(ll.x)));
-    const double feat = static_cast<double>(sum) / 1000.0;
-    return feat / static_cast<double>(numValid);
+    return depthSumToMetres(sum) / static_cast<double>(numValid);
 }
 
 // calculateColorFeature / calculateDepthFeature (random_tree_image.h:564-623)
@@ -210,7 +220,7 @@ __device__ __forceinline__ double depthBoxFromCorners(const int2* d) {
     if (numValid == 0) return cf_nan();
     const int sum = static_cast<int>((static_cast<uint32_t>(d[0].x) - static_cast<uint32_t>(d[2].x)) +
             (static_cast<uint32_t>(d[3].x) - static_cast<uint32_t>(d[1].x)));
-    return (static_cast<double>(sum) / 1000.0) / static_cast<double>(numValid);
+    return depthSumToMetres(sum) / static_cast<double>(numValid);
 }
 
 // averageRegionDepth of both boxes and their difference (NaN if a box has no

@@ -201,8 +201,8 @@ void launchLevelKeys(hipStream_t s, const cf_sample* samples, int begin, int n, 
         uint32_t* keys, uint32_t* vals, int flipN, uint32_t absentKey, int yBits);
 // label per level index, bit 7 set for samples still set to Both (counted in both orientations)
 void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* labels);
-// Stable counting sort of n level entries by key (< levelSortBuckets(numSegs, yBits)): out = the values
-// in key order, starts[seg] = first position of segment seg (key >> yBits) or -1 if it is empty, and
+// Counting sort of n level entries by key (< levelSortBuckets(numSegs, yBits)): out = the values in key
+// order (input order inside every segment bucket of up to 64 entries), starts[seg] = first position of segment seg (key >> yBits) or -1 if it is empty, and
 // starts[numSegs] = the number of entries with a key below numSegs << yBits.  count and cursor hold
 // levelSortBuckets entries, blockSum levelSortScanBlocks.
 size_t levelSortBuckets(int numSegs, int yBits);

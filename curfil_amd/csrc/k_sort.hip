@@ -1,4 +1,4 @@
-// Image-major order of a batched level: a stable counting sort of the level
+// Image-major order of a batched level: a counting sort of the level
 // entries by their key ((tree, image) segment << yBits | row).
 //
 // The keys are small integers (segments x rows: 4000 x 480 buckets for 5
@@ -9,15 +9,20 @@
 //                   scan of the block totals, the block offsets added);
 //   k_sort_scatter  places every entry at its bucket's cursor (atomic, so
 //                   the order inside a bucket follows arrival);
-//   k_sort_settle   restores the input order inside every bucket by
-//                   sorting its values (the entries' level indices, with
-//                   kFlipEntry on the flipped evaluations that follow all
-//                   plain entries in the input): the result is exactly the
-//                   stable sort of the entries by key;
+//   k_sort_settle   restores the input order inside every segment bucket of
+//                   at most kSettleMax entries by sorting its values (the
+//                   entries' level indices, with kFlipEntry on the flipped
+//                   evaluations that follow all plain entries in the input),
+//                   one lane per bucket;
 //   k_sort_starts   the first position of every (tree, image) segment.
 // Buckets hold a few entries on average (one sample per 77 pixels per tree
-// at the benchmark's sampling density); a bucket holds at most the entries
-// of one row of one segment.
+// at the benchmark's sampling density; a bucket holds at most the entries of
+// one row of one segment), so the result is the stable sort by key.  Longer
+// buckets, and the bucket past the segments that collects the flipped
+// entries of samples that are not Both (up to one per sample, never
+// evaluated), keep their arrival order: the order inside a bucket only
+// decides which lanes of pass 1 evaluate which samples of one row, never a
+// result, since every bin row is stored at its entry's level index.
 #include "cf_internal.h"
 
 namespace cf {
@@ -25,6 +30,7 @@ namespace cf {
 namespace {
 
 constexpr int kScanBlock = 1024;   // counts scanned per workgroup (256 lanes x 4)
+constexpr uint32_t kSettleMax = 64;   // longest bucket put back into input order
 
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -106,13 +112,13 @@ __global__ void __launch_bounds__(256) k_sort_scatter(const uint32_t* __restrict
     if (i < n) out[atomicAdd(&cursor[keys[i]], 1u)] = vals[i];
 }
 
-// One lane per bucket: insertion sort of its values (the buckets are short).
+// One lane per segment bucket of 2..kSettleMax entries: insertion sort of its values.
 __global__ void __launch_bounds__(256) k_sort_settle(const uint32_t* __restrict__ count,
         const uint32_t* __restrict__ cursor, size_t buckets, uint32_t* __restrict__ out) {
     const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= buckets) return;
     const uint32_t c = count[b];
-    if (c < 2) return;
+    if (c < 2 || c > kSettleMax) return;
     const uint32_t e = cursor[b], s = e - c;
     for (uint32_t i = s + 1; i < e; i++) {
         const uint32_t v = out[i];
@@ -164,7 +170,9 @@ void levelCountingSort(hipStream_t s, const uint32_t* keys, const uint32_t* vals
     hipLaunchKernelGGL(k_scan_add, dim3(blocksFor(buckets, 256)), dim3(256), 0, s, cursor, buckets, blockSum);
     if (n > 0) {
         hipLaunchKernelGGL(k_sort_scatter, dim3(blocksFor(n, 256)), dim3(256), 0, s, keys, vals, n, cursor, out);
-        hipLaunchKernelGGL(k_sort_settle, dim3(blocksFor(buckets, 256)), dim3(256), 0, s, count, cursor, buckets,
+        // the segments' buckets only: the bucket past them holds the flipped entries of samples that are not Both
+        const size_t segBuckets = static_cast<size_t>(numSegs) << yBits;
+        hipLaunchKernelGGL(k_sort_settle, dim3(blocksFor(segBuckets, 256)), dim3(256), 0, s, count, cursor, segBuckets,
                 out);
     }
     hipLaunchKernelGGL(k_sort_starts, dim3(blocksFor(static_cast<size_t>(numSegs) + 1, 256)), dim3(256), 0, s,

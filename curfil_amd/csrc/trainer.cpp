@@ -392,7 +392,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         auto evalTwoPass = [&]() {
             int N = 0;
             for (int a = 0; a < numActive; a++) N = std::max(N, active[a].begin + active[a].n);
-            // image-major order of the level: (tree, image) keys, stable counting sort of level indices.
+            // image-major order of the level: (tree, image, row) keys, counting sort of level indices.
             // With Both samples a second entry per level index evaluates it flipped; the entries of samples
             // that are not Both get the key one past the last (tree, image) and sort to the end.
             const int M = anyBoth ? 2 * N : N;

@@ -87,3 +87,15 @@ def test_branch_free_next_float_up():
     nxt = np.where(f == 0, 1, np.where(u & 0x80000000, u - 1, u + 1)).astype(np.uint32).view(np.float32)
     ref = np.nextafter(f, np.float32(np.inf))
     assert np.array_equal(nxt.view(np.uint32), ref.view(np.uint32))
+
+
+def test_division_by_1000_as_corrected_product_is_exact_for_every_int32(tmp_path):
+    # averageRegionDepth's (double)sum / 1000.0 (random_tree_image.h:288-316): the kernels compute it as
+    # the product with RN(1/1000) plus one fma-corrected step; every 32-bit sum gives the same double
+    import subprocess
+    import os
+    src = os.path.join(os.path.dirname(__file__), "native", "check_div1000.c")
+    exe = str(tmp_path / "check_div1000")
+    subprocess.run(["gcc", "-O2", "-march=native", "-ffp-contract=off", "-o", exe, src, "-lm"], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.strip()
+    assert out == "0"
