@@ -881,8 +881,11 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(),
                 static_cast<int>(fo->trees.size()), fo->C};
+        // without posteriors to return, the tree-major passes keep their running sums in forest staging
+        if (!probs) fo->dAcc.resize(static_cast<size_t>(count) * fo->C * plane);
         launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df,
-                probs ? fo->dProbs.get() : nullptr, labels ? fo->dLabels.get() : nullptr);
+                probs ? fo->dProbs.get() : nullptr, labels ? fo->dLabels.get() : nullptr,
+                probs ? nullptr : fo->dAcc.get());
         if (probs) {
             CF_HIP(hipMemcpyAsync(probs, fo->dProbs.get(), static_cast<size_t>(count) * fo->C * plane * sizeof(float),
                     hipMemcpyDeviceToHost, ctx->stream));
@@ -904,9 +907,11 @@ int cf_predict_batch_device(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int
         bindForest(fo, ctx);
         uploadForest(fo);
         fo->dFrames.resize(count);
+        if (!d_probs) fo->dAcc.resize(static_cast<size_t>(count) * fo->C * im->W * im->H);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()), fo->C};
-        launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df, d_probs, d_labels);
+        launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df, d_probs, d_labels,
+                d_probs ? nullptr : fo->dAcc.get());
         CF_HIP(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -279,8 +279,10 @@ struct DevForest {
     int numTrees;
     int C;
 };
+// probs [frames][C][H][W] and labels [frames][H][W] may be null; with probs or acc (scratch of the same
+// shape as probs) the trees run tree-major over groups of frames, the running sums kept there
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
-        const DevForest& forest, float* probs, uint8_t* labels);
+        const DevForest& forest, float* probs, uint8_t* labels, float* acc = nullptr);
 // one frame of a serving batch: its integrals and its outputs ([C][H][W] posteriors, [H][W] labels)
 struct FrameJob {
     DevImage im;

@@ -6,13 +6,18 @@
 //   * per tree, the leaf's double posterior is added into a float
 //     accumulator; the double sum normalises; the label is the first class
 //     with the largest probability (strict >, starting from -1).
-// One thread per pixel; the frames of a batch are the grid's y dimension, so
-// one launch classifies a whole batch against all trees.
+// One thread per pixel; the frames of a batch are the grid's y dimension.
+// With an accumulator (the posterior output, or a scratch buffer) a batch is
+// classified tree-major: frames in groups, each group one pass per tree, so
+// the passes running at a time share one tree's nodes and leaves (530 MB for
+// ten complete trees of depth 20 do not fit the last-level cache, one tree
+// does); without one, every tree of a pixel runs in one thread.
+#include <algorithm>
+#include <cstdlib>
+
 #include "cf_internal.h"
 
 namespace cf {
-
-constexpr int kLockstep = 1;   // trees traversed together by one thread (1 measured fastest: 2 and 4 raise register use)
 
 // One node record: the feature (16 bytes) and {threshold, child, -, -}.
 struct NodeRec {
@@ -41,19 +46,10 @@ __device__ __forceinline__ NodeRec loadNode(const SplitRec* nodes, int i) {
     return NodeRec{p[0], p[1]};
 }
 
-// One thread per pixel; the frames of a batch are the grid's y dimension.
-// Trees are walked kLockstep at a time: each step loads the current node of
-// every unfinished tree, issues all their corner gathers, then forms the
-// responses, so the dependent load chains of the trees overlap.  Leaf
-// posteriors are then added in tree order, as the reference does.
-template<int CM>
-__device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const DevForest& forest,
-        float* __restrict__ pout, uint8_t* __restrict__ lout) {
-    const int W = im.W, H = im.H;
-    const int x = px % W, y = px / W;
-    const int C = forest.C;
-    const size_t plane = static_cast<size_t>(W) * H;
-    // PixelInstance depth (random_tree_image.h:140-165)
+// PixelInstance depth (random_tree_image.h:140-165) by inclusion-exclusion on the integral: true and
+// the depth in mm when the pixel's depth is valid.
+__device__ __forceinline__ bool pixelDepthAt(const DevImage& im, int x, int y, int32_t& depth) {
+    const int W = im.W;
     auto at = [&](int xx, int yy) -> int2 { return im.dep[yy * W + xx]; };
     const int2 c00 = at(x, y);
     const int2 cl = x > 0 ? at(x - 1, y) : make_int2(0, 0);
@@ -61,93 +57,64 @@ __device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const 
     const int2 cal = (x > 0 && y > 0) ? at(x - 1, y - 1) : make_int2(0, 0);
     const int valid = static_cast<int>(static_cast<uint32_t>(c00.y) -
             (static_cast<uint32_t>(cl.y) + static_cast<uint32_t>(ca.y) - static_cast<uint32_t>(cal.y)));
-    int32_t depth = -1;
+    depth = -1;
     if (valid == 1) {
         depth = static_cast<int32_t>(static_cast<uint32_t>(c00.x) -
                 (static_cast<uint32_t>(cl.x) + static_cast<uint32_t>(ca.x) - static_cast<uint32_t>(cal.x)));
     }
-    const bool depthOk = depth > 0;
-    const double rinv = depthOk ? depthReciprocal(depthToFloat(depth)) : 0.0;
-    float acc[CM];
+    return depth > 0;
+}
+
+// The leaf index one tree sends the pixel to.  Per split the 8 corner gathers are issued before the
+// response is formed; SplitFunction::split: left iff r <= threshold, NaN (invalid depth, a box outside
+// the image, a depth box without valid pixels) goes right.
+__device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool depthOk, double rinv,
+        const DevForest& forest, int root) {
+    const int W = im.W, H = im.H;
+    const size_t plane = static_cast<size_t>(W) * H;
+    NodeRec rec = loadNode(forest.nodes, root);
+    while (static_cast<int>(rec.tail.y) >= 0) {
+        const Feat f = unpackFeat(rec.feat);
+        int idx[8];
+        const bool isDepth = f.type == T_DEPTH;
+        const bool ok = depthOk && responseCorners(f, x, y, W, H, rinv, false, idx);
+        int2 cv[8];   // depth corners, or colour corners' bits in .x
+        if (ok) {
+            if (isDepth) {
 #pragma unroll
-    for (int c = 0; c < CM; c++) acc[c] = 0.0f;
-    for (int t0 = 0; t0 < forest.numTrees; t0 += kLockstep) {
-        NodeRec rec[kLockstep];
-        int leaf[kLockstep];
-        bool open[kLockstep];
+                for (int c = 0; c < 8; c++) cv[c] = planeAt(im.dep, idx[c]);
+            } else {
+                const float* colA = im.col + static_cast<size_t>(f.c1) * plane;
+                const float* colB = im.col + static_cast<size_t>(f.c2) * plane;
 #pragma unroll
-        for (int k = 0; k < kLockstep; k++) {
-            open[k] = t0 + k < forest.numTrees;
-            leaf[k] = 0;
-            if (open[k]) rec[k] = loadNode(forest.nodes, forest.treeRoot[t0 + k]);
-        }
-        for (;;) {
-            bool any = false;
+                for (int c = 0; c < 4; c++) cv[c].x = __float_as_int(planeAt(colA, idx[c]));
 #pragma unroll
-            for (int k = 0; k < kLockstep; k++) {
-                if (open[k] && static_cast<int>(rec[k].tail.y) < 0) {
-                    leaf[k] = -static_cast<int>(rec[k].tail.y) - 1;
-                    open[k] = false;
-                }
-                any |= open[k];
-            }
-            if (!any) break;
-            bool ok[kLockstep], isDepth[kLockstep];
-            int2 cv[kLockstep][8];   // depth corners, or colour corners' bits in .x
-#pragma unroll
-            for (int k = 0; k < kLockstep; k++) {
-                ok[k] = false;
-                isDepth[k] = false;
-                if (open[k]) {
-                    const Feat f = unpackFeat(rec[k].feat);
-                    int idx[8];
-                    isDepth[k] = f.type == T_DEPTH;
-                    ok[k] = depthOk && responseCorners(f, x, y, W, H, rinv, false, idx);
-                    if (ok[k]) {
-                        if (isDepth[k]) {
-#pragma unroll
-                            for (int c = 0; c < 8; c++) cv[k][c] = planeAt(im.dep, idx[c]);
-                        } else {
-                            const float* colA = im.col + static_cast<size_t>(f.c1) * plane;
-                            const float* colB = im.col + static_cast<size_t>(f.c2) * plane;
-#pragma unroll
-                            for (int c = 0; c < 4; c++) cv[k][c].x = __float_as_int(planeAt(colA, idx[c]));
-#pragma unroll
-                            for (int c = 4; c < 8; c++) cv[k][c].x = __float_as_int(planeAt(colB, idx[c]));
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kLockstep; k++) {
-                if (!open[k]) continue;
-                double r = cf_nan();
-                if (ok[k]) {
-                    if (isDepth[k]) {
-                        r = depthResponseFromCorners(cv[k]);
-                    } else {
-                        float c8[8];
-#pragma unroll
-                        for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[k][c].x);
-                        r = colorResponseFromCorners(c8);
-                    }
-                }
-                // SplitFunction::split: left iff r <= threshold; NaN goes right
-                const int child = static_cast<int>(rec[k].tail.y);
-                const float thr = __uint_as_float(rec[k].tail.x);
-                rec[k] = loadNode(forest.nodes, (r <= static_cast<double>(thr)) ? child : child + 1);
+                for (int c = 4; c < 8; c++) cv[c].x = __float_as_int(planeAt(colB, idx[c]));
             }
         }
+        double r = cf_nan();
+        if (ok) {
+            if (isDepth) {
+                r = depthResponseFromCorners(cv);
+            } else {
+                float c8[8];
 #pragma unroll
-        for (int k = 0; k < kLockstep; k++) {
-            if (t0 + k >= forest.numTrees) break;
-            const double* post = forest.post + static_cast<size_t>(leaf[k]) * C;
-#pragma unroll
-            for (int c = 0; c < CM; c++) {
-                if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + post[c]);
+                for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[c].x);
+                r = colorResponseFromCorners(c8);
             }
         }
+        const int child = static_cast<int>(rec.tail.y);
+        const float thr = __uint_as_float(rec.tail.x);
+        rec = loadNode(forest.nodes, (r <= static_cast<double>(thr)) ? child : child + 1);
     }
+    return -static_cast<int>(rec.tail.y) - 1;
+}
+
+// The double sum of the float accumulator normalises it; the label is the first class with the
+// largest probability (strict >, from -1).
+template<int CM>
+__device__ __forceinline__ void finishPixel(const float (&acc)[CM], int C, size_t plane, int px,
+        float* __restrict__ pout, uint8_t* __restrict__ lout) {
     double sum = 0.0;
 #pragma unroll
     for (int c = 0; c < CM; c++) {
@@ -167,6 +134,32 @@ __device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const 
         }
     }
     if (lout) lout[px] = static_cast<uint8_t>(label);
+}
+
+// One thread per pixel, the trees one after another (walking several trees in lockstep per thread
+// was measured slower): per tree, the leaf's double posterior is added into the float accumulator.
+template<int CM>
+__device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const DevForest& forest,
+        float* __restrict__ pout, uint8_t* __restrict__ lout) {
+    const int W = im.W, H = im.H;
+    const int x = px % W, y = px / W;
+    const int C = forest.C;
+    const size_t plane = static_cast<size_t>(W) * H;
+    int32_t depth;
+    const bool depthOk = pixelDepthAt(im, x, y, depth);
+    const double rinv = depthOk ? depthReciprocal(depthToFloat(depth)) : 0.0;
+    float acc[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) acc[c] = 0.0f;
+    for (int t = 0; t < forest.numTrees; t++) {
+        const int leaf = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
+        const double* post = forest.post + static_cast<size_t>(leaf) * C;
+#pragma unroll
+        for (int c = 0; c < CM; c++) {
+            if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + post[c]);
+        }
+    }
+    finishPixel<CM>(acc, C, plane, px, pout, lout);
 }
 
 // frames of one image set: frame f of the batch is image frameIdx[f]; outputs stacked per frame
@@ -192,27 +185,136 @@ __global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restric
     classifyPixel<CM>(job.im, px, forest, job.probs, job.labels);
 }
 
+// Trees [t0, t1) for one pixel, added into its float accumulator acc[c * plane] (zero when t0 == 0),
+// which keeps the running sums between passes; after the last tree the accumulator becomes the
+// normalised posteriors (probs may be the accumulator itself) and the label is written.  The sums are
+// the single pass's, tree by tree in tree order.
+template<int CM>
+__device__ __forceinline__ void classifyPixelTrees(const DevImage& im, int px, const DevForest& forest, int t0, int t1,
+        float* __restrict__ acc, float* __restrict__ pout, uint8_t* __restrict__ lout) {
+    const int W = im.W, H = im.H;
+    const int x = px % W, y = px / W;
+    const int C = forest.C;
+    const size_t plane = static_cast<size_t>(W) * H;
+    int32_t depth;
+    const bool depthOk = pixelDepthAt(im, x, y, depth);
+    const double rinv = depthOk ? depthReciprocal(depthToFloat(depth)) : 0.0;
+    float a[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) a[c] = (t0 > 0 && c < C) ? acc[c * plane + px] : 0.0f;
+    for (int t = t0; t < t1; t++) {
+        const int leaf = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
+        const double* post = forest.post + static_cast<size_t>(leaf) * C;
+#pragma unroll
+        for (int c = 0; c < CM; c++) {
+            if (c < C) a[c] = static_cast<float>(static_cast<double>(a[c]) + post[c]);
+        }
+    }
+    if (t1 < forest.numTrees) {
+#pragma unroll
+        for (int c = 0; c < CM; c++) {
+            if (c < C) acc[c * plane + px] = a[c];
+        }
+        return;
+    }
+    finishPixel<CM>(a, C, plane, px, pout, lout);
+}
+
+template<int CM>
+__global__ void __launch_bounds__(256) k_classify_pass(const DevImage* __restrict__ imgs,
+        const int* __restrict__ frameIdx, int f0, int W, int H, DevForest forest, int t0, int t1,
+        float* __restrict__ acc, float* __restrict__ probs, uint8_t* __restrict__ labels) {
+    const int frame = f0 + blockIdx.y;
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= W * H) return;
+    const size_t plane = static_cast<size_t>(W) * H;
+    classifyPixelTrees<CM>(imgs[frameIdx[frame]], px, forest, t0, t1,
+            acc + static_cast<size_t>(frame) * forest.C * plane,
+            probs ? probs + static_cast<size_t>(frame) * forest.C * plane : nullptr,
+            labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
+}
+
+template<int CM>
+__global__ void __launch_bounds__(256) k_classify_jobs_pass(const FrameJob* __restrict__ jobs, int j0, int W, int H,
+        DevForest forest, int t0, int t1) {
+    const FrameJob job = jobs[j0 + blockIdx.y];
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= W * H) return;
+    classifyPixelTrees<CM>(job.im, px, forest, t0, t1, job.probs, job.probs, job.labels);
+}
+
+namespace {
+
+// Passes of the tree-major schedule: frames in groups whose integral images take about
+// kGroupBytes (they stay in the last-level cache while every tree runs over them), trees in passes of
+// kPassTrees (one tree's nodes and leaves, 53 MB for a complete depth-20 tree, are what the passes
+// share).  CURFIL_CLASSIFY_GROUP_MB and CURFIL_CLASSIFY_PASS_TREES override them.
+constexpr double kGroupBytes = 96.0 * (1 << 20);
+constexpr int kPassTrees = 1;
+
+int envInt(const char* name, int def) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::max(1, std::atoi(v)) : def;
+}
+
+int groupFrames(int W, int H, int numFrames) {
+    const double perFrame = static_cast<double>(W) * H * (3 * sizeof(float) + sizeof(int2));
+    const double bytes = envInt("CURFIL_CLASSIFY_GROUP_MB", 0) > 0 ? envInt("CURFIL_CLASSIFY_GROUP_MB", 0) * 1048576.0
+                                                                    : kGroupBytes;
+    return std::max(1, std::min(numFrames, static_cast<int>(bytes / perFrame)));
+}
+
+}  // namespace
+
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
-        const DevForest& forest, float* probs, uint8_t* labels) {
-    const dim3 grid((W * H + 255) / 256, numFrames);
-    if (forest.C <= 8) {
-        hipLaunchKernelGGL(k_classify<8>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
-    } else if (forest.C <= kMaxClasses) {
-        hipLaunchKernelGGL(k_classify<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
-    } else {
-        throw Error(CF_ERR_ARG, "too many classes (max 32)");
+        const DevForest& forest, float* probs, uint8_t* labels, float* acc) {
+    if (forest.C > kMaxClasses) throw Error(CF_ERR_ARG, "too many classes (max 32)");
+    float* accum = probs ? probs : acc;
+    if (!accum || forest.numTrees <= 1) {
+        // every tree of a pixel in one thread: no accumulator between passes
+        const dim3 grid((W * H + 255) / 256, numFrames);
+        if (forest.C <= 8) {
+            hipLaunchKernelGGL(k_classify<8>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
+        } else {
+            hipLaunchKernelGGL(k_classify<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs,
+                    labels);
+        }
+        return;
+    }
+    const int fg = groupFrames(W, H, numFrames);
+    const int pt = envInt("CURFIL_CLASSIFY_PASS_TREES", kPassTrees);
+    for (int f0 = 0; f0 < numFrames; f0 += fg) {
+        const dim3 grid((W * H + 255) / 256, std::min(fg, numFrames - f0));
+        for (int t0 = 0; t0 < forest.numTrees; t0 += pt) {
+            const int t1 = std::min(forest.numTrees, t0 + pt);
+            if (forest.C <= 8) {
+                hipLaunchKernelGGL(k_classify_pass<8>, grid, dim3(256), 0, s, imgs, frameIdx, f0, W, H, forest, t0, t1,
+                        accum, probs, labels);
+            } else {
+                hipLaunchKernelGGL(k_classify_pass<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, f0, W, H, forest,
+                        t0, t1, accum, probs, labels);
+            }
+        }
     }
 }
 
 void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int W, int H, const DevForest& forest) {
     if (numJobs == 0) return;
-    const dim3 grid((W * H + 255) / 256, numJobs);
-    if (forest.C <= 8) {
-        hipLaunchKernelGGL(k_classify_jobs<8>, grid, dim3(256), 0, s, jobs, W, H, forest);
-    } else if (forest.C <= kMaxClasses) {
-        hipLaunchKernelGGL(k_classify_jobs<kMaxClasses>, grid, dim3(256), 0, s, jobs, W, H, forest);
-    } else {
-        throw Error(CF_ERR_ARG, "too many classes (max 32)");
+    if (forest.C > kMaxClasses) throw Error(CF_ERR_ARG, "too many classes (max 32)");
+    // the jobs' posterior buffers are the accumulators (every serving slot has one)
+    const int fg = groupFrames(W, H, numJobs);
+    const int pt = envInt("CURFIL_CLASSIFY_PASS_TREES", kPassTrees);
+    for (int j0 = 0; j0 < numJobs; j0 += fg) {
+        const dim3 grid((W * H + 255) / 256, std::min(fg, numJobs - j0));
+        for (int t0 = 0; t0 < forest.numTrees; t0 += pt) {
+            const int t1 = std::min(forest.numTrees, t0 + pt);
+            if (forest.C <= 8) {
+                hipLaunchKernelGGL(k_classify_jobs_pass<8>, grid, dim3(256), 0, s, jobs, j0, W, H, forest, t0, t1);
+            } else {
+                hipLaunchKernelGGL(k_classify_jobs_pass<kMaxClasses>, grid, dim3(256), 0, s, jobs, j0, W, H, forest, t0,
+                        t1);
+            }
+        }
     }
 }
 
