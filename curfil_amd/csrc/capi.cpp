@@ -448,7 +448,7 @@ int cf_generate_features(cf_ctx* ctx, cf_images* im, const cf_sample* samples, i
         DevBuf<FeatGenJob> dJ(1);
         CF_HIP(hipMemcpyAsync(dJ.get(), &j, sizeof(j), hipMemcpyHostToDevice, ctx->stream));
         FeatTables tabs;
-        launchFeatureGeneration(ctx->stream, im->desc.get(), dJ.get(), 1, F, T, boxRadius, regionSize, useDepth, &seed, tabs);
+        launchFeatureGeneration(ctx->stream, im->set(), dJ.get(), 1, F, T, boxRadius, regionSize, useDepth, &seed, tabs);
         CF_HIP(hipMemcpyAsync(featuresOut, tabs.soa.get(), static_cast<size_t>(F) * kFeatParams, hipMemcpyDeviceToHost,
                 ctx->stream));
         CF_HIP(hipMemcpyAsync(thresholdsOut, tabs.thrTF.get(), static_cast<size_t>(F) * T * sizeof(float),

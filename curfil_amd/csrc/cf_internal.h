@@ -144,7 +144,7 @@ struct FeatTables {
     int trees = 0, F = 0, T = 0;
 };
 
-void launchFeatureGeneration(hipStream_t s, const DevImage* imgs, const FeatGenJob* jobsDev, int numJobs,
+void launchFeatureGeneration(hipStream_t s, const ImageSet& imgs, const FeatGenJob* jobsDev, int numJobs,
         int F, int T, int boxRadius, int regionSize, int useDepth, const int* seedsHost, FeatTables& tabs);
 // thresholds given in reference layout [trees][T][F] + features SoA [trees][11][F] -> tables
 void launchPrepareTables(hipStream_t s, int trees, int F, int T, FeatTables& tabs);

@@ -196,7 +196,7 @@ __device__ int refillDraws(WalkShared& sh, int tail, const UniformIntParams& up)
 // candidate carries its state into the next round, which starts with it.
 // The bands only decide how far a round gets, never which draws a candidate
 // uses.
-__global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ jobs, const DevImage* __restrict__ imgs,
+__global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ jobs, ImageSet imgs,
         const FeatRec* __restrict__ cand, int numCand, int F, int T, FeatRec* __restrict__ outFeat,
         float* __restrict__ outThr, int* __restrict__ outKey, int* __restrict__ err) {
     __shared__ WalkShared sh;
@@ -238,9 +238,27 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
         if (tid < kWalkCand && k0 + tid < numCand) sh.cand[tid] = myCand[k0 + tid];
         __syncthreads();
         {
-            // all slots' corner gathers are issued before any response is formed
+            // two rounds of loads for all slots together: first every slot's sample record, then every
+            // slot's corner gathers (planes addressed from the image index, no descriptor load); a
+            // response is formed only after all of them are in flight
             bool ok[kSlots], isDepth[kSlots], active[kSlots];
             int2 cv[kSlots][8];   // depth corners, or colour corners' bits in .x
+            cf_sample smp[kSlots];
+#pragma unroll
+            for (int q = 0; q < kPerWave; q++) {
+                const int c = wave + 4 * q;
+                const bool candOk = k0 + c < numCand;
+#pragma unroll
+                for (int r = 0; r < kReps; r++) {
+                    const int i = q * kReps + r;
+                    const int dpos = c * S + 64 * r + lane;
+                    active[i] = candOk && dpos < kWalkDraws;
+                    // inactive slots read the level's first sample (in range) and issue no gathers
+                    const int d = active[i] ? sh.dq[(head + dpos) % kDrawQueue] : 0;
+                    smp[i] = job.samples[d];
+                }
+            }
+            const size_t plane = static_cast<size_t>(imgs.W) * imgs.H;
 #pragma unroll
             for (int q = 0; q < kPerWave; q++) {
                 const int c = wave + 4 * q;
@@ -250,31 +268,25 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
 #pragma unroll
                 for (int r = 0; r < kReps; r++) {
                     const int i = q * kReps + r;
-                    const int dpos = c * S + 64 * r + lane;
-                    active[i] = candOk && dpos < kWalkDraws;
-                    ok[i] = false;
                     isDepth[i] = ff.type == T_DEPTH;
-                    if (active[i]) {
-                        const int d = sh.dq[(head + dpos) % kDrawQueue];
-                        const cf_sample smp = job.samples[d];
-                        const DevImage im = imgs[smp.image];
-                        const bool valid = smp.depth_mm > 0;
-                        const double rinv = valid ? depthReciprocal(depthToFloat(smp.depth_mm)) : 0.0;
-                        int idx[8];
-                        ok[i] = valid && responseCorners(ff, smp.x, smp.y, im.W, im.H, rinv, false, idx);
-                        if (ok[i]) {
-                            if (isDepth[i]) {
+                    const bool valid = smp[i].depth_mm > 0;
+                    const double rinv = valid ? depthReciprocal(depthToFloat(smp[i].depth_mm)) : 0.0;
+                    int idx[8];
+                    ok[i] = active[i] && valid && responseCorners(ff, smp[i].x, smp[i].y, imgs.W, imgs.H, rinv, false,
+                            idx);
+                    if (ok[i]) {
+                        if (isDepth[i]) {
+                            const int2* dep = imgs.dep + static_cast<size_t>(smp[i].image) * plane;
 #pragma unroll
-                                for (int e = 0; e < 8; e++) cv[i][e] = planeAt(im.dep, idx[e]);
-                            } else {
-                                const size_t plane = static_cast<size_t>(im.W) * im.H;
-                                const float* colA = im.col + static_cast<size_t>(ff.c1) * plane;
-                                const float* colB = im.col + static_cast<size_t>(ff.c2) * plane;
+                            for (int e = 0; e < 8; e++) cv[i][e] = planeAt(dep, idx[e]);
+                        } else {
+                            const float* col = imgs.col + static_cast<size_t>(smp[i].image) * 3 * plane;
+                            const float* colA = col + static_cast<size_t>(ff.c1) * plane;
+                            const float* colB = col + static_cast<size_t>(ff.c2) * plane;
 #pragma unroll
-                                for (int e = 0; e < 4; e++) cv[i][e].x = __float_as_int(planeAt(colA, idx[e]));
+                            for (int e = 0; e < 4; e++) cv[i][e].x = __float_as_int(planeAt(colA, idx[e]));
 #pragma unroll
-                                for (int e = 4; e < 8; e++) cv[i][e].x = __float_as_int(planeAt(colB, idx[e]));
-                            }
+                            for (int e = 4; e < 8; e++) cv[i][e].x = __float_as_int(planeAt(colB, idx[e]));
                         }
                     }
                 }
@@ -502,7 +514,7 @@ __global__ void __launch_bounds__(256) k_threshold_ranks(const float* __restrict
     }
 }
 
-void launchFeatureGeneration(hipStream_t s, const DevImage* imgs, const FeatGenJob* jobsDev, int numJobs, int F,
+void launchFeatureGeneration(hipStream_t s, const ImageSet& imgs, const FeatGenJob* jobsDev, int numJobs, int F,
         int T, int boxRadius, int regionSize, int useDepth, const int* seedsHost, FeatTables& tabs) {
     if (T > kMaxThresholds) throw Error(CF_ERR_ARG, "too many thresholds (max 256)");
     if (F > 8192) throw Error(CF_ERR_ARG, "too many features (max 8192)");

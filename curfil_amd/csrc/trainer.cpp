@@ -263,7 +263,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         dJobs.resize(numActive);
         CF_HIP(hipMemcpyAsync(dJobs.get(), fj.data(), numActive * sizeof(FeatGenJob), hipMemcpyHostToDevice, s));
         timer.begin();
-        launchFeatureGeneration(s, images->desc.get(), dJobs.get(), numActive, F, T, cfg.box_radius, cfg.region_size,
+        launchFeatureGeneration(s, images->set(), dJobs.get(), numActive, F, T, cfg.box_radius, cfg.region_size,
                 cfg.use_depth_images, seeds.data(), tabs);
         timer.end(stats.featGenMs);
 
