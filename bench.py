@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--io-frames", type=int, default=32, help="frames written as PNG files for the I/O leg")
     ap.add_argument("--dense-only", action="store_true",
                     help="only the dense-prediction leg (10 trees of depth 20, 64-frame batches); for kernel profiles")
+    ap.add_argument("--predict-only", action="store_true",
+                    help="only the dense-prediction and serving legs")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy a device buffer of this many MiB 3 times (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -82,10 +84,11 @@ def main():
     import numpy as np
     import curfil_amd as cf
 
-    if args.dense_only:
+    if args.dense_only or args.predict_only:
         ctx = cf.Context(0)
-        predict10, _ = dense_prediction_leg(ctx)
-        print(json.dumps({"predict_forest10": predict10}), flush=True)
+        predict10, forest10 = dense_prediction_leg(ctx)
+        serving = serving_leg(ctx, forest10, args.serve_seconds) if args.predict_only else None
+        print(json.dumps({"predict_forest10": predict10, "serving": serving}), flush=True)
         return
 
     W, H, N = 640, 480, args.frames

@@ -86,7 +86,6 @@ struct cf_forest {
     cf::DevBuf<int> dFrames;
     cf::DevBuf<float> dProbs;
     cf::DevBuf<uint8_t> dLabels;
-    cf::DevBuf<float> dAcc;   // running sums of the tree-major passes when no posteriors are returned
     cf::TrainStats stats;
 };
 

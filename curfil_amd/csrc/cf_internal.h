@@ -273,16 +273,16 @@ void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk
 
 // Prediction
 struct DevForest {
-    const SplitRec* nodes;      // all trees concatenated; flags: left child (global index) or -(leaf index)-1
+    const SplitRec* nodes;      // split nodes of all trees in 128-byte blocks; flags / pad0: left / right child
+                                // reference (node index, or -(leaf index)-1)
     const double* post;         // [leaves][C]
-    const int32_t* treeRoot;    // root node index of each tree
+    const int32_t* treeRoot;    // root reference of each tree (a node, or a leaf for a one-node tree)
     int numTrees;
     int C;
 };
-// probs [frames][C][H][W] and labels [frames][H][W] may be null; with probs or acc (scratch of the same
-// shape as probs) the trees run tree-major over groups of frames, the running sums kept there
+// probs [frames][C][H][W] and labels [frames][H][W] may be null
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
-        const DevForest& forest, float* probs, uint8_t* labels, float* acc = nullptr);
+        const DevForest& forest, float* probs, uint8_t* labels);
 // one frame of a serving batch: its integrals and its outputs ([C][H][W] posteriors, [H][W] labels)
 struct FrameJob {
     DevImage im;

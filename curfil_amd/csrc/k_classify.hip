@@ -6,20 +6,17 @@
 //   * per tree, the leaf's double posterior is added into a float
 //     accumulator; the double sum normalises; the label is the first class
 //     with the largest probability (strict >, starting from -1).
-// One thread per pixel; the frames of a batch are the grid's y dimension.
-// With an accumulator (the posterior output, or a scratch buffer) a batch is
-// classified tree-major: frames in groups, each group one pass per tree, so
-// the passes running at a time share one tree's nodes and leaves (530 MB for
-// ten complete trees of depth 20 do not fit the last-level cache, one tree
-// does); without one, every tree of a pixel runs in one thread.
-#include <algorithm>
-#include <cstdlib>
-
+// One thread per pixel; the frames of a batch are the grid's y dimension, so
+// one launch classifies a whole batch against all trees.  The forest's split
+// nodes are packed in 128-byte blocks of a node and its two children (see
+// uploadForest), so a walk of 19 splits reads 10 node lines; a child that is
+// a leaf is referenced by its leaf index, so no leaf record is read.
 #include "cf_internal.h"
 
 namespace cf {
 
-// One node record: the feature (16 bytes) and {threshold, child, -, -}.
+// One node record: the feature (16 bytes) and {threshold, left, right, -}; a child reference >= 0 is
+// a node, < 0 the leaf -(ref)-1.
 struct NodeRec {
     uint4 feat;
     uint4 tail;
@@ -65,15 +62,16 @@ __device__ __forceinline__ bool pixelDepthAt(const DevImage& im, int x, int y, i
     return depth > 0;
 }
 
-// The leaf index one tree sends the pixel to.  Per split the 8 corner gathers are issued before the
-// response is formed; SplitFunction::split: left iff r <= threshold, NaN (invalid depth, a box outside
-// the image, a depth box without valid pixels) goes right.
+// The leaf index one tree sends the pixel to, from the tree's root reference.  Per split the 8 corner
+// gathers are issued before the response is formed; SplitFunction::split: left iff r <= threshold, NaN
+// (invalid depth, a box outside the image, a depth box without valid pixels) goes right.
 __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool depthOk, double rinv,
         const DevForest& forest, int root) {
     const int W = im.W, H = im.H;
     const size_t plane = static_cast<size_t>(W) * H;
-    NodeRec rec = loadNode(forest.nodes, root);
-    while (static_cast<int>(rec.tail.y) >= 0) {
+    int ref = root;
+    while (ref >= 0) {
+        const NodeRec rec = loadNode(forest.nodes, ref);
         const Feat f = unpackFeat(rec.feat);
         int idx[8];
         const bool isDepth = f.type == T_DEPTH;
@@ -103,11 +101,10 @@ __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool d
                 r = colorResponseFromCorners(c8);
             }
         }
-        const int child = static_cast<int>(rec.tail.y);
         const float thr = __uint_as_float(rec.tail.x);
-        rec = loadNode(forest.nodes, (r <= static_cast<double>(thr)) ? child : child + 1);
+        ref = static_cast<int>((r <= static_cast<double>(thr)) ? rec.tail.y : rec.tail.z);
     }
-    return -static_cast<int>(rec.tail.y) - 1;
+    return -ref - 1;
 }
 
 // The double sum of the float accumulator normalises it; the label is the first class with the
@@ -185,136 +182,27 @@ __global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restric
     classifyPixel<CM>(job.im, px, forest, job.probs, job.labels);
 }
 
-// Trees [t0, t1) for one pixel, added into its float accumulator acc[c * plane] (zero when t0 == 0),
-// which keeps the running sums between passes; after the last tree the accumulator becomes the
-// normalised posteriors (probs may be the accumulator itself) and the label is written.  The sums are
-// the single pass's, tree by tree in tree order.
-template<int CM>
-__device__ __forceinline__ void classifyPixelTrees(const DevImage& im, int px, const DevForest& forest, int t0, int t1,
-        float* __restrict__ acc, float* __restrict__ pout, uint8_t* __restrict__ lout) {
-    const int W = im.W, H = im.H;
-    const int x = px % W, y = px / W;
-    const int C = forest.C;
-    const size_t plane = static_cast<size_t>(W) * H;
-    int32_t depth;
-    const bool depthOk = pixelDepthAt(im, x, y, depth);
-    const double rinv = depthOk ? depthReciprocal(depthToFloat(depth)) : 0.0;
-    float a[CM];
-#pragma unroll
-    for (int c = 0; c < CM; c++) a[c] = (t0 > 0 && c < C) ? acc[c * plane + px] : 0.0f;
-    for (int t = t0; t < t1; t++) {
-        const int leaf = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
-        const double* post = forest.post + static_cast<size_t>(leaf) * C;
-#pragma unroll
-        for (int c = 0; c < CM; c++) {
-            if (c < C) a[c] = static_cast<float>(static_cast<double>(a[c]) + post[c]);
-        }
-    }
-    if (t1 < forest.numTrees) {
-#pragma unroll
-        for (int c = 0; c < CM; c++) {
-            if (c < C) acc[c * plane + px] = a[c];
-        }
-        return;
-    }
-    finishPixel<CM>(a, C, plane, px, pout, lout);
-}
-
-template<int CM>
-__global__ void __launch_bounds__(256) k_classify_pass(const DevImage* __restrict__ imgs,
-        const int* __restrict__ frameIdx, int f0, int W, int H, DevForest forest, int t0, int t1,
-        float* __restrict__ acc, float* __restrict__ probs, uint8_t* __restrict__ labels) {
-    const int frame = f0 + blockIdx.y;
-    const int px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= W * H) return;
-    const size_t plane = static_cast<size_t>(W) * H;
-    classifyPixelTrees<CM>(imgs[frameIdx[frame]], px, forest, t0, t1,
-            acc + static_cast<size_t>(frame) * forest.C * plane,
-            probs ? probs + static_cast<size_t>(frame) * forest.C * plane : nullptr,
-            labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
-}
-
-template<int CM>
-__global__ void __launch_bounds__(256) k_classify_jobs_pass(const FrameJob* __restrict__ jobs, int j0, int W, int H,
-        DevForest forest, int t0, int t1) {
-    const FrameJob job = jobs[j0 + blockIdx.y];
-    const int px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= W * H) return;
-    classifyPixelTrees<CM>(job.im, px, forest, t0, t1, job.probs, job.probs, job.labels);
-}
-
-namespace {
-
-// Passes of the tree-major schedule: frames in groups whose integral images take about
-// kGroupBytes (they stay in the last-level cache while every tree runs over them), trees in passes of
-// kPassTrees (one tree's nodes and leaves, 53 MB for a complete depth-20 tree, are what the passes
-// share).  CURFIL_CLASSIFY_GROUP_MB and CURFIL_CLASSIFY_PASS_TREES override them.
-constexpr double kGroupBytes = 96.0 * (1 << 20);
-constexpr int kPassTrees = 1;
-
-int envInt(const char* name, int def) {
-    const char* v = std::getenv(name);
-    return (v && *v) ? std::max(1, std::atoi(v)) : def;
-}
-
-int groupFrames(int W, int H, int numFrames) {
-    const double perFrame = static_cast<double>(W) * H * (3 * sizeof(float) + sizeof(int2));
-    const double bytes = envInt("CURFIL_CLASSIFY_GROUP_MB", 0) > 0 ? envInt("CURFIL_CLASSIFY_GROUP_MB", 0) * 1048576.0
-                                                                    : kGroupBytes;
-    return std::max(1, std::min(numFrames, static_cast<int>(bytes / perFrame)));
-}
-
-}  // namespace
-
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
-        const DevForest& forest, float* probs, uint8_t* labels, float* acc) {
-    if (forest.C > kMaxClasses) throw Error(CF_ERR_ARG, "too many classes (max 32)");
-    float* accum = probs ? probs : acc;
-    if (!accum || forest.numTrees <= 1) {
-        // every tree of a pixel in one thread: no accumulator between passes
-        const dim3 grid((W * H + 255) / 256, numFrames);
-        if (forest.C <= 8) {
-            hipLaunchKernelGGL(k_classify<8>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
-        } else {
-            hipLaunchKernelGGL(k_classify<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs,
-                    labels);
-        }
-        return;
-    }
-    const int fg = groupFrames(W, H, numFrames);
-    const int pt = envInt("CURFIL_CLASSIFY_PASS_TREES", kPassTrees);
-    for (int f0 = 0; f0 < numFrames; f0 += fg) {
-        const dim3 grid((W * H + 255) / 256, std::min(fg, numFrames - f0));
-        for (int t0 = 0; t0 < forest.numTrees; t0 += pt) {
-            const int t1 = std::min(forest.numTrees, t0 + pt);
-            if (forest.C <= 8) {
-                hipLaunchKernelGGL(k_classify_pass<8>, grid, dim3(256), 0, s, imgs, frameIdx, f0, W, H, forest, t0, t1,
-                        accum, probs, labels);
-            } else {
-                hipLaunchKernelGGL(k_classify_pass<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, f0, W, H, forest,
-                        t0, t1, accum, probs, labels);
-            }
-        }
+        const DevForest& forest, float* probs, uint8_t* labels) {
+    const dim3 grid((W * H + 255) / 256, numFrames);
+    if (forest.C <= 8) {
+        hipLaunchKernelGGL(k_classify<8>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
+    } else if (forest.C <= kMaxClasses) {
+        hipLaunchKernelGGL(k_classify<kMaxClasses>, grid, dim3(256), 0, s, imgs, frameIdx, W, H, forest, probs, labels);
+    } else {
+        throw Error(CF_ERR_ARG, "too many classes (max 32)");
     }
 }
 
 void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int W, int H, const DevForest& forest) {
     if (numJobs == 0) return;
-    if (forest.C > kMaxClasses) throw Error(CF_ERR_ARG, "too many classes (max 32)");
-    // the jobs' posterior buffers are the accumulators (every serving slot has one)
-    const int fg = groupFrames(W, H, numJobs);
-    const int pt = envInt("CURFIL_CLASSIFY_PASS_TREES", kPassTrees);
-    for (int j0 = 0; j0 < numJobs; j0 += fg) {
-        const dim3 grid((W * H + 255) / 256, std::min(fg, numJobs - j0));
-        for (int t0 = 0; t0 < forest.numTrees; t0 += pt) {
-            const int t1 = std::min(forest.numTrees, t0 + pt);
-            if (forest.C <= 8) {
-                hipLaunchKernelGGL(k_classify_jobs_pass<8>, grid, dim3(256), 0, s, jobs, j0, W, H, forest, t0, t1);
-            } else {
-                hipLaunchKernelGGL(k_classify_jobs_pass<kMaxClasses>, grid, dim3(256), 0, s, jobs, j0, W, H, forest, t0,
-                        t1);
-            }
-        }
+    const dim3 grid((W * H + 255) / 256, numJobs);
+    if (forest.C <= 8) {
+        hipLaunchKernelGGL(k_classify_jobs<8>, grid, dim3(256), 0, s, jobs, W, H, forest);
+    } else if (forest.C <= kMaxClasses) {
+        hipLaunchKernelGGL(k_classify_jobs<kMaxClasses>, grid, dim3(256), 0, s, jobs, W, H, forest);
+    } else {
+        throw Error(CF_ERR_ARG, "too many classes (max 32)");
     }
 }
 
