@@ -63,6 +63,9 @@ def main():
                     help="only the dense-prediction leg (10 trees of depth 20, 64-frame batches); for kernel profiles")
     ap.add_argument("--predict-only", action="store_true",
                     help="only the dense-prediction and serving legs")
+    ap.add_argument("--plain-full", action="store_true",
+                    help="also grow the whole batched workload (5 trees to depth 20) with the plain per-node path "
+                         "(about a minute)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy a device buffer of this many MiB 3 times (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -149,6 +152,19 @@ def main():
                       "size class) key (the warm-up step here) and keeps the fastest; shape 0 is the default"}
     log(f"tuning: {tuning['table']} (shape used {tuning['shape_used']})")
 
+    plain_full = None
+    if args.plain_full:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        forest.grow(cfg, cf.MODE_PLAIN)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        pp = forest.profile()
+        plain_full = {"seconds": dt, "evals_per_s": pp["evaluations"] / dt, "nodes": int(pp["nodes"]),
+                      "batched_evals_per_s": value, "speedup": value / (pp["evaluations"] / dt)}
+        log(f"plain per-node path on the whole workload: {dt:.1f}s, {plain_full['evals_per_s']:.3e} evals/s, "
+            f"batched is {plain_full['speedup']:.2f}x")
+
     full_frame = plain_vs_batched = predict = predict10 = serving = io = None
     if not args.no_extras:
         full_frame = full_frame_leg(ctx, images, full_kw, N, W, H)
@@ -188,6 +204,7 @@ def main():
                                                     "partition_ms")},
         "full_frame_training": full_frame,
         "plain_vs_batched": plain_vs_batched,
+        "plain_whole_workload": plain_full,
         "predict": predict,
         "predict_forest10": predict10,
         "serving": serving,

@@ -146,74 +146,45 @@ void uploadForest(cf_forest* fo) {
     if (fo->uploaded) return;
     if (!fo->ctx) throw Error(CF_ERR_ARG, "forest has no context");
     if (fo->post.size() != fo->trees.size()) throw Error(CF_ERR_RUNTIME, "histograms not normalized");
-    // Split nodes only, as 32-byte records {feature, threshold, left, right} in 128-byte blocks of a node and
-    // its split children (4 slots, the last unused), blocks in breadth-first order of their roots, so the
-    // two levels a walk takes inside a block share one line.  A child that is a leaf is referenced by
-    // -(leaf index)-1, and its posterior row is the only thing a walk reads of it.
+    // Split nodes only, in the trees' creation (breadth-first) order, as 32-byte records {feature,
+    // threshold, left, right}; a child that is a leaf is referenced by -(leaf index)-1, so a walk reads
+    // its posterior row and no leaf record.  Breadth-first order keeps the nodes that neighbouring pixels
+    // reach at one level within a few lines.
     std::vector<SplitRec> nodes;
     std::vector<int32_t> roots;
     std::vector<double> post;
-    constexpr int kBlock = 4;
     for (size_t t = 0; t < fo->trees.size(); t++) {
         const TreeBuild& tb = fo->trees[t];
         const int n = static_cast<int>(tb.nodes.size());
-        std::vector<int32_t> ref(n, 0);   // node i's reference: its slot, or its leaf
+        std::vector<int32_t> ref(n, 0);   // node i's reference: its record, or its leaf
+        for (int i = 0; i < n; i++) {
+            const cf_tree_node& nd = tb.nodes[i];
+            if (nd.is_leaf) {
+                const int leaf = static_cast<int>(post.size() / fo->C);
+                ref[i] = -leaf - 1;
+                post.insert(post.end(), fo->post[t].begin() + static_cast<size_t>(i) * fo->C,
+                        fo->post[t].begin() + static_cast<size_t>(i + 1) * fo->C);
+                continue;
+            }
+            if (nd.left <= i || nd.right <= i || nd.left >= n || nd.right >= n) {
+                throw Error(CF_ERR_ARG, "child indices must follow their parent");
+            }
+            if (nodes.size() >= static_cast<size_t>(INT32_MAX)) throw Error(CF_ERR_UNSUPPORTED, "forest too large");
+            ref[i] = static_cast<int32_t>(nodes.size());
+            nodes.push_back(SplitRec{});
+        }
+        roots.push_back(n == 0 ? -1 : ref[0]);
         for (int i = 0; i < n; i++) {
             const cf_tree_node& nd = tb.nodes[i];
             if (nd.is_leaf) continue;
-            if (nd.left < 0 || nd.right < 0 || nd.left >= n || nd.right >= n || nd.left <= i || nd.right <= i) {
-                throw Error(CF_ERR_ARG, "child indices must follow their parent");
-            }
+            SplitRec& sr = nodes[ref[i]];
+            for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = nd.feature[p];
+            sr.threshold = nd.threshold;
+            sr.flags = ref[nd.left];
+            sr.pad0 = ref[nd.right];
         }
-        // leaves in node order
-        for (int i = 0; i < n; i++) {
-            if (!tb.nodes[i].is_leaf) continue;
-            const int leaf = static_cast<int>(post.size() / fo->C);
-            ref[i] = -leaf - 1;
-            post.insert(post.end(), fo->post[t].begin() + static_cast<size_t>(i) * fo->C,
-                    fo->post[t].begin() + static_cast<size_t>(i + 1) * fo->C);
-        }
-        // slots: blocks rooted at split nodes, breadth first
-        const size_t base = nodes.size();
-        std::vector<int> order;   // node of each slot, -1 for an unused slot
-        std::vector<int> queue;
-        if (!tb.nodes.empty() && !tb.nodes[0].is_leaf) queue.push_back(0);
-        for (size_t q = 0; q < queue.size(); q++) {
-            const int r = queue[q];
-            const int kids[2] = {tb.nodes[r].left, tb.nodes[r].right};
-            const size_t slot0 = base + order.size();
-            order.push_back(r);
-            ref[r] = static_cast<int32_t>(slot0);
-            for (int k = 0; k < 2; k++) {
-                const int c = kids[k];
-                if (tb.nodes[c].is_leaf) {
-                    order.push_back(-1);
-                    continue;
-                }
-                ref[c] = static_cast<int32_t>(base + order.size());
-                order.push_back(c);
-                for (int g : {tb.nodes[c].left, tb.nodes[c].right}) {
-                    if (!tb.nodes[g].is_leaf) queue.push_back(g);
-                }
-            }
-            order.push_back(-1);
-        }
-        if (base + order.size() > static_cast<size_t>(INT32_MAX)) throw Error(CF_ERR_UNSUPPORTED, "forest too large");
-        roots.push_back(tb.nodes.empty() ? -1 : ref[0]);
-        for (int i : order) {
-            SplitRec sr{};
-            if (i >= 0) {
-                const cf_tree_node& nd = tb.nodes[i];
-                for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = nd.feature[p];
-                sr.threshold = nd.threshold;
-                sr.flags = ref[nd.left];
-                sr.pad0 = ref[nd.right];
-            }
-            nodes.push_back(sr);
-        }
-        static_assert(sizeof(SplitRec) * kBlock == 128, "a block of four records is one 128-byte line");
     }
-    if (nodes.empty()) nodes.resize(kBlock);
+    if (nodes.empty()) nodes.resize(1);
     fo->dNodes.resize(nodes.size());
     fo->dPost.resize(std::max<size_t>(post.size(), 1));
     fo->dRoots.resize(roots.size());

@@ -273,8 +273,8 @@ void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk
 
 // Prediction
 struct DevForest {
-    const SplitRec* nodes;      // split nodes of all trees in 128-byte blocks; flags / pad0: left / right child
-                                // reference (node index, or -(leaf index)-1)
+    const SplitRec* nodes;      // split nodes of all trees; flags / pad0: left / right child reference (node
+                                // index, or -(leaf index)-1)
     const double* post;         // [leaves][C]
     const int32_t* treeRoot;    // root reference of each tree (a node, or a leaf for a one-node tree)
     int numTrees;

@@ -48,25 +48,13 @@ struct BinSample {
 };
 
 // #{t : th[t] < q} over a row padded with +inf (never below q), T for NaN.
-// The thresholds hold no NaN and no -0 (k_bin_tables maps NaN to +inf and -0
-// to +0, which leaves every comparison unchanged), and q + 0 turns a -0
-// response into +0.  Then for finite q, th < q exactly when th - q is
-// negative: with denormals kept, x - y is zero only for x == y and then +0,
-// it is -0 only for -0 - +0, and overflow keeps the sign.  So the sign bit of
-// th[t] - q counts the threshold, with no compare-and-select.  An infinite q
-// (a response beyond the float range) takes the comparisons, since
-// inf - inf is NaN.
+// One compare and one add-with-carry per threshold: the compare's lane mask is
+// the carry-in of the count.
 template<int TM>
 __device__ __forceinline__ int binOfT(float q, const float (&th)[TM], int T) {
-    const float qc = q + 0.0f;
     uint32_t b = 0;
 #pragma unroll
-    for (int t = 0; t < TM; t++) b += __float_as_uint(th[t] - qc) >> 31;
-    if (__builtin_isinf(q)) {
-        b = 0;
-#pragma unroll
-        for (int t = 0; t < TM; t++) b += (th[t] < q) ? 1u : 0u;
-    }
+    for (int t = 0; t < TM; t++) b += (th[t] < q) ? 1u : 0u;
     return (q != q) ? T : static_cast<int>(b);
 }
 
@@ -236,9 +224,9 @@ __global__ void __launch_bounds__(LANES) k_eval_bins(const float* __restrict__ c
 
 // Feature records and thresholds in evaluation order, thresholds padded to
 // TM per feature with +inf: gFeat[tree][pos], gThr[tree][pos][TM].  A NaN
-// threshold becomes +inf and -0 becomes +0, for binOfT's sign-bit count;
-// neither changes any comparison th < q (NaN < q and +inf < q are false for
-// every q, and -0 and +0 compare equal).
+// threshold becomes +inf and -0 becomes +0; neither changes any comparison
+// th < q (NaN < q and +inf < q are false for every q, and -0 and +0 compare
+// equal).
 __global__ void __launch_bounds__(256) k_bin_tables(const FeatRec* __restrict__ feat, const float* __restrict__ thrFT,
         const uint16_t* __restrict__ evalOrder, int total, int F, int T, int TM, FeatRec* __restrict__ gFeat,
         float* __restrict__ gThr) {
@@ -467,6 +455,8 @@ constexpr EvalShape kEvalShapes[] = {
     {256, 2, 1, 1},
     {512, 1, 1, 1},
     {128, 1, 2, 4},
+    {128, 1, 4, 4},
+    {256, 1, 4, 2},
 };
 constexpr int kNumEvalShapes = static_cast<int>(sizeof(kEvalShapes) / sizeof(kEvalShapes[0]));
 static_assert(kNumEvalShapes <= 16, "TrainStats counts launches of at most 16 shapes");
@@ -541,6 +531,8 @@ void launchEvalBins(hipStream_t s, const ImageSet& imgs, const cf_sample* sample
     CF_SHAPE(256, 2, 1)
     CF_SHAPE(512, 1, 1)
     CF_SHAPE(128, 1, 2)
+    CF_SHAPE(128, 1, 4)
+    CF_SHAPE(256, 1, 4)
 #undef CF_SHAPE
     throw Error(CF_ERR_RUNTIME, "pass-1 launch shape without a kernel");
 }

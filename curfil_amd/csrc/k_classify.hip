@@ -7,10 +7,10 @@
 //     accumulator; the double sum normalises; the label is the first class
 //     with the largest probability (strict >, starting from -1).
 // One thread per pixel; the frames of a batch are the grid's y dimension, so
-// one launch classifies a whole batch against all trees.  The forest's split
-// nodes are packed in 128-byte blocks of a node and its two children (see
-// uploadForest), so a walk of 19 splits reads 10 node lines; a child that is
-// a leaf is referenced by its leaf index, so no leaf record is read.
+// one launch classifies a whole batch against all trees.  The forest holds
+// split nodes only (see uploadForest): a child that is a leaf is referenced by
+// its leaf index, so a walk ends at the last split without reading a leaf
+// record.
 #include "cf_internal.h"
 
 namespace cf {
