@@ -214,13 +214,16 @@ __device__ __forceinline__ double colorResponseFromCorners(const float* c) {
     return a - b;
 }
 
+// The mean is formed whether or not the box holds a valid pixel and then
+// selected, with no branch: a branch lets the compiler sink the loads of the
+// sums behind the count check and issue a corner's two halves apart.
 __device__ __forceinline__ double depthBoxFromCorners(const int2* d) {
     const int numValid = static_cast<int>((static_cast<uint32_t>(d[0].y) - static_cast<uint32_t>(d[2].y)) +
             (static_cast<uint32_t>(d[3].y) - static_cast<uint32_t>(d[1].y)));
-    if (numValid == 0) return cf_nan();
     const int sum = static_cast<int>((static_cast<uint32_t>(d[0].x) - static_cast<uint32_t>(d[2].x)) +
             (static_cast<uint32_t>(d[3].x) - static_cast<uint32_t>(d[1].x)));
-    return depthSumToMetres(sum) / static_cast<double>(numValid);
+    const double mean = depthSumToMetres(sum) / static_cast<double>(numValid);
+    return numValid == 0 ? cf_nan() : mean;
 }
 
 // averageRegionDepth of both boxes and their difference (NaN if a box has no
@@ -228,9 +231,7 @@ __device__ __forceinline__ double depthBoxFromCorners(const int2* d) {
 __device__ __forceinline__ double depthResponseFromCorners(const int2* d) {
     const double a = depthBoxFromCorners(d);
     const double b = depthBoxFromCorners(d + 4);
-    if (__builtin_isnan(a)) return a;
-    if (__builtin_isnan(b)) return b;
-    return a - b;
+    return __builtin_isnan(a) ? a : (__builtin_isnan(b) ? b : a - b);
 }
 
 // Smallest float >= r (NaN stays NaN).  For a float threshold t:

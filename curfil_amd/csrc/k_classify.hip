@@ -76,30 +76,27 @@ __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool d
         int idx[8];
         const bool isDepth = f.type == T_DEPTH;
         const bool ok = depthOk && responseCorners(f, x, y, W, H, rinv, false, idx);
-        int2 cv[8];   // depth corners, or colour corners' bits in .x
-        if (ok) {
-            if (isDepth) {
-#pragma unroll
-                for (int c = 0; c < 8; c++) cv[c] = planeAt(im.dep, idx[c]);
-            } else {
-                const float* colA = im.col + static_cast<size_t>(f.c1) * plane;
-                const float* colB = im.col + static_cast<size_t>(f.c2) * plane;
-#pragma unroll
-                for (int c = 0; c < 4; c++) cv[c].x = __float_as_int(planeAt(colA, idx[c]));
-#pragma unroll
-                for (int c = 4; c < 8; c++) cv[c].x = __float_as_int(planeAt(colB, idx[c]));
-            }
-        }
+        // one 8-byte load per depth corner and one 4-byte load per colour corner, in separate branches
+        // (a shared corner array makes the compiler split the depth loads in two)
         double r = cf_nan();
-        if (ok) {
-            if (isDepth) {
-                r = depthResponseFromCorners(cv);
-            } else {
-                float c8[8];
+        if (ok && isDepth) {
+            int2 dv[8];
 #pragma unroll
-                for (int c = 0; c < 8; c++) c8[c] = __int_as_float(cv[c].x);
-                r = colorResponseFromCorners(c8);
-            }
+            for (int c = 0; c < 8; c++) dv[c] = planeAt(im.dep, idx[c]);
+            // all 8 corners are needed here: the compiler may not split a corner's two halves and sink the
+            // sums behind the count check (that doubles the loads and serialises them)
+#pragma unroll
+            for (int c = 0; c < 8; c++) __asm__ volatile("" : : "v"(dv[c].x), "v"(dv[c].y));
+            r = depthResponseFromCorners(dv);
+        } else if (ok) {
+            const float* colA = im.col + static_cast<size_t>(f.c1) * plane;
+            const float* colB = im.col + static_cast<size_t>(f.c2) * plane;
+            float cv[8];
+#pragma unroll
+            for (int c = 0; c < 4; c++) cv[c] = planeAt(colA, idx[c]);
+#pragma unroll
+            for (int c = 4; c < 8; c++) cv[c] = planeAt(colB, idx[c]);
+            r = colorResponseFromCorners(cv);
         }
         const float thr = __uint_as_float(rec.tail.x);
         ref = static_cast<int>((r <= static_cast<double>(thr)) ? rec.tail.y : rec.tail.z);
@@ -151,9 +148,14 @@ __device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const 
     for (int t = 0; t < forest.numTrees; t++) {
         const int leaf = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
         const double* post = forest.post + static_cast<size_t>(leaf) * C;
+        // the leaf's row is read whole (indices past C repeat the last class, in bounds and unused) so the
+        // loads go out together instead of one conditional load after another
+        double pv[CM];
+#pragma unroll
+        for (int c = 0; c < CM; c++) pv[c] = post[c < C ? c : C - 1];
 #pragma unroll
         for (int c = 0; c < CM; c++) {
-            if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + post[c]);
+            if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + pv[c]);
         }
     }
     finishPixel<CM>(acc, C, plane, px, pout, lout);
