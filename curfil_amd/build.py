@@ -25,6 +25,7 @@ SOURCES = [
     "k_classify.hip",
     "predict_noalloc.hip",
     "capi.cpp",
+    "forest_pack.cpp",
     "trainer.cpp",
     "tuner.cpp",
     "sampling.cpp",

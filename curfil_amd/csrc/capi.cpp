@@ -6,6 +6,7 @@
 #include <thread>
 
 #include "cf_host.h"
+#include "forest_pack.h"
 
 using namespace cf;
 
@@ -142,56 +143,39 @@ void bindForest(cf_forest* fo, cf_ctx* ctx) {
     if (fo->ctx != ctx) throw Error(CF_ERR_ARG, "forest belongs to another context");
 }
 
+static_assert(FC_TYPE == P_TYPE && FC_O1X == P_O1X && FC_O2Y == P_O2Y && FC_R1X == P_R1X && FC_R2Y == P_R2Y &&
+                FC_C1 == P_C1 && FC_C2 == P_C2,
+        "feature columns");
+static_assert(sizeof(NodeBlock) == kBlockWords * sizeof(uint32_t), "node block size");
+static_assert(postStride(5) == rowStride(5) && postStride(8) == rowStride(8), "posterior row stride");
+
 void uploadForest(cf_forest* fo) {
     if (fo->uploaded) return;
     if (!fo->ctx) throw Error(CF_ERR_ARG, "forest has no context");
     if (fo->post.size() != fo->trees.size()) throw Error(CF_ERR_RUNTIME, "histograms not normalized");
-    // Split nodes only, in the trees' creation (breadth-first) order, as 32-byte records {feature,
-    // threshold, left, right}; a child that is a leaf is referenced by -(leaf index)-1, so a walk reads
-    // its posterior row and no leaf record.  Breadth-first order keeps the nodes that neighbouring pixels
-    // reach at one level within a few lines.
-    std::vector<SplitRec> nodes;
-    std::vector<int32_t> roots;
-    std::vector<double> post;
+    std::vector<const cf_tree_node*> trees;
+    std::vector<int> sizes;
+    std::vector<const double*> post;
     for (size_t t = 0; t < fo->trees.size(); t++) {
-        const TreeBuild& tb = fo->trees[t];
-        const int n = static_cast<int>(tb.nodes.size());
-        std::vector<int32_t> ref(n, 0);   // node i's reference: its record, or its leaf
-        for (int i = 0; i < n; i++) {
-            const cf_tree_node& nd = tb.nodes[i];
-            if (nd.is_leaf) {
-                const int leaf = static_cast<int>(post.size() / fo->C);
-                ref[i] = -leaf - 1;
-                post.insert(post.end(), fo->post[t].begin() + static_cast<size_t>(i) * fo->C,
-                        fo->post[t].begin() + static_cast<size_t>(i + 1) * fo->C);
-                continue;
-            }
-            if (nd.left <= i || nd.right <= i || nd.left >= n || nd.right >= n) {
-                throw Error(CF_ERR_ARG, "child indices must follow their parent");
-            }
-            if (nodes.size() >= static_cast<size_t>(INT32_MAX)) throw Error(CF_ERR_UNSUPPORTED, "forest too large");
-            ref[i] = static_cast<int32_t>(nodes.size());
-            nodes.push_back(SplitRec{});
-        }
-        roots.push_back(n == 0 ? -1 : ref[0]);
-        for (int i = 0; i < n; i++) {
-            const cf_tree_node& nd = tb.nodes[i];
-            if (nd.is_leaf) continue;
-            SplitRec& sr = nodes[ref[i]];
-            for (int p = 0; p < kFeatParams; p++) sr.feat.p[p] = nd.feature[p];
-            sr.threshold = nd.threshold;
-            sr.flags = ref[nd.left];
-            sr.pad0 = ref[nd.right];
-        }
+        trees.push_back(fo->trees[t].nodes.data());
+        sizes.push_back(static_cast<int>(fo->trees[t].nodes.size()));
+        post.push_back(fo->post[t].data());
     }
-    if (nodes.empty()) nodes.resize(1);
-    fo->dNodes.resize(nodes.size());
-    fo->dPost.resize(std::max<size_t>(post.size(), 1));
-    fo->dRoots.resize(roots.size());
+    PackedForest pf;
+    std::string msg;
+    const int rc = packForest(trees, sizes, post, fo->C, pf, msg);
+    if (rc != CF_OK) throw Error(rc, msg);
+    if (pf.blocks.empty()) pf.blocks.resize(kBlockWords, 0u);
+    if (pf.post.empty()) pf.post.resize(postStride(fo->C));
+    const size_t numBlocks = pf.blocks.size() / kBlockWords;
+    fo->dBlocks.resize(numBlocks);
+    fo->dPost.resize(pf.post.size());
+    fo->dRoots.resize(pf.roots.size());
     hipStream_t s = fo->ctx->stream;
-    CF_HIP(hipMemcpyAsync(fo->dNodes.get(), nodes.data(), nodes.size() * sizeof(SplitRec), hipMemcpyHostToDevice, s));
-    CF_HIP(hipMemcpyAsync(fo->dPost.get(), post.data(), post.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    CF_HIP(hipMemcpyAsync(fo->dRoots.get(), roots.data(), roots.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemcpyAsync(fo->dBlocks.get(), pf.blocks.data(), numBlocks * sizeof(NodeBlock), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemcpyAsync(fo->dPost.get(), pf.post.data(), pf.post.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemcpyAsync(fo->dRoots.get(), pf.roots.data(), pf.roots.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+            s));
     CF_HIP(hipStreamSynchronize(s));
     fo->uploaded = true;
 }
@@ -893,7 +877,7 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         if (probs) fo->dProbs.resize(static_cast<size_t>(count) * fo->C * plane);
         if (labels) fo->dLabels.resize(static_cast<size_t>(count) * plane);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(),
+        DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(),
                 static_cast<int>(fo->trees.size()), fo->C};
         launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df,
                 probs ? fo->dProbs.get() : nullptr, labels ? fo->dLabels.get() : nullptr);
@@ -919,7 +903,7 @@ int cf_predict_batch_device(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int
         uploadForest(fo);
         fo->dFrames.resize(count);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()), fo->C};
+        DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()), fo->C};
         launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df, d_probs, d_labels);
         CF_HIP(hipStreamSynchronize(ctx->stream));
     });

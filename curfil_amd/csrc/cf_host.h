@@ -79,7 +79,7 @@ struct cf_forest {
     std::vector<std::vector<uint32_t>> treePriors;
     std::vector<std::vector<double>> post;  // normalized histograms [nodes][C] per tree
     bool uploaded = false;
-    cf::DevBuf<cf::SplitRec> dNodes;
+    cf::DevBuf<cf::NodeBlock> dBlocks;
     cf::DevBuf<double> dPost;
     cf::DevBuf<int32_t> dRoots;
     // prediction staging kept between calls (grown on demand)

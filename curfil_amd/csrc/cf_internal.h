@@ -13,6 +13,10 @@
 
 namespace cf {
 
+// XCDs of an MI355X: the dispatcher sends workgroup i of a launch to XCD i % kNumXcds, each XCD with
+// its own L2.
+constexpr int kNumXcds = 8;
+
 // Thrown inside the library, turned into a status at the C ABI.
 struct Error : std::runtime_error {
     int code;
@@ -272,11 +276,25 @@ void launchPartitionScatter(hipStream_t s, const cf_sample* samples, const Chunk
         const uint8_t* dir, const int2* dst, cf_sample* out);
 
 // Prediction
+//
+// The split nodes of a forest, packed in 128-byte blocks (one cache line): a block holds a subtree of up to
+// three split levels, its nodes at heap positions 0..6 (position p's children at 2p+1 and 2p+2).  Slots 0..6
+// are node records {threshold bits, offsets o1x o1y o2x o2y, regions r1x r1y r2x r2y, type c1 c2 -} (signed
+// bytes, as in FeatRec); slot 7 is the block's tail {first child block, first leaf, leaf mask, child-block
+// mask | first exit position << 16}.  A child position q below the first exit position (7, or 1 / 3 for a
+// tree's top block of one or two levels) holds a split of the block; at or past it, a child that is a
+// split starts another block.  Bit q of the leaf mask marks a leaf child, numbered from the first leaf in
+// position order; bit q of the child-block mask a child block, numbered from the first child block in
+// position order.
+struct NodeBlock {
+    uint4 slot[8];
+};
+// doubles per posterior row of a packed forest: C rounded up to even, so rows are 16-byte aligned
+__host__ __device__ constexpr int postStride(int C) { return (C + 1) & ~1; }
 struct DevForest {
-    const SplitRec* nodes;      // split nodes of all trees; flags / pad0: left / right child reference (node
-                                // index, or -(leaf index)-1)
-    const double* post;         // [leaves][C]
-    const int32_t* treeRoot;    // root reference of each tree (a node, or a leaf for a one-node tree)
+    const NodeBlock* blocks;    // all trees' blocks, each tree's breadth-first from its root block
+    const double* post;         // [leaves][postStride(C)], a block's leaves together
+    const int32_t* treeRoot;    // root block of each tree, or -(leaf index)-1 for a one-node tree
     int numTrees;
     int C;
 };

@@ -201,7 +201,7 @@ void cf_server::run() {
         CF_HIP(hipSetDevice(ctx->device));
         std::vector<Request*> ready;     // integrating or integrated, not yet classified
         std::vector<Request*> inflight;  // classified, results on their way to the host
-        const DevForest df{fo->dNodes.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()),
+        const DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()),
                 fo->C};
         for (;;) {
             std::vector<Request*> admit;

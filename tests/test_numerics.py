@@ -99,3 +99,16 @@ def test_division_by_1000_as_corrected_product_is_exact_for_every_int32(tmp_path
     subprocess.run(["gcc", "-O2", "-march=native", "-ffp-contract=off", "-o", exe, src, "-lm"], check=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.strip()
     assert out == "0"
+
+
+def test_forest_block_packing_walks_reach_the_plain_walks_leaves(tmp_path):
+    # the prediction layout (curfil_amd/csrc/forest_pack.cpp): every walk stepped through the 128-byte
+    # blocks the way k_classify steps stays in bounds, visits the plain walk's nodes and reaches its leaf
+    import subprocess
+    import os
+    here = os.path.dirname(__file__)
+    exe = str(tmp_path / "check_forest_pack")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(here, "native", "check_forest_pack.cpp"),
+                    os.path.join(here, "..", "curfil_amd", "csrc", "forest_pack.cpp")], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.strip()
+    assert out.startswith("ok ") and int(out.split()[1]) > 30000, out
