@@ -63,6 +63,8 @@ def main():
                     help="only the dense-prediction leg (10 trees of depth 20, 64-frame batches); for kernel profiles")
     ap.add_argument("--predict-only", action="store_true",
                     help="only the dense-prediction and serving legs")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving loop (no dense timings, no plain one-frame path); for tuning the loop")
     ap.add_argument("--plain-full", action="store_true",
                     help="also grow the whole batched workload (5 trees to depth 20) with the plain per-node path "
                          "(about a minute)")
@@ -87,11 +89,22 @@ def main():
     import numpy as np
     import curfil_amd as cf
 
+    if args.serve_only:
+        ctx = cf.Context(0)
+        forest10 = dense_forest(ctx)
+        serving = serving_leg(ctx, forest10, args.serve_seconds, plain=False)
+        serving640 = serving_leg(ctx, forest10, args.serve_seconds, plain=False, mix=(0, 1, 0), rates=(2000, 3000))
+        print(json.dumps({"serving": serving, "serving_640x480": serving640}), flush=True)
+        return
     if args.dense_only or args.predict_only:
         ctx = cf.Context(0)
         predict10, forest10 = dense_prediction_leg(ctx)
-        serving = serving_leg(ctx, forest10, args.serve_seconds) if args.predict_only else None
-        print(json.dumps({"predict_forest10": predict10, "serving": serving}), flush=True)
+        serving = serving640 = None
+        if args.predict_only:
+            serving = serving_leg(ctx, forest10, args.serve_seconds)
+            serving640 = serving_leg(ctx, forest10, args.serve_seconds, mix=(0, 1, 0), rates=(2000, 3000))
+        print(json.dumps({"predict_forest10": predict10, "serving": serving, "serving_640x480": serving640}),
+              flush=True)
         return
 
     W, H, N = 640, 480, args.frames
@@ -165,12 +178,13 @@ def main():
         log(f"plain per-node path on the whole workload: {dt:.1f}s, {plain_full['evals_per_s']:.3e} evals/s, "
             f"batched is {plain_full['speedup']:.2f}x")
 
-    full_frame = plain_vs_batched = predict = predict10 = serving = io = None
+    full_frame = plain_vs_batched = predict = predict10 = serving = serving640 = io = None
     if not args.no_extras:
         full_frame = full_frame_leg(ctx, images, full_kw, N, W, H)
         plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
         predict10, forest10 = dense_prediction_leg(ctx)
         serving = serving_leg(ctx, forest10, args.serve_seconds)
+        serving640 = serving_leg(ctx, forest10, args.serve_seconds, mix=(0, 1, 0), rates=(2000, 3000))
         io = io_leg(forest, cfg, rgb, depth, labels, args.io_frames)
 
     cpu = None
@@ -208,6 +222,7 @@ def main():
         "predict": predict,
         "predict_forest10": predict10,
         "serving": serving,
+        "serving_640x480": serving640,
         "io": io,
     }
     print(json.dumps(out), flush=True)
@@ -389,10 +404,10 @@ def extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees):
     return plain_vs_batched, predict
 
 
-def dense_prediction_leg(ctx):
-    """Batched dense prediction: 10 complete trees of depth 20 with random splits (thresholds = responses
-    at random pixels, Dirichlet(1) leaves over 5 classes, seed 4711) on 64 synthetic 640x480 frames
-    (12 Voronoi sites, 5 classes, 5% missing depth)."""
+def dense_forest(ctx, frames=False):
+    """10 complete trees of depth 20 with random splits (thresholds = responses at random pixels,
+    Dirichlet(1) leaves over 5 classes, seed 4711) drawn on 64 synthetic 640x480 frames (12 Voronoi sites,
+    5 classes, 5% missing depth); with frames=True also the frames' images."""
     import numpy as np
     import curfil_amd as cf
     W, H, n = 640, 480, 64
@@ -405,6 +420,15 @@ def dense_prediction_leg(ctx):
     forest.normalizeHistograms(0.0, False)
     forest.predict_batch(im, list(range(n)), probabilities=True)   # forest upload + warm
     log(f"forest of 10 trees x depth 20 built in {time.time() - t0:.1f}s")
+    return (forest, im) if frames else forest
+
+
+def dense_prediction_leg(ctx):
+    """Batched dense prediction with the forest of dense_forest on its 64 frames."""
+    import numpy as np
+    import curfil_amd as cf
+    W, H, n = 640, 480, 64
+    forest, im = dense_forest(ctx, frames=True)
     out = {"workload": "64 frames 640x480, 10 complete trees of depth 20 (random splits), 5 classes"}
     # outputs to host: into page-locked arrays (DMA at the link's rate), and into ordinary numpy arrays
     pl = cf.pinned_empty(ctx, (n, H, W), np.uint8)
@@ -450,15 +474,15 @@ def dense_prediction_leg(ctx):
     return out, forest
 
 
-def serving_leg(ctx, forest, seconds):
-    """Poisson arrivals of RGB-D frames (320x240 40%, 640x480 50%, 1280x960 10%) at 1000-4000 frames/s
-    into the serving loop (8 GB budget), against the plain path (one frame at a time, in arrival order).
-    Latency = results in host memory - arrival."""
+def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(1000, 2000, 3000, 4000)):
+    """Poisson arrivals of RGB-D frames (by default 320x240 40%, 640x480 50%, 1280x960 10%) at 1000-4000
+    frames/s into the serving loop (8 GB budget), against the plain path (one frame at a time, in arrival
+    order).  Latency = results in host memory - arrival."""
     import threading
     import numpy as np
     import curfil_amd as cf
     sizes = [(320, 240), (640, 480), (1280, 960)]
-    mix = np.array([0.4, 0.5, 0.1])
+    mix = np.array(mix, dtype=np.float64)
     frames = []
     for k, (W, H) in enumerate(sizes):
         rgb, depth, _ = cf.synth_dataset(8, W, H, sites=12, classes=5, zero_frac=0.05, seed0=7711 + 100 * k)
@@ -468,15 +492,16 @@ def serving_leg(ctx, forest, seconds):
         im = cf.Images(ctx, 1, W, H)
         singles.append(im)
     sys.setswitchinterval(0.0002)
-    results = {"workload": "Poisson arrivals, frames 320x240 (40%) / 640x480 (50%) / 1280x960 (10%), "
-                           "forest of 10 trees x depth 20, posteriors + labels to host", "rates": []}
+    shares = " / ".join(f"{W}x{H} ({100 * m:.0f}%)" for (W, H), m in zip(sizes, mix) if m > 0)
+    results = {"workload": f"Poisson arrivals, frames {shares}, forest of 10 trees x depth 20, posteriors + "
+                           f"labels to host", "rates": []}
     rng = np.random.default_rng(4711)
     server = cf.Server(ctx, forest, budget_mb=8192.0, max_batch=64)
     # warm-up: the slots and pinned buffers a burst needs are allocated before timing
     warm = [server.submit(frames[k][0][i % 8], frames[k][1][i % 8]) for k in range(3) for i in range(40)]
     for rid in warm:
         server.wait(rid, probabilities=False, labels=False)
-    for lam in (1000, 2000, 3000, 4000):
+    for lam in rates:
         n = int(lam * seconds)
         arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
         kinds = rng.choice(3, n, p=mix)
@@ -516,6 +541,11 @@ def serving_leg(ctx, forest, seconds):
                           "median_parts_ms": {"queued": float(np.median(parts[:, 0])),
                                               "to_launch": float(np.median(parts[:, 1])),
                                               "device_and_copy": float(np.median(parts[:, 2]))}}
+        if not plain:
+            log(f"serving {lam} fps offered: {row['serving']['frames_per_s']:.0f} fps, p50 "
+                f"{row['serving']['p50_ms']:.2f} p99 {row['serving']['p99_ms']:.2f} ms")
+            results["rates"].append(row)
+            continue
         # plain path: one frame at a time in arrival order (upload + integrate + predict + results to host)
         lat2 = np.zeros(n)
         t0 = time.perf_counter()

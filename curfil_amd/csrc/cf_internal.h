@@ -302,11 +302,16 @@ struct DevForest {
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
         const DevForest& forest, float* probs, uint8_t* labels);
 // one frame of a serving batch: its integrals and its outputs ([C][H][W] posteriors, [H][W] labels)
+// Frames of any sizes share a launch: job i's tiles are firstTile .. (job i + 1's firstTile) - 1 of the
+// launch, firstTile ascending from 0 in job order (classifyTiles(W, H) tiles per frame).
 struct FrameJob {
     DevImage im;
     float* probs;
     uint8_t* labels;
+    int firstTile;
+    int pad;
 };
-void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int W, int H, const DevForest& forest);
+int classifyTiles(int W, int H);
+void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int totalTiles, const DevForest& forest);
 
 }  // namespace cf

@@ -236,14 +236,25 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
             labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
 }
 
-// independent frames of one size, each with its own images and outputs
+// independent frames of any sizes, each with its own images and outputs; the launch's tiles are cut into
+// XCD ranges as a batch's are, and a workgroup finds its frame by binary search over the jobs' first tiles
 template<int CM>
-__global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restrict__ jobs, int numJobs, int W, int H,
+__global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restrict__ jobs, int numJobs, int totalTiles,
         DevForest forest) {
     ClassifyTile tile;
-    if (!classifyTile(tileBlocks(W, H), numJobs, tile)) return;
-    const FrameJob job = jobs[tile.frame];
-    const int px = tilePixel(tile.block, W, H);
+    if (!classifyTile(totalTiles, 1, tile)) return;
+    const int j = tile.block;
+    int lo = 0, hi = numJobs - 1;   // the last job whose first tile is <= j
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].firstTile <= j) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    const FrameJob job = jobs[lo];
+    const int px = tilePixel(j - job.firstTile, job.im.W, job.im.H);
     if (px < 0) return;
     classifyPixel<CM>(job.im, px, forest, job.probs, job.labels);
 }
@@ -271,11 +282,14 @@ void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, in
     });
 }
 
-void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int W, int H, const DevForest& forest) {
+int classifyTiles(int W, int H) { return tileBlocks(W, H); }
+
+void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int totalTiles, const DevForest& forest) {
     if (numJobs == 0) return;
-    const dim3 grid = classifyGrid(tileBlocks(W, H), numJobs);
+    const dim3 grid = classifyGrid(totalTiles, 1);
     dispatchClasses(forest.C, [&](auto cm) {
-        hipLaunchKernelGGL(k_classify_jobs<decltype(cm)::value>, grid, dim3(256), 0, s, jobs, numJobs, W, H, forest);
+        hipLaunchKernelGGL(k_classify_jobs<decltype(cm)::value>, grid, dim3(256), 0, s, jobs, numJobs, totalTiles,
+                forest);
     });
 }
 

@@ -1,8 +1,9 @@
 // Serving loop for dense prediction: RGB-D frames of any resolution arrive
 // one by one (cf_server_submit), are admitted in arrival order while device
 // memory of the budget is free, have their integral images built on an
-// upload stream, are classified in batches of equal resolution on a compute
-// stream, and return their posteriors and labels through a download stream.
+// upload stream, are classified in batches (frames of all sizes in one
+// launch) on a compute stream, and return their posteriors and labels
+// through a download stream.
 // One worker thread keeps the three stages busy: frames that arrive while a
 // batch runs are admitted and integrated meanwhile and go into the next
 // batch (continuous batching), and a frame's device slot is reused as soon
@@ -11,7 +12,9 @@
 // RandomForestImage::predict (random_forest_image.cpp:258-352) per frame: the
 // same kernel body as cf_predict, so a frame's results do not depend on what
 // else is in flight.
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -240,9 +243,9 @@ void cf_server::run() {
                 CF_HIP(hipEventRecord(r->ready, up));
                 ready.push_back(r);
             }
-            // classify what is ready, one launch per resolution (at most maxBatch frames each), while the
-            // unfinished launches hold less than kMaxPendingPixels of work: under load, ready frames
-            // gather into larger batches instead of queueing many small launches
+            // classify what is ready, frames of all sizes together (at most maxBatch frames per launch, in
+            // arrival order), while the unfinished launches hold less than kMaxPendingPixels of work: under
+            // load, ready frames gather into larger launches instead of queueing many small ones
             double pendingPx = 0;
             while (!launchedPx.empty()) {
                 const hipError_t q = hipEventQuery(jobDone[launchedPx.front().first]);
@@ -252,46 +255,54 @@ void cf_server::run() {
             }
             for (const auto& lp : launchedPx) pendingPx += lp.second;
             if (!ready.empty() && pendingPx < kMaxPendingPixels) {
-                std::map<std::pair<int, int>, std::vector<Request*>> groups;
-                for (Request* r : ready) groups[{r->W, r->H}].push_back(r);
-                ready.clear();
-                for (auto& kv : groups) {
-                    auto& v = kv.second;
-                    for (size_t b = 0; b < v.size(); b += maxBatch) {
-                        const int n = static_cast<int>(std::min<size_t>(maxBatch, v.size() - b));
-                        const int e = jobNext;
-                        jobNext = (jobNext + 1) % kJobRing;
-                        CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
-                        hipStream_t cs = comp;
-                        FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
-                        for (int i = 0; i < n; i++) {
-                            Request* r = v[b + i];
-                            Slot* s = r->slot;
-                            hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
-                            hj[i].probs = s->probs.get();
-                            hj[i].labels = s->labels.get();
-                            CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
-                        }
-                        FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
-                        CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
-                        launchClassifyJobs(cs, dj, n, kv.first.first, kv.first.second, df);
-                        CF_HIP(hipEventRecord(jobDone[e], cs));
-                        launchedPx.emplace_back(e, static_cast<double>(n) * kv.first.first * kv.first.second);
-                        launches++;
-                        CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
-                        for (int i = 0; i < n; i++) {
-                            Request* r = v[b + i];
-                            Slot* s = r->slot;
-                            const size_t px = static_cast<size_t>(r->W) * r->H;
-                            CF_HIP(hipMemcpyAsync(r->hostProbs, s->probs.get(), px * fo->C * sizeof(float),
-                                    hipMemcpyDeviceToHost, down));
-                            CF_HIP(hipMemcpyAsync(r->hostLabels, s->labels.get(), px, hipMemcpyDeviceToHost, down));
-                            CF_HIP(hipEventRecord(r->done, down));
-                            r->launched = Clock::now();
-                            inflight.push_back(r);
-                        }
+                for (size_t b = 0; b < ready.size(); b += maxBatch) {
+                    const int n = static_cast<int>(std::min<size_t>(maxBatch, ready.size() - b));
+                    const int e = jobNext;
+                    jobNext = (jobNext + 1) % kJobRing;
+                    CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
+                    hipStream_t cs = comp;
+                    FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
+                    int tiles = 0;
+                    double px = 0;
+                    for (int i = 0; i < n; i++) {
+                        Request* r = ready[b + i];
+                        Slot* s = r->slot;
+                        hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
+                        hj[i].probs = s->probs.get();
+                        hj[i].labels = s->labels.get();
+                        hj[i].firstTile = tiles;
+                        hj[i].pad = 0;
+                        const int t = classifyTiles(r->W, r->H);
+                        if (tiles > INT32_MAX - t) throw Error(CF_ERR_UNSUPPORTED, "too many pixels in one launch");
+                        tiles += t;
+                        px += static_cast<double>(r->W) * r->H;
+                        CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
+                    }
+                    FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
+                    CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
+                    launchClassifyJobs(cs, dj, n, tiles, df);
+                    CF_HIP(hipEventRecord(jobDone[e], cs));
+                    launchedPx.emplace_back(e, px);
+                    launches++;
+                    CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
+                    // results of the launch's frames to the host, smallest first (a large frame's copy does
+                    // not hold up the small ones finished with it)
+                    std::vector<Request*> order(ready.begin() + b, ready.begin() + b + n);
+                    std::stable_sort(order.begin(), order.end(), [](const Request* x, const Request* y) {
+                        return static_cast<int64_t>(x->W) * x->H < static_cast<int64_t>(y->W) * y->H;
+                    });
+                    for (Request* r : order) {
+                        Slot* s = r->slot;
+                        const size_t rpx = static_cast<size_t>(r->W) * r->H;
+                        CF_HIP(hipMemcpyAsync(r->hostProbs, s->probs.get(), rpx * fo->C * sizeof(float),
+                                hipMemcpyDeviceToHost, down));
+                        CF_HIP(hipMemcpyAsync(r->hostLabels, s->labels.get(), rpx, hipMemcpyDeviceToHost, down));
+                        CF_HIP(hipEventRecord(r->done, down));
+                        r->launched = Clock::now();
+                        inflight.push_back(r);
                     }
                 }
+                ready.clear();
             }
             // finished frames: results are in pinned host memory; their slots are free again
             std::vector<Request*> still;
