@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=120000)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
+    ap.add_argument("--graph-step", action="store_true",
+                    help="with --no-extras: also the training step with level graphs (profiling off)")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window per serving rate")
     ap.add_argument("--io-frames", type=int, default=32, help="frames written as PNG files for the I/O leg")
     ap.add_argument("--dense-only", action="store_true",
@@ -178,7 +180,9 @@ def main():
         log(f"plain per-node path on the whole workload: {dt:.1f}s, {plain_full['evals_per_s']:.3e} evals/s, "
             f"batched is {plain_full['speedup']:.2f}x")
 
-    full_frame = plain_vs_batched = predict = predict10 = serving = serving640 = io = None
+    full_frame = plain_vs_batched = predict = predict10 = serving = serving640 = io = graph_step = None
+    if not args.no_extras or args.graph_step:
+        graph_step = level_graph_step(ctx, forest, cfg, nodes, step_times)
     if not args.no_extras:
         full_frame = full_frame_leg(ctx, images, full_kw, N, W, H)
         plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
@@ -216,6 +220,7 @@ def main():
         "cpu_baseline": cpu,
         "phase_ms": {k: round(prof[k], 1) for k in ("featgen_ms", "eval_kernel_ms", "hist_ms", "score_ms",
                                                     "partition_ms")},
+        "level_graph_step": graph_step,
         "full_frame_training": full_frame,
         "plain_vs_batched": plain_vs_batched,
         "plain_whole_workload": plain_full,
@@ -226,6 +231,25 @@ def main():
         "io": io,
     }
     print(json.dumps(out), flush=True)
+
+
+def level_graph_step(ctx, forest, cfg, nodes, step_times):
+    """One more training step with profiling off: every level whose pass-1 shape is settled runs as one
+    captured graph (the timed steps launch kernel by kernel, since their phase timers need events on the
+    stream).  Reports its time beside the timed steps' and whether it grew the same trees."""
+    import curfil_amd as cf
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    forest.grow(cfg, cf.MODE_BATCHED)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    p = forest.profile()
+    same = [len(forest.tree(t)[0]) for t in range(len(nodes))] == nodes
+    out = {"seconds": dt, "timed_steps_mean_s": sum(step_times) / len(step_times), "level_graphs": int(p["level_graphs"]),
+           "levels": int(p["levels"]), "same_node_counts": same}
+    log(f"step with level graphs: {dt:.3f}s ({out['level_graphs']} of {out['levels']} levels captured), "
+        f"same trees: {same}")
+    return out
 
 
 def eval_roofline(profiles, N, W, H, T):

@@ -414,7 +414,7 @@ class RandomForestImage:
 
     PROFILE_KEYS = ("evaluations", "seconds", "eval_kernel_ms", "eval_launches", "responses", "featgen_ms",
                     "score_ms", "partition_ms", "levels", "replays", "nodes", "hist_ms", "group_batches_max",
-                    "slot_batches_max", "eval_shape")
+                    "slot_batches_max", "eval_shape", "level_graphs")
 
     def profile(self):
         out = np.zeros(len(self.PROFILE_KEYS), np.float64)

@@ -688,7 +688,8 @@ int cf_forest_profile(const cf_forest* fo, double* out, int n) {
     const double v[] = {st.evaluations, st.seconds, st.evalKernelMs, st.evalLaunches, st.responses, st.featGenMs,
             st.scoreMs, st.partitionMs, static_cast<double>(st.levels), static_cast<double>(st.replays), st.nodes,
             st.histMs, static_cast<double>(st.groupBatchesMax), static_cast<double>(st.slotBatchesMax),
-            static_cast<double>(std::max_element(st.shapeLaunches, st.shapeLaunches + 16) - st.shapeLaunches)};
+            static_cast<double>(std::max_element(st.shapeLaunches, st.shapeLaunches + 16) - st.shapeLaunches),
+            static_cast<double>(st.levelGraphs)};
     const int k = static_cast<int>(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < k; i++) out[i] = v[i];
     return k;

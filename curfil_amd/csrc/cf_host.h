@@ -30,6 +30,8 @@ struct TrainStats {
     int groupBatchesMax = 0, slotBatchesMax = 0;
     // two-pass evaluation: pass-1 launches per launch shape (see evalShape)
     double shapeLaunches[16] = {};
+    // two-pass evaluation: levels evaluated as one captured graph
+    int levelGraphs = 0;
 };
 
 // One tree to grow: its id, random source state and root samples.

@@ -210,6 +210,8 @@ void launchLevelLabels(hipStream_t s, const cf_sample* samples, int n, uint8_t* 
 // starts[numSegs] = the number of entries with a key below numSegs << yBits.  count and cursor hold
 // levelSortBuckets entries, blockSum levelSortScanBlocks.
 size_t levelSortBuckets(int numSegs, int yBits);
+// p[0 .. n) = v by a kernel (usable inside a captured graph, where memset nodes are avoided)
+void launchFillU32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 size_t levelSortScanBlocks(int numSegs, int yBits);
 void levelCountingSort(hipStream_t s, const uint32_t* keys, const uint32_t* vals, int n, int numSegs, int yBits,
         uint32_t* count, uint32_t* cursor, uint32_t* blockSum, uint32_t* out, int* starts);

@@ -152,6 +152,11 @@ inline unsigned blocksFor(size_t n, int per) { return static_cast<unsigned>((n +
 
 }  // namespace
 
+void launchFillU32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocksFor(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
 size_t levelSortBuckets(int numSegs, int yBits) {
     // the segments, one past the last for the flipped evaluations of samples that are not Both, and
     // the empty bucket after them that k_sort_starts reads

@@ -68,6 +68,44 @@ public:
     hipEvent_t a = nullptr, b = nullptr;
 };
 
+// A level's evaluation captured from the stream into a graph, instantiated and launched once.  The
+// capture holds kernels only (buffers are cleared by a fill kernel, not a memset node; tables are
+// uploaded before it), so nothing in it depends on host memory.
+class LevelGraph {
+public:
+    explicit LevelGraph(hipStream_t s) : s_(s) {}
+    ~LevelGraph() {
+        if (capturing_) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s_, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (launched_) (void)hipStreamSynchronize(s_);   // normally done already: the level waits for its argmax
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+    }
+    LevelGraph(const LevelGraph&) = delete;
+    LevelGraph& operator=(const LevelGraph&) = delete;
+    void beginCapture() {
+        CF_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
+    }
+    void launch() {
+        capturing_ = false;
+        CF_HIP(hipStreamEndCapture(s_, &graph_));
+        CF_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        CF_HIP(hipGraphLaunch(exec_, s_));
+        launched_ = true;
+    }
+    bool launched() const { return launched_; }
+
+private:
+    hipStream_t s_;
+    bool capturing_ = false, launched_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
 struct LevelNode {
     int active;  // index into the active list (feature table)
     int node;    // node index in its tree
@@ -75,38 +113,67 @@ struct LevelNode {
     int chunkBegin, chunkEnd;
 };
 
-// Device time of the work enqueued between begin() and end() on the stream.
+// Device time of the work enqueued between begin() and end() on the stream.  The events are read by
+// flush(), called where the host waits for the stream anyway, so timing adds no host waits between
+// the kernels of a level.
 class PhaseTimer {
 public:
-    PhaseTimer(hipStream_t s, bool on) : s_(s), on_(on) {
-        if (on_) {
-            CF_HIP(hipEventCreate(&a_));
-            CF_HIP(hipEventCreate(&b_));
-        }
-    }
+    PhaseTimer(hipStream_t s, bool on) : s_(s), on_(on) {}
     ~PhaseTimer() {
-        if (on_) {
-            (void)hipEventDestroy(a_);
-            (void)hipEventDestroy(b_);
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        for (const Pending& p : pending_) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
+        if (cur_) (void)hipEventDestroy(cur_);
     }
     void begin() {
-        if (on_) CF_HIP(hipEventRecord(a_, s_));
+        if (!on_) return;
+        if (!cur_) cur_ = take();
+        CF_HIP(hipEventRecord(cur_, s_));
     }
-    // adds the elapsed milliseconds to `acc` (synchronises on the end event)
+    // the phase's elapsed milliseconds are added to `acc` at the next flush()
     void end(double& acc) {
         if (!on_) return;
-        CF_HIP(hipEventRecord(b_, s_));
-        CF_HIP(hipEventSynchronize(b_));
-        float ms = 0;
-        CF_HIP(hipEventElapsedTime(&ms, a_, b_));
-        acc += ms;
+        hipEvent_t b = take();
+        CF_HIP(hipEventRecord(b, s_));
+        pending_.push_back(Pending{cur_, b, &acc});
+        cur_ = nullptr;
+    }
+    // waits for the last ended phase and accumulates every ended phase
+    void flush() {
+        if (!on_ || pending_.empty()) return;
+        CF_HIP(hipEventSynchronize(pending_.back().b));
+        for (const Pending& p : pending_) {
+            float ms = 0;
+            CF_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            *p.acc += ms;
+            pool_.push_back(p.a);
+            pool_.push_back(p.b);
+        }
+        pending_.clear();
     }
 
 private:
+    struct Pending {
+        hipEvent_t a, b;
+        double* acc;
+    };
+    hipEvent_t take() {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        CF_HIP(hipEventCreate(&e));
+        return e;
+    }
     hipStream_t s_;
     bool on_;
-    hipEvent_t a_ = nullptr, b_ = nullptr;
+    hipEvent_t cur_ = nullptr;
+    std::vector<hipEvent_t> pool_;
+    std::vector<Pending> pending_;
 };
 
 // Device memory budgets of a batched level.  CURFIL_*_BUDGET_MB sets them in MiB
@@ -211,6 +278,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     FeatTables tabs;
     DevBuf<FeatGenJob> dJobs;
     DevBuf<Chunk> dChunks;
+    DevBuf<int2> dGraphWork[2];   // work lists of a captured level: full and last feature-group batch
     DevBuf<uint32_t> dNodeHist;
     DevBuf<int> dSlotTree;
     DevBuf<uint32_t> dG;
@@ -238,6 +306,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<uint16_t> dEvalOrder;
     DevBuf<uint4> dBins;
     const size_t binsBudget = binsBudgetBytes();
+    // levels whose pass-1 shape is settled run as one captured graph (CURFIL_LEVEL_GRAPH=0: kernel by kernel).
+    // Not while the context profiles: the phase timers' events have to be recorded on the stream (events
+    // recorded inside a captured graph give no elapsed times); their reads wait for nothing in between.
+    const char* lg = std::getenv("CURFIL_LEVEL_GRAPH");
+    const bool levelGraphs = !(lg && std::strcmp(lg, "0") == 0) && !ctx->profiling;
     PhaseTimer timer(s, ctx->profiling);
     const size_t perSlot = static_cast<size_t>(F) * (T + 1) * C;
     const size_t budget = evalBudgetBytes();
@@ -335,6 +408,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             timer.end(stats.scoreMs);
             CF_HIP(hipMemcpyAsync(&best[s0], dBest.get(), ns * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
+            timer.flush();
             for (int i = 0; i < ns; i++) {
                 if (!best[s0 + i].flag) continue;
                 // near-tie band: replay the reference's sequential scan for this node
@@ -343,6 +417,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 launchSequentialArgmax(s, dAllScores.get(), T, F, dBest.get() + i);
                 CF_HIP(hipMemcpyAsync(&best[s0 + i], dBest.get() + i, sizeof(NodeBest), hipMemcpyDeviceToHost, s));
                 CF_HIP(hipStreamSynchronize(s));
+                timer.flush();
                 stats.replays++;
             }
         };
@@ -384,6 +459,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                 std::vector<NodeBest> rb(ns);
                 CF_HIP(hipMemcpyAsync(rb.data(), dRepBest.get(), ns * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
                 CF_HIP(hipStreamSynchronize(s));
+                timer.flush();
                 for (int k = 0; k < ns; k++) best[flagged[a + k]] = rb[k];
                 stats.replays += ns;
                 a += ns;
@@ -426,6 +502,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             std::vector<int> starts(numKeys + 1);
             CF_HIP(hipMemcpyAsync(starts.data(), dStarts.get(), (numKeys + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
+            timer.flush();
             if (starts[numKeys] < 0) starts[numKeys] = M;   // the end of the last segment
             // blocks of one pass-1 block size: consecutive runs of <= blockSamples entries within a (image, tree)
             // segment; the segments of one image form a group of consecutive blocks.  Built on the host from the
@@ -482,16 +559,20 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                         hipMemcpyHostToDevice, s));
                 return bs;
             };
-            auto workFor = [&](BlockSet& bs, int ngb) {
-                if (bs.workGroups == ngb) return;
-                constexpr int kXcds = 8;
-                bs.workGroups = ngb;
+            // the work list of a batch of ngb feature groups into `work`; returns its length
+            auto buildWork = [&](BlockSet& bs, int ngb, DevBuf<int2>& work) {
                 const int maxLen = bs.maxXcdBlocks * ngb;
-                bs.numWork = maxLen * kXcds;
-                bs.work.resize(std::max<size_t>(static_cast<size_t>(bs.numWork), 1));
+                const int numWork = maxLen * kNumXcds;
+                work.resize(std::max<size_t>(static_cast<size_t>(numWork), 1));
                 const int ns = bs.numSegs;
                 launchBuildWork(s, bs.segInfo.get(), bs.segInfo.get() + ns, bs.segInfo.get() + 2 * ns,
-                        bs.segInfo.get() + 3 * ns, ns, ngb, maxLen, bs.work.get());
+                        bs.segInfo.get() + 3 * ns, ns, ngb, maxLen, work.get());
+                return numWork;
+            };
+            auto workFor = [&](BlockSet& bs, int ngb) {
+                if (bs.workGroups == ngb) return;
+                bs.workGroups = ngb;
+                bs.numWork = buildWork(bs, ngb, bs.work);
             };
             dEvalOrder.resize(static_cast<size_t>(numActive) * F);
             launchFeatureEvalOrder(s, tabs, numActive, dEvalOrder.get());
@@ -522,7 +603,94 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
             dBins.resize(static_cast<size_t>(ng) * N * (anyBoth ? 2 : 1));
             stats.groupBatchesMax = std::max(stats.groupBatchesMax, (groupsTotal + ng - 1) / ng);
             const int TM = binThresholdWidth(T);
-            for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
+            bool tuning = false;
+            const int levelShape = evalTunerChoose(TM, static_cast<size_t>(M), &tuning);
+            // node-slot batches of a batch of ngb feature groups: the slots whose histograms fit the budget
+            auto slotBatches = [&](int ngb) {
+                const size_t perSlot = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
+                const int maxSlots = static_cast<int>(std::max<size_t>(1, budget / (perSlot * sizeof(uint32_t))));
+                std::vector<std::pair<int, int>> out;
+                for (int s0 = 0; s0 < numNodes; s0 += maxSlots) out.emplace_back(s0, std::min(numNodes, s0 + maxSlots));
+                return out;
+            };
+            LevelGraph graph(s);
+            if (!tuning && levelGraphs) {
+                // The shape is settled: the level's evaluation (every batch's pass 1; per node-slot batch the
+                // histogram clear, pass 2 and the scores; the argmax) is captured into one graph and launched
+                // once, so the device runs it back to back with no host work between its kernels.  Work
+                // lists, chunk tables and buffers are set up before the capture.
+                const EvalShape& es = evalShape(levelShape);
+                BlockSet& bs = blocksFor(es.lanes * es.perLane);
+                struct Batch {
+                    int g0, ngb;
+                    const DevBuf<int2>* work;
+                    int numWork;
+                    std::vector<std::pair<int, int>> slots;
+                };
+                const int lastNgb = groupsTotal - ((groupsTotal - 1) / ng) * ng;
+                const int numWorkFull = buildWork(bs, ng, dGraphWork[0]);
+                const int numWorkLast = lastNgb == ng ? numWorkFull : buildWork(bs, lastNgb, dGraphWork[1]);
+                bs.workGroups = -1;
+                std::vector<Batch> batches;
+                std::vector<Chunk> allChunks;
+                std::vector<size_t> chunkOff;   // per (batch, slot batch), in order
+                size_t maxG = 1;
+                for (int g0 = 0; g0 < groupsTotal; g0 += ng) {
+                    const int ngb = std::min(ng, groupsTotal - g0);
+                    Batch b{g0, ngb, ngb == ng ? &dGraphWork[0] : &dGraphWork[1], ngb == ng ? numWorkFull : numWorkLast,
+                            slotBatches(ngb)};
+                    const size_t perSlotB = static_cast<size_t>(ngb) * kBinGroup * (T + 1) * C;
+                    for (const auto& sb : b.slots) {
+                        chunkOff.push_back(allChunks.size());
+                        for (int c = nodes[sb.first].chunkBegin; c < nodes[sb.second - 1].chunkEnd; c++) {
+                            Chunk ch = chunks[c];
+                            ch.slot -= sb.first;
+                            allChunks.push_back(ch);
+                        }
+                        maxG = std::max(maxG, static_cast<size_t>(sb.second - sb.first) * perSlotB);
+                    }
+                    batches.push_back(std::move(b));
+                }
+                dChunks.resize(std::max<size_t>(allChunks.size(), 1));
+                CF_HIP(hipMemcpyAsync(dChunks.get(), allChunks.data(), allChunks.size() * sizeof(Chunk),
+                        hipMemcpyHostToDevice, s));
+                dG.resize(maxG);
+                CF_HIP(hipStreamSynchronize(s));   // allChunks is read by the copy; nothing host-side in the capture
+                timer.flush();
+                graph.beginCapture();
+                size_t k = 0;
+                for (const Batch& b : batches) {
+                    timer.begin();
+                    launchEvalBins(s, images->set(), level[cur].get(), dPerm.get(), bs.blocks.get(), b.work->get(),
+                            b.numWork, b.g0, dGFeat.get(), dGThr.get(), tabs, N, dBins.get(), flipOffset, levelShape);
+                    timer.end(stats.evalKernelMs);
+                    stats.evalLaunches += 1;
+                    stats.shapeLaunches[levelShape] += 1;
+                    stats.slotBatchesMax = std::max(stats.slotBatchesMax, static_cast<int>(b.slots.size()));
+                    const size_t perSlotB = static_cast<size_t>(b.ngb) * kBinGroup * (T + 1) * C;
+                    for (const auto& sb : b.slots) {
+                        const int s0 = sb.first, ns = sb.second - sb.first;
+                        const int nc = nodes[sb.second - 1].chunkEnd - nodes[s0].chunkBegin;
+                        launchFillU32(s, dG.get(), static_cast<size_t>(ns) * perSlotB, 0u);
+                        timer.begin();
+                        launchHistBins(s, dBins.get(), dLab8.get(), dChunks.get() + chunkOff[k], nc, b.g0, b.ngb, tabs,
+                                C, N, dG.get(), flipOffset);
+                        timer.end(stats.histMs);
+                        timer.begin();
+                        launchScores(s, dG.get(), dNodeHist.get() + static_cast<size_t>(s0) * C, dSlotTree.get() + s0,
+                                ns, tabs, C, dSumm.get() + static_cast<size_t>(s0) * F, nullptr, -1, b.g0 * kBinGroup,
+                                b.ngb * kBinGroup, dEvalOrder.get());
+                        timer.end(stats.scoreMs);
+                        k++;
+                    }
+                }
+                timer.begin();
+                launchReduce(s, dSumm.get(), numNodes, F, dBest.get());
+                timer.end(stats.scoreMs);
+                graph.launch();
+                stats.levelGraphs += 1;
+            }
+            for (int g0 = 0; g0 < (graph.launched() ? 0 : groupsTotal); g0 += ng) {
                 const int ngb = std::min(ng, groupsTotal - g0);
                 bool measure = false;
                 const int shape = evalTunerChoose(TM, static_cast<size_t>(M), &measure);
@@ -569,11 +737,14 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
                     s0 = s1;
                 }
             }
-            timer.begin();
-            launchReduce(s, dSumm.get(), numNodes, F, dBest.get());
-            timer.end(stats.scoreMs);
+            if (!graph.launched()) {
+                timer.begin();
+                launchReduce(s, dSumm.get(), numNodes, F, dBest.get());
+                timer.end(stats.scoreMs);
+            }
             CF_HIP(hipMemcpyAsync(best.data(), dBest.get(), numNodes * sizeof(NodeBest), hipMemcpyDeviceToHost, s));
             CF_HIP(hipStreamSynchronize(s));
+            timer.flush();
             std::vector<int> flagged;
             for (int i = 0; i < numNodes; i++) {
                 if (best[i].flag) flagged.push_back(i);
@@ -626,6 +797,7 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         CF_HIP(hipMemcpyAsync(leftHist.data(), dLeftHist.get(), leftHist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         CF_HIP(hipMemcpyAsync(splits.data(), dSplits.get(), numNodes * sizeof(SplitRec), hipMemcpyDeviceToHost, s));
         CF_HIP(hipStreamSynchronize(s));
+        timer.flush();
 
         // ---- tree growth on the host, next frontier and sample destinations
         std::vector<int2> dst(chunks.size(), make_int2(-1, -1));
@@ -712,9 +884,11 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
         }
         // the host vectors handed to the copies above live until here
         CF_HIP(hipStreamSynchronize(s));
+        timer.flush();
         active.swap(next);
     }
     CF_HIP(hipStreamSynchronize(s));
+    timer.flush();
     stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -189,12 +189,13 @@ int cf_forest_grow(cf_ctx* ctx, cf_images* images, cf_forest* forest, const cf_t
 /* per-phase device timing of training (hipEvents; the reference's utils::Profile,
  * utils.h:159-203, and per-node timers, random_tree.h:634-679) */
 int cf_ctx_set_profiling(cf_ctx* ctx, int enable);
-/* out[0..13]: evaluations, wall seconds, response-kernel ms, response-kernel
+/* out[0..15]: evaluations, wall seconds, response-kernel ms, response-kernel
  * launches, responses, feature-generation ms, score+argmax ms, partition ms,
  * levels, sequential-replay nodes, evaluated nodes, node-histogram ms (the
  * second pass of batched levels), the most feature-group batches and the most
  * node-slot batches of one batched level, the pass-1 launch shape used most
- * (cf_eval_shape_info).  The response kernel is the one that
+ * (cf_eval_shape_info), the batched levels evaluated as one captured graph
+ * (CURFIL_LEVEL_GRAPH=0 launches them one by one).  The response kernel is the one that
  * gathers the integrals: k_eval_hist (direct path) or k_eval_bins (batched
  * levels).  Returns the number of entries. */
 int cf_forest_profile(const cf_forest* forest, double* out, int n);

@@ -303,6 +303,35 @@ def test_every_launch_shape_trains_like_oracle(ctx, small, shape):
     _compare_forest(forest, _small_oracle_flip(ds), 2)
 
 
+def test_captured_level_graphs_and_eager_launches_train_like_oracle(ctx, small):
+    """Levels whose pass-1 shape is settled run as one captured graph each (several feature-group and
+    node-slot batches per level here); the same training with every kernel launched on its own
+    (CURFIL_LEVEL_GRAPH=0, or a profiling context, whose phase timers need events on the stream) gives the
+    same trees, all equal to the oracle's."""
+    im, ds = small
+    params = dict(CONFIG0, maxDepth=8, horizontalFlipping=True)
+    forests = {}
+    for mode in ("graphs", "eager", "profiled"):
+        ctx.set_profiling(mode == "profiled")
+        try:
+            with _Env(CURFIL_LEVEL_GRAPH=0 if mode == "eager" else 1, CURFIL_BINS_BUDGET_KB=160,
+                      CURFIL_EVAL_BUDGET_KB=96):
+                forests[mode] = cf.RandomForestImage.train(ctx, im, cf.TrainingConfiguration(**params),
+                                                           tree_count=2, num_labels=6, mode=cf.MODE_BATCHED)
+        finally:
+            ctx.set_profiling(False)
+    prof = {m: f.profile() for m, f in forests.items()}
+    print({m: (p["level_graphs"], p["eval_launches"], p["group_batches_max"], p["slot_batches_max"])
+           for m, p in prof.items()})
+    assert prof["graphs"]["level_graphs"] >= 1
+    assert prof["eager"]["level_graphs"] == 0 and prof["profiled"]["level_graphs"] == 0
+    assert prof["graphs"]["group_batches_max"] > 1 and prof["graphs"]["slot_batches_max"] > 1
+    assert prof["graphs"]["eval_launches"] == prof["eager"]["eval_launches"] == prof["profiled"]["eval_launches"]
+    assert prof["profiled"]["eval_kernel_ms"] > 0 and prof["profiled"]["hist_ms"] > 0
+    for f in forests.values():
+        _compare_forest(f, _small_oracle_flip(ds), 2)
+
+
 _ORACLE_CACHE = {}
 
 
