@@ -58,8 +58,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="only the training steps (no plain-vs-batched or prediction legs); for kernel profiles")
     ap.add_argument("--graph-step", action="store_true",
-                    help="with --no-extras: also the training step with level graphs (profiling off)")
-    ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window per serving rate")
+                    help="with --no-extras: also a training step with level graphs (CURFIL_LEVEL_GRAPH=1, profiling off)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
     ap.add_argument("--io-frames", type=int, default=32, help="frames written as PNG files for the I/O leg")
     ap.add_argument("--dense-only", action="store_true",
                     help="only the dense-prediction leg (10 trees of depth 20, 64-frame batches); for kernel profiles")
@@ -181,8 +181,6 @@ def main():
             f"batched is {plain_full['speedup']:.2f}x")
 
     full_frame = plain_vs_batched = predict = predict10 = serving = serving640 = io = graph_step = None
-    if not args.no_extras or args.graph_step:
-        graph_step = level_graph_step(ctx, forest, cfg, nodes, step_times)
     if not args.no_extras:
         full_frame = full_frame_leg(ctx, images, full_kw, N, W, H)
         plain_vs_batched, predict = extra_legs(ctx, images, forest, cfg_kw, N, W, H, trees)
@@ -190,6 +188,11 @@ def main():
         serving = serving_leg(ctx, forest10, args.serve_seconds)
         serving640 = serving_leg(ctx, forest10, args.serve_seconds, mix=(0, 1, 0), rates=(2000, 3000))
         io = io_leg(forest, cfg, rgb, depth, labels, args.io_frames)
+
+    # last of the GPU legs: once a graph has been instantiated the runtime holds a hardware queue, which
+    # would slow the serving loop's streams created after it
+    if not args.no_extras or args.graph_step:
+        graph_step = level_graph_step(ctx, forest, cfg, nodes, step_times)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -234,15 +237,23 @@ def main():
 
 
 def level_graph_step(ctx, forest, cfg, nodes, step_times):
-    """One more training step with profiling off: every level whose pass-1 shape is settled runs as one
-    captured graph (the timed steps launch kernel by kernel, since their phase timers need events on the
-    stream).  Reports its time beside the timed steps' and whether it grew the same trees."""
+    """One more training step with CURFIL_LEVEL_GRAPH=1 and profiling off: every level whose pass-1 shape
+    is settled runs as one captured graph (the timed steps launch kernel by kernel, the library's
+    default).  Reports its time beside the timed steps' and whether it grew the same trees."""
     import curfil_amd as cf
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    forest.grow(cfg, cf.MODE_BATCHED)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
+    old = os.environ.get("CURFIL_LEVEL_GRAPH")
+    os.environ["CURFIL_LEVEL_GRAPH"] = "1"
+    try:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        forest.grow(cfg, cf.MODE_BATCHED)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        if old is None:
+            del os.environ["CURFIL_LEVEL_GRAPH"]
+        else:
+            os.environ["CURFIL_LEVEL_GRAPH"] = old
     p = forest.profile()
     same = [len(forest.tree(t)[0]) for t in range(len(nodes))] == nodes
     out = {"seconds": dt, "timed_steps_mean_s": sum(step_times) / len(step_times), "level_graphs": int(p["level_graphs"]),
@@ -525,6 +536,11 @@ def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(10
     warm = [server.submit(frames[k][0][i % 8], frames[k][1][i % 8]) for k in range(3) for i in range(40)]
     for rid in warm:
         server.wait(rid, probabilities=False, labels=False)
+    # the client's own pauses are not the library's latency: no cyclic garbage collection while timing
+    import gc
+    gc.collect()
+    gc_was_enabled = gc.isenabled()
+    gc.disable()
     for lam in rates:
         n = int(lam * seconds)
         arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
@@ -591,6 +607,8 @@ def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(10
             f"p50 {row['plain']['p50_ms']:.1f} p99 {row['plain']['p99_ms']:.1f} ms")
         results["rates"].append(row)
     server.close()
+    if gc_was_enabled:
+        gc.enable()
     return results
 
 

@@ -302,26 +302,43 @@ __global__ void __launch_bounds__(256) k_hist_bins(const uint4* __restrict__ bin
 
 // Per tree: feature evaluation order grouped by the planes a feature reads
 // (depth first, then colour by (channel1, channel2)), stable in feature index.
+// One wave per table: a stable counting sort over the 10 keys, 64 features at a time -- a lane's place
+// among the lanes of its key is the count of its key's ballot bits below it.
 __global__ void __launch_bounds__(64) k_feature_eval_order(const FeatRec* __restrict__ feat, int F,
         uint16_t* __restrict__ evalOrder) {
     constexpr int kKeys = 10;
-    __shared__ int count[kKeys];
-    __shared__ int start[kKeys];
     const FeatRec* ft = feat + static_cast<size_t>(blockIdx.x) * F;
     uint16_t* eo = evalOrder + static_cast<size_t>(blockIdx.x) * F;
-    if (threadIdx.x < kKeys) count[threadIdx.x] = 0;
-    __syncthreads();
-    auto keyOf = [](const Feat& x) { return x.type == T_DEPTH ? 0 : 1 + x.c1 * 3 + x.c2; };
-    for (int f = threadIdx.x; f < F; f += blockDim.x) atomicAdd(&count[keyOf(loadFeat(ft + f))], 1);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
+    const int lane = threadIdx.x;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    auto keyOf = [&](int f) {
+        if (f >= F) return -1;
+        const Feat x = loadFeat(ft + f);
+        return x.type == T_DEPTH ? 0 : 1 + x.c1 * 3 + x.c2;
+    };
+    int start[kKeys];
+#pragma unroll
+    for (int k = 0; k < kKeys; k++) start[k] = 0;
+    for (int f0 = 0; f0 < F; f0 += 64) {
+        const int key = keyOf(f0 + lane);
+#pragma unroll
+        for (int k = 0; k < kKeys; k++) start[k] += __popcll(__ballot(key == k));
+    }
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kKeys; k++) {
+        const int c = start[k];
+        start[k] = acc;
+        acc += c;
+    }
+    for (int f0 = 0; f0 < F; f0 += 64) {
+        const int key = keyOf(f0 + lane);
+#pragma unroll
         for (int k = 0; k < kKeys; k++) {
-            start[k] = acc;
-            acc += count[k];
+            const uint64_t m = __ballot(key == k);
+            if (key == k) eo[start[k] + __popcll(m & below)] = static_cast<uint16_t>(f0 + lane);
+            start[k] += __popcll(m);
         }
-        // stable placement (F is small: a few thousand features per table)
-        for (int f = 0; f < F; f++) eo[start[keyOf(loadFeat(ft + f))]++] = static_cast<uint16_t>(f);
     }
 }
 

@@ -443,16 +443,20 @@ __global__ void __launch_bounds__(256) k_fg_walk(const FeatGenJob* __restrict__ 
 }
 
 // Stable sort of each job's F features by key (rank = #smaller keys + #equal
-// keys earlier), scattering features and thresholds into both layouts.
+// keys earlier), scattering features and thresholds into both layouts.  The
+// grid is (F / 256 blocks, jobs): every workgroup holds all of its job's keys
+// in LDS and ranks 256 of its features.
 __global__ void __launch_bounds__(256) k_fg_sort(const int* __restrict__ keys, const FeatRec* __restrict__ inFeat,
         const float* __restrict__ inThr, int F, int T, FeatRec* __restrict__ feat, float* __restrict__ thrFT,
         float* __restrict__ thrTF, int8_t* __restrict__ soa) {
     extern __shared__ int sk[];
-    const int job = blockIdx.x;
+    const int job = blockIdx.y;
     const int* jk = keys + static_cast<size_t>(job) * F;
     for (int i = threadIdx.x; i < F; i += blockDim.x) sk[i] = jk[i];
     __syncthreads();
-    for (int i = threadIdx.x; i < F; i += blockDim.x) {
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= F) return;
         const int ki = sk[i];
         int rank = 0;
         for (int j = 0; j < F; j++) {
@@ -541,7 +545,7 @@ void launchFeatureGeneration(hipStream_t s, const ImageSet& imgs, const FeatGenJ
             boxRadius, regionSize, useDepth, tabs.cand.get(), tabs.err.get());
     hipLaunchKernelGGL(k_fg_walk, dim3(numJobs), dim3(256), 0, s, jobsDev, imgs, tabs.cand.get(), numCand, F, T,
             tabs.unsortedFeat.get(), tabs.unsortedThr.get(), tabs.unsortedKey.get(), tabs.err.get());
-    hipLaunchKernelGGL(k_fg_sort, dim3(numJobs), dim3(256), F * sizeof(int), s, tabs.unsortedKey.get(),
+    hipLaunchKernelGGL(k_fg_sort, dim3((F + 255) / 256, numJobs), dim3(256), F * sizeof(int), s, tabs.unsortedKey.get(),
             tabs.unsortedFeat.get(), tabs.unsortedThr.get(), F, T, tabs.feat.get(), tabs.thrFT.get(),
             tabs.thrTF.get(), tabs.soa.get());
     const int total = numJobs * F;

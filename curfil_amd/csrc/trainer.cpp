@@ -306,11 +306,14 @@ void trainTrees(cf_ctx* ctx, cf_images* images, const cf_train_config& cfg, int 
     DevBuf<uint16_t> dEvalOrder;
     DevBuf<uint4> dBins;
     const size_t binsBudget = binsBudgetBytes();
-    // levels whose pass-1 shape is settled run as one captured graph (CURFIL_LEVEL_GRAPH=0: kernel by kernel).
-    // Not while the context profiles: the phase timers' events have to be recorded on the stream (events
-    // recorded inside a captured graph give no elapsed times); their reads wait for nothing in between.
+    // CURFIL_LEVEL_GRAPH=1: levels whose pass-1 shape is settled run as one captured graph.  Off by default:
+    // once a graph has been instantiated the HIP runtime holds one of the process's hardware queues
+    // (GPU_MAX_HW_QUEUES, 4 by default), so streams created later share queues (the serving loop's three
+    // lost a fifth of their capacity), and a level launched kernel by kernel is as fast: its host reads
+    // wait for nothing between kernels.  Not while the context profiles either: the phase timers' events
+    // have to be recorded on the stream (events recorded inside a captured graph give no elapsed times).
     const char* lg = std::getenv("CURFIL_LEVEL_GRAPH");
-    const bool levelGraphs = !(lg && std::strcmp(lg, "0") == 0) && !ctx->profiling;
+    const bool levelGraphs = lg && std::strcmp(lg, "1") == 0 && !ctx->profiling;
     PhaseTimer timer(s, ctx->profiling);
     const size_t perSlot = static_cast<size_t>(F) * (T + 1) * C;
     const size_t budget = evalBudgetBytes();

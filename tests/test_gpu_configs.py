@@ -304,10 +304,10 @@ def test_every_launch_shape_trains_like_oracle(ctx, small, shape):
 
 
 def test_captured_level_graphs_and_eager_launches_train_like_oracle(ctx, small):
-    """Levels whose pass-1 shape is settled run as one captured graph each (several feature-group and
-    node-slot batches per level here); the same training with every kernel launched on its own
-    (CURFIL_LEVEL_GRAPH=0, or a profiling context, whose phase timers need events on the stream) gives the
-    same trees, all equal to the oracle's."""
+    """With CURFIL_LEVEL_GRAPH=1, levels whose pass-1 shape is settled run as one captured graph each
+    (several feature-group and node-slot batches per level here); the same training with every kernel
+    launched on its own (the default, CURFIL_LEVEL_GRAPH=0, and a profiling context, whose phase timers
+    need events on the stream) gives the same trees, all equal to the oracle's."""
     im, ds = small
     params = dict(CONFIG0, maxDepth=8, horizontalFlipping=True)
     forests = {}
