@@ -11,6 +11,8 @@ CIELab and a CUDA device), so these functions are "parity unpinned" beyond the i
   * horizontal flipping (random_tree.h:122-144, 521-527, 1295-1360): root samples are Both and count twice
     in the root histogram; every split's children histograms add up to the parent's; a tree grown with
     flipping differs from one grown without.
+The flipped response itself is pinned by a mathematical identity (the mirror test at the end): copies of
+the oracle that keep the second box's x offset or negate a y offset fail it.
 """
 import numpy as np
 import pytest
@@ -100,3 +102,36 @@ def test_horizontal_flipping_invariants(data):
     pn, ph = plain.tree(0)
     _check_histograms(pn, ph)
     assert len(pn) != len(nodes) or not np.array_equal(pn["feature_id"], nodes["feature_id"])
+
+
+def test_flipped_response_equals_mirrored_image_response():
+    """The flipped response (random_tree_image.h:564-620: both boxes' x offsets negated, y offsets and
+    regions kept) pinned by the mirror identity.  A box of averageRegionColor / averageRegionDepth
+    (random_tree_image.h:288-316) spans the columns x - w + 1 .. x + w between its integral corners, so
+    pixel x of an image corresponds to pixel W - 2 - x of the horizontally mirrored image: the flipped
+    response at (x, y) equals the unflipped response of the mirrored image at (W - 2 - x, y), with the same
+    sample depth, wherever both are finite (a box touching an edge can differ: column 0 never lies inside
+    a box, column W - 1 can).  u8 colour on a small image keeps every colour prefix sum an exact float,
+    and depth sums are integers, so the identity holds bit for bit.  Without the flip it fails."""
+    from helpers import random_features, random_samples, random_u8_dataset
+    rng = np.random.default_rng(77)
+    W, H, n = 64, 48, 2
+    rgb, depth, _ = random_u8_dataset(n, W, H, seed=5)
+    ds = orc.Dataset.from_u8(rgb, depth)
+    dm = orc.Dataset.from_u8(np.ascontiguousarray(rgb[:, :, ::-1]), np.ascontiguousarray(depth[:, :, ::-1]))
+    feats = random_features(rng, 300, box_radius=20, region_size=6)
+    s = random_samples(rng, 3000, n, W - 1, H, depth)   # x <= W - 2
+    s = s[s["depth_mm"] > 0]
+    s["flip"] = 1
+    m = s.copy()
+    m["x"] = W - 2 - s["x"]
+    m["flip"] = 0
+    a, b = ds.responses(s, feats), dm.responses(m, feats)
+    both = np.isfinite(a) & np.isfinite(b)
+    assert both.mean() > 0.3
+    assert np.array_equal(a[both], b[both])
+    s0 = s.copy()
+    s0["flip"] = 0
+    c = ds.responses(s0, feats)
+    both0 = np.isfinite(c) & np.isfinite(b)
+    assert np.mean(c[both0] != b[both0]) > 0.5
