@@ -319,6 +319,33 @@ def measured_traffic(kernel, N, W, H, T):
     return None
 
 
+def classify_roofline(batch_ms, n, W, H):
+    """HBM roofline of the dense-prediction launch: its HBM bytes per launch (FETCH_SIZE and WRITE_SIZE of
+    the committed counter passes, scaled by the copy calibration) over the launch time measured here (the
+    device-output call, which waits for the launch).  The kernel is bound by the vector L1 (its lookups
+    per CU and cycle, from the same file), not by HBM: the fraction is low because the forest's blocks
+    and the frames' planes are served from the caches."""
+    path = os.path.join(ROOT, "profiles", "k_classify_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in t.get("entries", []):
+        w = e.get("workload", {})
+        if (w.get("frames"), w.get("width"), w.get("height"), w.get("trees"), w.get("depth")) == (n, W, H, 10, 20):
+            b = e["bytes_per_launch"]
+            achieved = b / (batch_ms * 1e-3) / 1e9
+            return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": b, "kernel": "k_classify",
+                    "avg_launch_ms": round(batch_ms, 3),
+                    "l1_tag_lookups_per_cu_cycle": e.get("l1_tag_lookups_per_cu_cycle"),
+                    "note": "traffic = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch (profiles/k_classify_traffic.json); "
+                            "the launch is bound by the vector L1 (about one tag lookup per CU and cycle), see "
+                            "profiles/vector_memory_path_counters.md"}
+    return None
+
+
 IO_PALETTE = [(0, 0, 0), (128, 0, 0), (0, 128, 0), (128, 128, 0), (0, 0, 128)]
 
 
@@ -490,6 +517,7 @@ def dense_prediction_leg(ctx):
         ts.append(time.perf_counter() - t1)
     ms = float(np.median(ts)) * 1e3
     out["device_outputs"] = {"batch_ms": ms, "frames_per_s": n / ms * 1e3, "mpix_per_s": n * W * H / ms / 1e3}
+    out["roofline"] = classify_roofline(ms, n, W, H)
     dp.close()
     dl.close()
     lat = []
