@@ -613,7 +613,13 @@ def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(10
                                                 "device_and_copy": float(parts[slow, 2].mean())},
                           "median_parts_ms": {"queued": float(np.median(parts[:, 0])),
                                               "to_launch": float(np.median(parts[:, 1])),
-                                              "device_and_copy": float(np.median(parts[:, 2]))}}
+                                              "device_and_copy": float(np.median(parts[:, 2]))},
+                          "by_size": {f"{sizes[k][0]}x{sizes[k][1]}": {
+                              "frames": int(np.sum(kinds == k)),
+                              "p50_ms": float(np.percentile(lat[kinds == k], 50)),
+                              "p99_ms": float(np.percentile(lat[kinds == k], 99)),
+                              "share_of_p99_tail": float(np.mean(kinds[slow] == k))}
+                              for k in range(3) if np.any(kinds == k)}}
         if not plain:
             log(f"serving {lam} fps offered: {row['serving']['frames_per_s']:.0f} fps, p50 "
                 f"{row['serving']['p50_ms']:.2f} p99 {row['serving']['p99_ms']:.2f} ms")

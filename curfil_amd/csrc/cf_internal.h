@@ -304,16 +304,18 @@ struct DevForest {
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
         const DevForest& forest, float* probs, uint8_t* labels);
 // one frame of a serving batch: its integrals and its outputs ([C][H][W] posteriors, [H][W] labels)
-// Frames of any sizes share a launch: job i's tiles are firstTile .. (job i + 1's firstTile) - 1 of the
-// launch, firstTile ascending from 0 in job order (classifyTiles(W, H) tiles per frame).
+// Frames of any sizes share a launch: job i classifies rows rowBegin .. rowEnd - 1 of its frame (its
+// whole integral image is there) with the tiles firstTile .. (job i + 1's firstTile) - 1 of the launch,
+// firstTile ascending from 0 in job order (classifyTiles(W, rows) tiles per job).
 struct FrameJob {
     DevImage im;
     float* probs;
     uint8_t* labels;
     int firstTile;
+    int rowBegin, rowEnd;
     int pad;
 };
-int classifyTiles(int W, int H);
+int classifyTiles(int W, int rows);
 void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int totalTiles, const DevForest& forest);
 
 }  // namespace cf

@@ -204,14 +204,15 @@ constexpr int kTileH = 64 / kTileW;
 __host__ __device__ __forceinline__ int tileBlocks(int W, int H) {
     return ((W + kTileW - 1) / kTileW) * ((H + 4 * kTileH - 1) / (4 * kTileH));
 }
-// the pixel of this lane in tile `block` of a frame, or -1 outside the frame
-__device__ __forceinline__ int tilePixel(int block, int W, int H) {
+// the pixel of this lane in tile `block` of the rows rowBegin .. rowEnd - 1 of a frame W wide, or -1
+// outside them
+__device__ __forceinline__ int tilePixel(int block, int W, int rowBegin, int rowEnd) {
     constexpr int TW = kTileW, TH = kTileH;
     const int tilesX = (W + TW - 1) / TW;
     const int bx = block % tilesX, by = block / tilesX;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x = bx * TW + lane % TW, y = (by * 4 + wave) * TH + lane / TW;
-    return (x < W && y < H) ? y * W + x : -1;
+    const int x = bx * TW + lane % TW, y = rowBegin + (by * 4 + wave) * TH + lane / TW;
+    return (x < W && y < rowEnd) ? y * W + x : -1;
 }
 
 inline dim3 classifyGrid(int blocksPerFrame, int numFrames) {
@@ -228,7 +229,7 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
     ClassifyTile tile;
     if (!classifyTile(tileBlocks(W, H), numFrames, tile)) return;
     const int frame = tile.frame;
-    const int px = tilePixel(tile.block, W, H);
+    const int px = tilePixel(tile.block, W, 0, H);
     if (px < 0) return;
     const size_t plane = static_cast<size_t>(W) * H;
     classifyPixel<CM>(imgs[frameIdx[frame]], px, forest,
@@ -236,8 +237,9 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
             labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
 }
 
-// independent frames of any sizes, each with its own images and outputs; the launch's tiles are cut into
-// XCD ranges as a batch's are, and a workgroup finds its frame by binary search over the jobs' first tiles
+// independent frames of any sizes (or row bands of them), each with its own images and outputs; the
+// launch's tiles are cut into XCD ranges as a batch's are, and a workgroup finds its job by binary search
+// over the jobs' first tiles
 template<int CM>
 __global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restrict__ jobs, int numJobs, int totalTiles,
         DevForest forest) {
@@ -254,7 +256,7 @@ __global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restric
         }
     }
     const FrameJob job = jobs[lo];
-    const int px = tilePixel(j - job.firstTile, job.im.W, job.im.H);
+    const int px = tilePixel(j - job.firstTile, job.im.W, job.rowBegin, job.rowEnd);
     if (px < 0) return;
     classifyPixel<CM>(job.im, px, forest, job.probs, job.labels);
 }
@@ -282,7 +284,7 @@ void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, in
     });
 }
 
-int classifyTiles(int W, int H) { return tileBlocks(W, H); }
+int classifyTiles(int W, int rows) { return tileBlocks(W, rows); }
 
 void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int totalTiles, const DevForest& forest) {
     if (numJobs == 0) return;

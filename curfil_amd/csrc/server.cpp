@@ -1,9 +1,9 @@
 // Serving loop for dense prediction: RGB-D frames of any resolution arrive
 // one by one (cf_server_submit), are admitted in arrival order while device
 // memory of the budget is free, have their integral images built on an
-// upload stream, are classified in batches (frames of all sizes in one
-// launch) on a compute stream, and return their posteriors and labels
-// through a download stream.
+// upload stream, are classified in launches of bounded size (frames of all
+// sizes together, large frames in row bands) on a compute stream, and return
+// their posteriors and labels through a download stream.
 // One worker thread keeps the three stages busy: frames that arrive while a
 // batch runs are admitted and integrated meanwhile and go into the next
 // batch (continuous batching), and a frame's device slot is reused as soon
@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -103,10 +104,31 @@ struct Request {
 };
 
 constexpr int kJobRing = 64;   // job tables in flight
-// classify work (pixels) allowed in unfinished launches: below it ready frames launch at once
-// (latency), above it they gather into larger batches (throughput).  About 0.5 ms of classify work
-// for the 10-tree depth-20 forest: 6e6 gave the same capacity and p99 with a higher p50.
+// Launch policy, measured with the 10-tree depth-20 forest and Poisson arrivals of 320x240 / 640x480 /
+// 1280x960 frames (p99 at 2000 frames/s and frames served at 3000 offered):
+// - classify work (pixels) allowed in unfinished launches: below it ready frames launch at once
+//   (latency), above it they wait and gather;
+// - a launch holds at most kLaunchPixels, so a burst becomes a pipeline of moderate launches instead of
+//   one large launch and an idle device while the next gathers (6.9 against 7.7-12.3 ms, 2974 against
+//   2490-2540 frames/s; 1.0 Mpix: 6.7 ms but 2894 frames/s; 2.0 Mpix: 8.6 ms);
+// - frames above kBandPixels are classified in row bands of at most that many pixels, so that a
+//   1280x960 frame does not hold up the frames behind it for a whole launch.
+// CURFIL_SERVE_PENDING_MPIX, CURFIL_SERVE_LAUNCH_MPIX (0: no limit) and CURFIL_SERVE_BAND_MPIX (0: no
+// bands) override them when a server is created.
 constexpr double kMaxPendingPixels = 1.5e6;
+constexpr double kLaunchPixels = 1.25e6;
+constexpr double kBandPixels = 0.35e6;
+double envMpix(const char* name, double def) {
+    const char* v = getenv(name);
+    return v ? atof(v) * 1e6 : def;
+}
+
+// a row band of an integrated frame waiting to be classified; the last band's launch completes the frame
+struct Band {
+    Request* r;
+    int rowBegin, rowEnd;
+    bool last;
+};
 
 }  // namespace
 }  // namespace cf
@@ -118,6 +140,8 @@ struct cf_server {
     cf_forest* fo = nullptr;
     size_t budget = 0;
     int maxBatch = 64;
+    double bandPx = 0, launchPx = 0;   // row bands of larger frames (0: none); pixels per launch (0: no limit)
+    double pendingMax = kMaxPendingPixels;
     // three streams (with the context's own, four: the hardware queues a process gets by default)
     hipStream_t up = nullptr, comp = nullptr, down = nullptr;
     PinnedPool pinned;
@@ -202,8 +226,8 @@ struct cf_server {
 void cf_server::run() {
     try {
         CF_HIP(hipSetDevice(ctx->device));
-        std::vector<Request*> ready;     // integrating or integrated, not yet classified
-        std::vector<Request*> inflight;  // classified, results on their way to the host
+        std::vector<Band> ready;         // bands of frames integrating or integrated, not yet classified
+        std::deque<Request*> inflight;   // classified, results on their way to the host, in copy order
         const DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()),
                 fo->C};
         for (;;) {
@@ -241,11 +265,20 @@ void cf_server::run() {
                 launchIntegralU8(up, s->in.get(), reinterpret_cast<const uint16_t*>(s->in.get() + depthOffset(r->W, r->H)),
                         r->W, r->H, s->tmp.get(), s->col.get(), s->dep.get());
                 CF_HIP(hipEventRecord(r->ready, up));
-                ready.push_back(r);
+                // frames above bandPx pixels in bands of whole 16-row tile rows, so that other frames can
+                // be classified between them
+                int rows = r->H;
+                if (bandPx > 0 && static_cast<double>(r->W) * r->H > bandPx) {
+                    rows = std::max(16, static_cast<int>(bandPx / r->W) / 16 * 16);
+                }
+                for (int y0 = 0; y0 < r->H; y0 += rows) {
+                    const int y1 = std::min(r->H, y0 + rows);
+                    ready.push_back(Band{r, y0, y1, y1 == r->H});
+                }
             }
-            // classify what is ready, frames of all sizes together (at most maxBatch frames per launch, in
-            // arrival order), while the unfinished launches hold less than kMaxPendingPixels of work: under
-            // load, ready frames gather into larger launches instead of queueing many small ones
+            // classify what is ready, bands of frames of all sizes together in arrival order (at most maxBatch
+            // bands and launchPx pixels per launch), while the unfinished launches hold less than pendingMax
+            // pixels of work: under load, ready frames gather into larger launches
             double pendingPx = 0;
             while (!launchedPx.empty()) {
                 const hipError_t q = hipEventQuery(jobDone[launchedPx.front().first]);
@@ -254,66 +287,82 @@ void cf_server::run() {
                 launchedPx.pop_front();
             }
             for (const auto& lp : launchedPx) pendingPx += lp.second;
-            if (!ready.empty() && pendingPx < kMaxPendingPixels) {
-                for (size_t b = 0; b < ready.size(); b += maxBatch) {
-                    const int n = static_cast<int>(std::min<size_t>(maxBatch, ready.size() - b));
-                    const int e = jobNext;
-                    jobNext = (jobNext + 1) % kJobRing;
-                    CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
-                    hipStream_t cs = comp;
-                    FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
-                    int tiles = 0;
-                    double px = 0;
-                    for (int i = 0; i < n; i++) {
-                        Request* r = ready[b + i];
-                        Slot* s = r->slot;
-                        hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
-                        hj[i].probs = s->probs.get();
-                        hj[i].labels = s->labels.get();
-                        hj[i].firstTile = tiles;
-                        hj[i].pad = 0;
-                        const int t = classifyTiles(r->W, r->H);
-                        if (tiles > INT32_MAX - t) throw Error(CF_ERR_UNSUPPORTED, "too many pixels in one launch");
-                        tiles += t;
-                        px += static_cast<double>(r->W) * r->H;
-                        CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
-                    }
-                    FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
-                    CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
-                    launchClassifyJobs(cs, dj, n, tiles, df);
-                    CF_HIP(hipEventRecord(jobDone[e], cs));
-                    launchedPx.emplace_back(e, px);
-                    launches++;
-                    CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
-                    // results of the launch's frames to the host, smallest first (a large frame's copy does
-                    // not hold up the small ones finished with it)
-                    std::vector<Request*> order(ready.begin() + b, ready.begin() + b + n);
-                    std::stable_sort(order.begin(), order.end(), [](const Request* x, const Request* y) {
-                        return static_cast<int64_t>(x->W) * x->H < static_cast<int64_t>(y->W) * y->H;
-                    });
-                    for (Request* r : order) {
-                        Slot* s = r->slot;
-                        const size_t rpx = static_cast<size_t>(r->W) * r->H;
-                        CF_HIP(hipMemcpyAsync(r->hostProbs, s->probs.get(), rpx * fo->C * sizeof(float),
-                                hipMemcpyDeviceToHost, down));
-                        CF_HIP(hipMemcpyAsync(r->hostLabels, s->labels.get(), rpx, hipMemcpyDeviceToHost, down));
-                        CF_HIP(hipEventRecord(r->done, down));
-                        r->launched = Clock::now();
-                        inflight.push_back(r);
-                    }
+            size_t next = 0;
+            while (next < ready.size() && pendingPx < pendingMax) {
+                // the next launch: bands in arrival order, up to maxBatch of them and launchPx pixels
+                size_t end = next;
+                double px = 0;
+                while (end < ready.size() && end - next < static_cast<size_t>(maxBatch)) {
+                    const Band& bd = ready[end];
+                    const double bpx = static_cast<double>(bd.r->W) * (bd.rowEnd - bd.rowBegin);
+                    if (end > next && launchPx > 0 && px + bpx > launchPx) break;
+                    px += bpx;
+                    end++;
                 }
-                ready.clear();
+                const int n = static_cast<int>(end - next);
+                const int e = jobNext;
+                jobNext = (jobNext + 1) % kJobRing;
+                CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
+                hipStream_t cs = comp;
+                FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
+                int tiles = 0;
+                for (int i = 0; i < n; i++) {
+                    const Band& bd = ready[next + i];
+                    Request* r = bd.r;
+                    Slot* s = r->slot;
+                    hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
+                    hj[i].probs = s->probs.get();
+                    hj[i].labels = s->labels.get();
+                    hj[i].firstTile = tiles;
+                    hj[i].rowBegin = bd.rowBegin;
+                    hj[i].rowEnd = bd.rowEnd;
+                    hj[i].pad = 0;
+                    const int t = classifyTiles(r->W, bd.rowEnd - bd.rowBegin);
+                    if (tiles > INT32_MAX - t) throw Error(CF_ERR_UNSUPPORTED, "too many pixels in one launch");
+                    tiles += t;
+                    if (bd.rowBegin == 0) CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
+                }
+                FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
+                CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
+                launchClassifyJobs(cs, dj, n, tiles, df);
+                CF_HIP(hipEventRecord(jobDone[e], cs));
+                launchedPx.emplace_back(e, px);
+                pendingPx += px;
+                launches++;
+                // results of the frames whose last band this launch holds (the earlier bands ran in earlier
+                // launches on the same stream), smallest first: a large frame's copy does not hold up the small
+                // ones finished with it
+                std::vector<Request*> order;
+                for (int i = 0; i < n; i++) {
+                    if (ready[next + i].last) order.push_back(ready[next + i].r);
+                }
+                if (!order.empty()) CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
+                std::stable_sort(order.begin(), order.end(), [](const Request* x, const Request* y) {
+                    return static_cast<int64_t>(x->W) * x->H < static_cast<int64_t>(y->W) * y->H;
+                });
+                for (Request* r : order) {
+                    Slot* s = r->slot;
+                    const size_t rpx = static_cast<size_t>(r->W) * r->H;
+                    CF_HIP(hipMemcpyAsync(r->hostProbs, s->probs.get(), rpx * fo->C * sizeof(float),
+                            hipMemcpyDeviceToHost, down));
+                    CF_HIP(hipMemcpyAsync(r->hostLabels, s->labels.get(), rpx, hipMemcpyDeviceToHost, down));
+                    CF_HIP(hipEventRecord(r->done, down));
+                    r->launched = Clock::now();
+                    inflight.push_back(r);
+                }
+                next = end;
             }
-            // finished frames: results are in pinned host memory; their slots are free again
-            std::vector<Request*> still;
+            ready.erase(ready.begin(), ready.begin() + next);
+            // finished frames: results are in pinned host memory; their slots are free again.  The copies run
+            // in order on one stream, so the first unfinished frame ends the scan (one query per loop pass
+            // however many frames are in flight)
             bool any = false;
-            for (Request* r : inflight) {
+            while (!inflight.empty()) {
+                Request* r = inflight.front();
                 const hipError_t q = hipEventQuery(r->done);
-                if (q == hipErrorNotReady) {
-                    still.push_back(r);
-                    continue;
-                }
+                if (q == hipErrorNotReady) break;
                 CF_HIP(q);
+                inflight.pop_front();
                 std::lock_guard<std::mutex> g(mu);
                 r->completed = Clock::now();
                 r->finished = true;
@@ -326,7 +375,6 @@ void cf_server::run() {
                 pixelsDone += static_cast<double>(r->W) * r->H;
                 any = true;
             }
-            inflight.swap(still);
             if (any) cvDone.notify_all();
         }
     } catch (const std::exception& e) {
@@ -352,6 +400,9 @@ int cf_server_create(cf_ctx* ctx, cf_forest* fo, double budget_mb, int max_batch
         sv->fo = fo;
         sv->budget = static_cast<size_t>(budget_mb * 1024.0 * 1024.0);
         sv->maxBatch = max_batch;
+        sv->bandPx = envMpix("CURFIL_SERVE_BAND_MPIX", kBandPixels);
+        sv->launchPx = envMpix("CURFIL_SERVE_LAUNCH_MPIX", kLaunchPixels);
+        sv->pendingMax = envMpix("CURFIL_SERVE_PENDING_MPIX", kMaxPendingPixels);
         bindForest(fo, ctx);
         uploadForest(fo);
         CF_HIP(hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking));
