@@ -325,11 +325,13 @@ def measured_traffic(kernel, N, W, H, T):
 
 
 def classify_roofline(batch_ms, n, W, H):
-    """HBM roofline of the dense-prediction launch: its HBM bytes per launch (FETCH_SIZE and WRITE_SIZE of
-    the committed counter passes, scaled by the copy calibration) over the launch time measured here (the
-    device-output call, which waits for the launch).  The kernel is bound by the vector L1 (its lookups
-    per CU and cycle, from the same file), not by HBM: the fraction is low because the forest's blocks
-    and the frames' planes are served from the caches."""
+    """HBM roofline of the dense-prediction launch.  achieved = its algorithmic bytes per launch (each
+    pixel's integral planes read and its outputs written once, 41 B, plus the forest read once) over the
+    launch time measured here (the device-output call, which waits for the launch); traffic = its HBM
+    bytes per launch (FETCH_SIZE and WRITE_SIZE of the committed counter passes, scaled by the copy
+    calibration).  The kernel is bound by the vector L1 (its lookups per CU and cycle, from the same file),
+    not by HBM: the fraction is low because the forest's blocks and the frames' planes are served from the
+    caches."""
     path = os.path.join(ROOT, "profiles", "k_classify_traffic.json")
     try:
         with open(path) as f:
@@ -339,13 +341,14 @@ def classify_roofline(batch_ms, n, W, H):
     for e in t.get("entries", []):
         w = e.get("workload", {})
         if (w.get("frames"), w.get("width"), w.get("height"), w.get("trees"), w.get("depth")) == (n, W, H, 10, 20):
-            b = e["bytes_per_launch"]
-            achieved = b / (batch_ms * 1e-3) / 1e9
+            alg = e["algorithmic_bytes_per_launch"]
+            achieved = alg / (batch_ms * 1e-3) / 1e9
             return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": b, "kernel": "k_classify",
-                    "avg_launch_ms": round(batch_ms, 3),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": e["bytes_per_launch"],
+                    "algorithmic_bytes": alg, "kernel": "k_classify", "avg_launch_ms": round(batch_ms, 3),
                     "l1_tag_lookups_per_cu_cycle": e.get("l1_tag_lookups_per_cu_cycle"),
-                    "note": "traffic = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch (profiles/k_classify_traffic.json); "
+                    "note": "achieved = algorithmic bytes (41 B per pixel + the forest once) per launch; traffic = "
+                            "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch (profiles/k_classify_traffic.json); "
                             "the launch is bound by the vector L1 (about one tag lookup per CU and cycle), see "
                             "profiles/vector_memory_path_counters.md"}
     return None

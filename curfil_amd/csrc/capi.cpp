@@ -878,8 +878,7 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         if (probs) fo->dProbs.resize(static_cast<size_t>(count) * fo->C * plane);
         if (labels) fo->dLabels.resize(static_cast<size_t>(count) * plane);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(),
-                static_cast<int>(fo->trees.size()), fo->C};
+        const DevForest df = fo->devForest();
         launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df,
                 probs ? fo->dProbs.get() : nullptr, labels ? fo->dLabels.get() : nullptr);
         if (probs) {
@@ -904,7 +903,7 @@ int cf_predict_batch_device(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int
         uploadForest(fo);
         fo->dFrames.resize(count);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()), fo->C};
+        const DevForest df = fo->devForest();
         launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df, d_probs, d_labels);
         CF_HIP(hipStreamSynchronize(ctx->stream));
     });

@@ -89,6 +89,10 @@ struct cf_forest {
     cf::DevBuf<float> dProbs;
     cf::DevBuf<uint8_t> dLabels;
     cf::TrainStats stats;
+    // the uploaded forest as the prediction kernels read it
+    cf::DevForest devForest() const {
+        return cf::DevForest{dBlocks.get(), dPost.get(), dRoots.get(), static_cast<int>(trees.size()), C};
+    }
 };
 
 namespace cf {

@@ -18,6 +18,39 @@
 
 namespace cf {
 
+// Loads and stores through address-space-qualified pointers.  The image planes, the forest and a serving
+// job's outputs are reached through pointers read from memory (frame descriptors, the forest's fields,
+// the jobs): through generic pointers their accesses are flat instructions, which count on both wait
+// counters and take a 64-bit address per lane; through global ones a plane's gathers take its base in
+// scalar registers and a 32-bit offset per lane (measured 2.5% faster on 64-frame batches of 640x480).
+// Vector values are loaded as the compiler's own vector types: the HIP vector classes' copy constructors
+// take generic references, through which the address space is lost.
+#define CF_GLOBAL __attribute__((address_space(1)))
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// element i of a depth integral plane / a colour integral plane (a plane is far below 4 GiB)
+__device__ __forceinline__ int2 depthAt(const int2* plane, int i) {
+    const uint32_t off = static_cast<uint32_t>(i) * 8u;
+    const i32x2 v = *(const CF_GLOBAL i32x2*)((const CF_GLOBAL char*)plane + off);
+    return make_int2(v.x, v.y);
+}
+__device__ __forceinline__ float colorAt(const float* plane, int i) {
+    const uint32_t off = static_cast<uint32_t>(i) * 4u;
+    return *(const CF_GLOBAL float*)((const CF_GLOBAL char*)plane + off);
+}
+// record s of block b of the forest
+__device__ __forceinline__ uint4 blockSlot(const NodeBlock* blocks, int b, int s) {
+    const u32x4 v = ((const CF_GLOBAL u32x4*)blocks)[static_cast<size_t>(b) * 8 + s];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+// the k-th pair of doubles from row (16-byte aligned)
+__device__ __forceinline__ double2 pairAt(const double* row, int k) {
+    const f64x2 v = ((const CF_GLOBAL f64x2*)row)[k];
+    return make_double2(v.x, v.y);
+}
+
 // A block's node record {threshold bits, o1x o1y o2x o2y, r1x r1y r2x r2y, type c1 c2 -} as a feature.
 __device__ __forceinline__ Feat unpackNode(const uint4& w) {
     Feat f;
@@ -39,7 +72,7 @@ __device__ __forceinline__ Feat unpackNode(const uint4& w) {
 // the depth in mm when the pixel's depth is valid.
 __device__ __forceinline__ bool pixelDepthAt(const DevImage& im, int x, int y, int32_t& depth) {
     const int W = im.W;
-    auto at = [&](int xx, int yy) -> int2 { return im.dep[yy * W + xx]; };
+    auto at = [&](int xx, int yy) -> int2 { return depthAt(im.dep, yy * W + xx); };
     const int2 c00 = at(x, y);
     const int2 cl = x > 0 ? at(x - 1, y) : make_int2(0, 0);
     const int2 ca = y > 0 ? at(x, y - 1) : make_int2(0, 0);
@@ -63,11 +96,11 @@ __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool d
     if (root < 0) return -root - 1;
     const int W = im.W, H = im.H;
     const size_t plane = static_cast<size_t>(W) * H;
-    const NodeBlock* blk = forest.blocks + root;
-    uint4 tail = blk->slot[7];
+    int b = root;
+    uint4 tail = blockSlot(forest.blocks, b, 7);
     int p = 0;
     for (;;) {
-        const uint4 rec = blk->slot[p];
+        const uint4 rec = blockSlot(forest.blocks, b, p);
         // the record as one 16-byte load: the compiler would split off the offsets and regions into the
         // branch that uses them, and every load instruction costs a tag lookup per line its lanes touch
         __asm__ volatile("" : : "v"(rec.x), "v"(rec.y), "v"(rec.z), "v"(rec.w));
@@ -81,7 +114,7 @@ __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool d
         if (ok && isDepth) {
             int2 dv[8];
 #pragma unroll
-            for (int c = 0; c < 8; c++) dv[c] = planeAt(im.dep, idx[c]);
+            for (int c = 0; c < 8; c++) dv[c] = depthAt(im.dep, idx[c]);
             // all 8 corners are needed here: the compiler may not split a corner's two halves and sink the
             // sums behind the count check (that doubles the loads and serialises them)
 #pragma unroll
@@ -92,9 +125,9 @@ __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool d
             const float* colB = im.col + static_cast<size_t>(f.c2) * plane;
             float cv[8];
 #pragma unroll
-            for (int c = 0; c < 4; c++) cv[c] = planeAt(colA, idx[c]);
+            for (int c = 0; c < 4; c++) cv[c] = colorAt(colA, idx[c]);
 #pragma unroll
-            for (int c = 4; c < 8; c++) cv[c] = planeAt(colB, idx[c]);
+            for (int c = 4; c < 8; c++) cv[c] = colorAt(colB, idx[c]);
             r = colorResponseFromCorners(cv);
         }
         const float thr = __uint_as_float(rec.x);
@@ -105,8 +138,8 @@ __device__ __forceinline__ int walkTree(const DevImage& im, int x, int y, bool d
             p = q;
             continue;
         }
-        blk = forest.blocks + (tail.x + __popc(tail.w & 0xffffu & below));
-        tail = blk->slot[7];
+        b = static_cast<int>(tail.x + __popc(tail.w & 0xffffu & below));
+        tail = blockSlot(forest.blocks, b, 7);
         p = 0;
     }
 }
@@ -127,14 +160,14 @@ __device__ __forceinline__ void finishPixel(const float (&acc)[CM], int C, size_
     for (int c = 0; c < CM; c++) {
         if (c < C) {
             const float p = static_cast<float>(static_cast<double>(acc[c]) / sum);
-            if (pout) pout[c * plane + px] = p;
+            if (pout) ((CF_GLOBAL float*)pout)[c * plane + px] = p;
             if (p > bestProb) {
                 label = c;
                 bestProb = p;
             }
         }
     }
-    if (lout) lout[px] = static_cast<uint8_t>(label);
+    if (lout) ((CF_GLOBAL uint8_t*)lout)[px] = static_cast<uint8_t>(label);
 }
 
 // One thread per pixel, the trees one after another (walking several trees in lockstep per thread
@@ -156,10 +189,10 @@ __device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const 
         const int leaf = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
         // the leaf's row (postStride(C) doubles, 16-byte aligned) in 16-byte loads that go out together;
         // pairs past the row repeat its last pair (in bounds, unused)
-        const double2* row = reinterpret_cast<const double2*>(forest.post + static_cast<size_t>(leaf) * postStride(C));
+        const double* row = forest.post + static_cast<size_t>(leaf) * postStride(C);
         double2 pair[CM / 2];
 #pragma unroll
-        for (int k = 0; k < CM / 2; k++) pair[k] = row[k < (C + 1) / 2 ? k : (C + 1) / 2 - 1];
+        for (int k = 0; k < CM / 2; k++) pair[k] = pairAt(row, k < (C + 1) / 2 ? k : (C + 1) / 2 - 1);
         // whole pairs, all issued before the first is summed
 #pragma unroll
         for (int k = 0; k < CM / 2; k++) __asm__ volatile("" : : "v"(pair[k].x), "v"(pair[k].y));

@@ -81,8 +81,7 @@ extern "C" int cf_predict_batch_device_noalloc(cf_ctx* ctx, cf_forest* fo, cf_im
         }
         CF_HIP(hipMemcpyAsync(frames, indices, static_cast<size_t>(count) * sizeof(int), hipMemcpyHostToDevice,
                 ctx->stream));
-        const DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()),
-                fo->C};
+        const DevForest df = fo->devForest();
         launchClassify(ctx->stream, im->desc.get(), frames, count, im->W, im->H, df, d_probs, d_labels);
         CF_HIP(hipStreamSynchronize(ctx->stream));
         ctx->error.clear();

@@ -228,8 +228,7 @@ void cf_server::run() {
         CF_HIP(hipSetDevice(ctx->device));
         std::vector<Band> ready;         // bands of frames integrating or integrated, not yet classified
         std::deque<Request*> inflight;   // classified, results on their way to the host, in copy order
-        const DevForest df{fo->dBlocks.get(), fo->dPost.get(), fo->dRoots.get(), static_cast<int>(fo->trees.size()),
-                fo->C};
+        const DevForest df = fo->devForest();
         for (;;) {
             std::vector<Request*> admit;
             {
