@@ -545,6 +545,15 @@ def dense_prediction_leg(ctx):
     return out, forest
 
 
+def worker_stats(before, after):
+    """The serving worker's passes and allocations between two stats() readings (the longest pass is the
+    longest since the server started)."""
+    keys = ("passes", "passes_over_1ms", "slot_allocs", "pinned_allocs", "event_allocs")
+    out = {k: int(after[k] - before[k]) for k in keys}
+    out["longest_pass_ms"] = round(after["longest_pass_ms"], 3)
+    return out
+
+
 def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(1000, 2000, 3000, 4000)):
     """Poisson arrivals of RGB-D frames (by default 320x240 40%, 640x480 50%, 1280x960 10%) at 1000-4000
     frames/s into the serving loop (8 GB budget), against the plain path (one frame at a time, in arrival
@@ -584,7 +593,7 @@ def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(10
         picks = rng.integers(0, 8, n)
         row = {"rate_fps": lam, "frames": n}
         # serving loop
-        launches0 = server.stats()["launches"]
+        st0 = server.stats()
         ids = [None] * n
         lat = np.zeros(n)
 
@@ -610,7 +619,8 @@ def serving_leg(ctx, forest, seconds, plain=True, mix=(0.4, 0.5, 0.1), rates=(10
         px = float(sum(sizes[k][0] * sizes[k][1] for k in kinds))
         row["serving"] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                           "frames_per_s": n / wall, "mpix_per_s": px / wall / 1e6,
-                          "launches": server.stats()["launches"] - launches0,
+                          "launches": server.stats()["launches"] - st0["launches"],
+                          "worker": worker_stats(st0, server.stats()),
                           "p99_tail_parts_ms": {"queued": float(parts[slow, 0].mean()),
                                                 "to_launch": float(parts[slow, 1].mean()),
                                                 "device_and_copy": float(parts[slow, 2].mean())},

@@ -897,10 +897,13 @@ class Server:
         self.last_breakdown = lat[1:].copy()
         return lab, probs, float(lat[0])
 
+    STATS_KEYS = ("frames", "pixels", "launches", "device_mb", "queued", "passes", "longest_pass_ms",
+                  "passes_over_1ms", "slot_allocs", "pinned_allocs", "event_allocs")
+
     def stats(self):
-        out = np.zeros(5, np.float64)
-        lib.cf_server_stats(self.h, _ptr(out), 5)
-        return dict(zip(("frames", "pixels", "launches", "device_mb", "queued"), out.tolist()))
+        out = np.zeros(len(self.STATS_KEYS), np.float64)
+        lib.cf_server_stats(self.h, _ptr(out), len(out))
+        return dict(zip(self.STATS_KEYS, out.tolist()))
 
     def close(self):
         if getattr(self, "h", None):

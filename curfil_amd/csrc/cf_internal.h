@@ -306,7 +306,7 @@ void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, in
 // one frame of a serving batch: its integrals and its outputs ([C][H][W] posteriors, [H][W] labels)
 // Frames of any sizes share a launch: job i classifies rows rowBegin .. rowEnd - 1 of its frame (its
 // whole integral image is there) with the tiles firstTile .. (job i + 1's firstTile) - 1 of the launch,
-// firstTile ascending from 0 in job order (classifyTiles(W, rows) tiles per job).
+// firstTile ascending from 0 in job order (set by planClassifyJobs).
 struct FrameJob {
     DevImage im;
     float* probs;
@@ -315,7 +315,14 @@ struct FrameJob {
     int rowBegin, rowEnd;
     int pad;
 };
-int classifyTiles(int W, int rows);
-void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int totalTiles, const DevForest& forest);
+// the shape of a launch over jobs: the tree-split kernel for few pixels, and the launch's tiles
+struct ClassifyPlan {
+    bool split = false;
+    int totalTiles = 0;
+};
+// chooses the shape by the jobs' pixels and sets their firstTile (host copies of the jobs)
+ClassifyPlan planClassifyJobs(FrameJob* jobs, int numJobs, const DevForest& forest);
+void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, const ClassifyPlan& plan,
+        const DevForest& forest);
 
 }  // namespace cf

@@ -12,6 +12,7 @@
 // forest holds split nodes only, in 128-byte blocks of up to three levels
 // (NodeBlock): a walk reads one line per three levels, and ends at the last
 // split with the leaf's number from its block, without a leaf record.
+#include <cstdlib>
 #include <type_traits>
 
 #include "cf_internal.h"
@@ -170,6 +171,37 @@ __device__ __forceinline__ void finishPixel(const float (&acc)[CM], int C, size_
     if (lout) ((CF_GLOBAL uint8_t*)lout)[px] = static_cast<uint8_t>(label);
 }
 
+// Adds the posterior rows of leaves[0 .. n - 1] (n <= K, the trees in order) into the float
+// accumulator.  A row (postStride(C) doubles, 16-byte aligned) is read in 16-byte pairs; all pairs of the
+// K rows go out before the first is summed; pairs past a row repeat its last pair and rows past n repeat
+// row 0 (in bounds, unused).
+template<int CM, int K>
+__device__ __forceinline__ void addLeaves(float (&acc)[CM], const DevForest& forest, const int* leaves, int n) {
+    const int C = forest.C;
+    double2 pair[K][CM / 2];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const double* row = forest.post + static_cast<size_t>(leaves[j < n ? j : 0]) * postStride(C);
+#pragma unroll
+        for (int k = 0; k < CM / 2; k++) pair[j][k] = pairAt(row, k < (C + 1) / 2 ? k : (C + 1) / 2 - 1);
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+#pragma unroll
+        for (int k = 0; k < CM / 2; k++) __asm__ volatile("" : : "v"(pair[j][k].x), "v"(pair[j][k].y));
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        if (j < n) {
+#pragma unroll
+            for (int c = 0; c < CM; c++) {
+                const double v = (c & 1) ? pair[j][c / 2].y : pair[j][c / 2].x;
+                if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + v);
+            }
+        }
+    }
+}
+
 // One thread per pixel, the trees one after another (walking several trees in lockstep per thread
 // was measured slower): per tree, the leaf's double posterior is added into the float accumulator.
 template<int CM>
@@ -187,25 +219,7 @@ __device__ __forceinline__ void classifyPixel(const DevImage& im, int px, const 
     for (int c = 0; c < CM; c++) acc[c] = 0.0f;
     for (int t = 0; t < forest.numTrees; t++) {
         const int leaf = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
-        // the leaf's row (postStride(C) doubles, 16-byte aligned) in 16-byte loads that go out together;
-        // pairs past the row repeat its last pair (in bounds, unused)
-        const double* row = forest.post + static_cast<size_t>(leaf) * postStride(C);
-        double2 pair[CM / 2];
-#pragma unroll
-        for (int k = 0; k < CM / 2; k++) pair[k] = pairAt(row, k < (C + 1) / 2 ? k : (C + 1) / 2 - 1);
-        // whole pairs, all issued before the first is summed
-#pragma unroll
-        for (int k = 0; k < CM / 2; k++) __asm__ volatile("" : : "v"(pair[k].x), "v"(pair[k].y));
-        double pv[CM];
-#pragma unroll
-        for (int k = 0; k < CM / 2; k++) {
-            pv[2 * k] = pair[k].x;
-            pv[2 * k + 1] = pair[k].y;
-        }
-#pragma unroll
-        for (int c = 0; c < CM; c++) {
-            if (c < C) acc[c] = static_cast<float>(static_cast<double>(acc[c]) + pv[c]);
-        }
+        addLeaves<CM, 1>(acc, forest, &leaf, 1);
     }
     finishPixel<CM>(acc, C, plane, px, pout, lout);
 }
@@ -270,6 +284,20 @@ __global__ void __launch_bounds__(256) k_classify(const DevImage* __restrict__ i
             labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
 }
 
+// the job of tile j of a launch: the last job whose first tile is <= j
+__device__ __forceinline__ int findJob(const FrameJob* __restrict__ jobs, int numJobs, int j) {
+    int lo = 0, hi = numJobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].firstTile <= j) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
 // independent frames of any sizes (or row bands of them), each with its own images and outputs; the
 // launch's tiles are cut into XCD ranges as a batch's are, and a workgroup finds its job by binary search
 // over the jobs' first tiles
@@ -279,19 +307,77 @@ __global__ void __launch_bounds__(256) k_classify_jobs(const FrameJob* __restric
     ClassifyTile tile;
     if (!classifyTile(totalTiles, 1, tile)) return;
     const int j = tile.block;
-    int lo = 0, hi = numJobs - 1;   // the last job whose first tile is <= j
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].firstTile <= j) {
-            lo = mid;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    const FrameJob job = jobs[lo];
+    const FrameJob job = jobs[findJob(jobs, numJobs, j)];
     const int px = tilePixel(j - job.firstTile, job.im.W, job.rowBegin, job.rowEnd);
     if (px < 0) return;
     classifyPixel<CM>(job.im, px, forest, job.probs, job.labels);
+}
+
+// Few pixels (a single frame, a few small frames): with one thread per pixel most SIMDs hold a single
+// wave, and a wave's time is its walk of all trees one after another.  Here a workgroup takes one 16 x 4
+// tile and its waves walk the trees t = w, w + kSplitWaves, ... (w the wave); the leaf numbers meet in
+// LDS and wave 0 adds the posteriors in tree order, the same float sums as one thread walking all trees.
+constexpr int kSplitWaves = 4;
+
+__host__ __device__ __forceinline__ int splitTiles(int W, int rows) {
+    return ((W + kTileW - 1) / kTileW) * ((rows + kTileH - 1) / kTileH);
+}
+
+template<int CM>
+__device__ __forceinline__ void classifyTileSplit(const DevImage& im, int tile, int rowBegin, int rowEnd,
+        const DevForest& forest, float* __restrict__ pout, uint8_t* __restrict__ lout) {
+    extern __shared__ int leafLds[];   // [numTrees][64]
+    const int W = im.W;
+    const int tilesX = (W + kTileW - 1) / kTileW;
+    const int bx = tile % tilesX, by = tile / tilesX;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x = bx * kTileW + lane % kTileW, y = rowBegin + by * kTileH + lane / kTileW;
+    const bool in = x < W && y < rowEnd;
+    const int T = forest.numTrees;
+    if (in) {
+        int32_t depth;
+        const bool depthOk = pixelDepthAt(im, x, y, depth);
+        const double rinv = depthOk ? depthReciprocal(depthToFloat(depth)) : 0.0;
+        for (int t = wave; t < T; t += kSplitWaves) {
+            leafLds[t * 64 + lane] = walkTree(im, x, y, depthOk, rinv, forest, forest.treeRoot[t]);
+        }
+    }
+    __syncthreads();
+    if (wave != 0 || !in) return;
+    constexpr int K = CM / 2 >= 8 ? 1 : 8 / (CM / 2);   // leaf rows read together
+    float acc[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) acc[c] = 0.0f;
+    for (int t0 = 0; t0 < T; t0 += K) {
+        int leaves[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) leaves[j] = leafLds[(t0 + j < T ? t0 + j : t0) * 64 + lane];
+        addLeaves<CM, K>(acc, forest, leaves, T - t0 < K ? T - t0 : K);
+    }
+    finishPixel<CM>(acc, forest.C, static_cast<size_t>(W) * im.H, y * W + x, pout, lout);
+}
+
+template<int CM>
+__global__ void __launch_bounds__(256) k_classify_split(const DevImage* __restrict__ imgs,
+        const int* __restrict__ frameIdx, int numFrames, int W, int H, DevForest forest, float* __restrict__ probs,
+        uint8_t* __restrict__ labels) {
+    ClassifyTile tile;
+    if (!classifyTile(splitTiles(W, H), numFrames, tile)) return;
+    const int frame = tile.frame;
+    const size_t plane = static_cast<size_t>(W) * H;
+    classifyTileSplit<CM>(imgs[frameIdx[frame]], tile.block, 0, H, forest,
+            probs ? probs + static_cast<size_t>(frame) * forest.C * plane : nullptr,
+            labels ? labels + static_cast<size_t>(frame) * plane : nullptr);
+}
+
+template<int CM>
+__global__ void __launch_bounds__(256) k_classify_jobs_split(const FrameJob* __restrict__ jobs, int numJobs,
+        int totalTiles, DevForest forest) {
+    ClassifyTile tile;
+    if (!classifyTile(totalTiles, 1, tile)) return;
+    const int j = tile.block;
+    const FrameJob job = jobs[findJob(jobs, numJobs, j)];
+    classifyTileSplit<CM>(job.im, j - job.firstTile, job.rowBegin, job.rowEnd, forest, job.probs, job.labels);
 }
 
 // the smallest instance whose class count CM covers C
@@ -308,8 +394,37 @@ void dispatchClasses(int C, Launch&& launch) {
     }
 }
 
+// Launches of at most kSplitPixels pixels (CURFIL_CLASSIFY_SPLIT_MPIX overrides it; 0: never) split each
+// tile's trees over the waves of a workgroup, given at least kSplitWaves trees and at most 128 (their
+// leaf numbers, 256 bytes per tree, fill the workgroup's LDS).  Measured with 10 trees of depth 20: one
+// 640x480 frame 0.80 against 0.89 ms to the host; 64 frames (19.7 Mpix) 2.7% slower split.  The bound
+// covers every launch of the serving loop (at most 1.25 Mpix), which served the 320x240 / 640x480 /
+// 1280x960 mix at 3000 frames/s with p99 12 ms instead of 30 ms; a bound of 0.5 or 0.8 Mpix, which
+// leaves its larger launches to the other kernel, did worse than no split at all.
+constexpr double kSplitPixels = 1.3e6;
+constexpr int kSplitMaxTrees = 128;
+static bool splitLaunch(double pixels, const DevForest& forest) {
+    static const double maxPx = [] {
+        const char* e = std::getenv("CURFIL_CLASSIFY_SPLIT_MPIX");
+        return e ? std::atof(e) * 1e6 : kSplitPixels;
+    }();
+    return pixels <= maxPx && forest.numTrees >= kSplitWaves && forest.numTrees <= kSplitMaxTrees;
+}
+static size_t splitLds(const DevForest& forest) {
+    return static_cast<size_t>(forest.numTrees) * 64 * sizeof(int);
+}
+
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
         const DevForest& forest, float* probs, uint8_t* labels) {
+    const double pixels = static_cast<double>(numFrames) * W * H;
+    if (splitLaunch(pixels, forest)) {
+        const dim3 grid = classifyGrid(splitTiles(W, H), numFrames);
+        dispatchClasses(forest.C, [&](auto cm) {
+            hipLaunchKernelGGL(k_classify_split<decltype(cm)::value>, grid, dim3(64 * kSplitWaves), splitLds(forest),
+                    s, imgs, frameIdx, numFrames, W, H, forest, probs, labels);
+        });
+        return;
+    }
     const dim3 grid = classifyGrid(tileBlocks(W, H), numFrames);
     dispatchClasses(forest.C, [&](auto cm) {
         hipLaunchKernelGGL(k_classify<decltype(cm)::value>, grid, dim3(256), 0, s, imgs, frameIdx, numFrames, W, H,
@@ -317,14 +432,35 @@ void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, in
     });
 }
 
-int classifyTiles(int W, int rows) { return tileBlocks(W, rows); }
+ClassifyPlan planClassifyJobs(FrameJob* jobs, int numJobs, const DevForest& forest) {
+    double pixels = 0;
+    for (int i = 0; i < numJobs; i++) pixels += static_cast<double>(jobs[i].im.W) * (jobs[i].rowEnd - jobs[i].rowBegin);
+    ClassifyPlan plan;
+    plan.split = splitLaunch(pixels, forest);
+    int tiles = 0;
+    for (int i = 0; i < numJobs; i++) {
+        const int rows = jobs[i].rowEnd - jobs[i].rowBegin;
+        const int t = plan.split ? splitTiles(jobs[i].im.W, rows) : tileBlocks(jobs[i].im.W, rows);
+        if (tiles > INT32_MAX - t) throw Error(CF_ERR_UNSUPPORTED, "too many pixels in one launch");
+        jobs[i].firstTile = tiles;
+        tiles += t;
+    }
+    plan.totalTiles = tiles;
+    return plan;
+}
 
-void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, int totalTiles, const DevForest& forest) {
+void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, const ClassifyPlan& plan,
+        const DevForest& forest) {
     if (numJobs == 0) return;
-    const dim3 grid = classifyGrid(totalTiles, 1);
+    const dim3 grid = classifyGrid(plan.totalTiles, 1);
     dispatchClasses(forest.C, [&](auto cm) {
-        hipLaunchKernelGGL(k_classify_jobs<decltype(cm)::value>, grid, dim3(256), 0, s, jobs, numJobs, totalTiles,
-                forest);
+        if (plan.split) {
+            hipLaunchKernelGGL(k_classify_jobs_split<decltype(cm)::value>, grid, dim3(64 * kSplitWaves),
+                    splitLds(forest), s, jobs, numJobs, plan.totalTiles, forest);
+        } else {
+            hipLaunchKernelGGL(k_classify_jobs<decltype(cm)::value>, grid, dim3(256), 0, s, jobs, numJobs,
+                    plan.totalTiles, forest);
+        }
     });
 }
 

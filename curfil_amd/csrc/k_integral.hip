@@ -6,7 +6,7 @@
 // 2^53, so its result is the exact sum rounded once to float (colour) or the
 // exact integer (depth, valid).  Here the sums are exact 32-bit integer prefix
 // sums (wrapping for depth, see boxDepth), built in two passes:
-//   pass A: column prefix sums, 32 row segments per column (coalesced along x);
+//   pass A: column prefix sums, 16 row segments per column (coalesced along x);
 //   pass B: one wave per row scans 64-pixel segments with DPP/shuffle scans.
 // Float colour (non-integer values): the reference's sequential recurrence is
 // replayed exactly, one thread per channel (used for small test images).
@@ -15,11 +15,15 @@
 namespace cf {
 
 // pass A: column prefix of 5 channels (r, g, b, depth mm, valid) into tmp[5][H][W].
-// A workgroup takes 32 columns; its 32 x 32 lanes split the rows into 32
+// A workgroup takes 16 columns; its 16 x 16 lanes split the rows into 16
 // segments: each lane sums its segment, the segment totals meet in LDS, and
 // each lane writes its segment's prefix sums starting from the totals of the
 // segments above (integer sums, so the order of additions does not matter).
-constexpr int kColChunk = 32, kRowSegs = 32;
+// Workgroups of 256 lanes: the serving loop integrates frames while the
+// classification of others fills the CUs, and a workgroup of 1024 lanes (16
+// waves) got a CU only when a classification launch drained, so the next
+// launch, which waits for the frame, started late.
+constexpr int kColChunk = 16, kRowSegs = 16;
 
 __global__ void __launch_bounds__(kColChunk * kRowSegs) k_integral_cols_u8(const uint8_t* __restrict__ rgb,
         const uint16_t* __restrict__ depth, int W, int H, uint32_t* __restrict__ tmp) {

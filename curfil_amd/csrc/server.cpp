@@ -13,6 +13,7 @@
 // same kernel body as cf_predict, so a frame's results do not depend on what
 // else is in flight.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdlib>
@@ -53,6 +54,7 @@ public:
         }
         void* p = nullptr;
         CF_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        allocs++;
         return p;
     }
     void give(void* p, size_t bytes) {
@@ -60,6 +62,7 @@ public:
         std::lock_guard<std::mutex> g(mu_);
         free_[bytes].push_back(p);
     }
+    std::atomic<uint64_t> allocs{0};   // buffers allocated (not taken from the pool)
 
 private:
     std::mutex mu_;
@@ -112,12 +115,16 @@ constexpr int kJobRing = 64;   // job tables in flight
 //   one large launch and an idle device while the next gathers (6.9 against 7.7-12.3 ms, 2974 against
 //   2490-2540 frames/s; 1.0 Mpix: 6.7 ms but 2894 frames/s; 2.0 Mpix: 8.6 ms);
 // - frames above kBandPixels are classified in row bands of at most that many pixels, so that a
-//   1280x960 frame does not hold up the frames behind it for a whole launch.
-// CURFIL_SERVE_PENDING_MPIX, CURFIL_SERVE_LAUNCH_MPIX (0: no limit) and CURFIL_SERVE_BAND_MPIX (0: no
-// bands) override them when a server is created.
+//   1280x960 frame does not hold up the frames behind it for a whole launch;
+// - frames admitted and not yet launched hold at most kMaxAdmittedPixels; later frames wait in the host
+//   queue.  Under overload the admitted frames would otherwise fill the device budget, and every
+//   admission of a size without a free slot then gave back other sizes' slots and allocated anew.
+// CURFIL_SERVE_PENDING_MPIX, CURFIL_SERVE_LAUNCH_MPIX (0: no limit), CURFIL_SERVE_BAND_MPIX (0: no
+// bands) and CURFIL_SERVE_ADMIT_MPIX (0: no limit) override them when a server is created.
 constexpr double kMaxPendingPixels = 1.5e6;
 constexpr double kLaunchPixels = 1.25e6;
 constexpr double kBandPixels = 0.35e6;
+constexpr double kMaxAdmittedPixels = 3.0e6;
 double envMpix(const char* name, double def) {
     const char* v = getenv(name);
     return v ? atof(v) * 1e6 : def;
@@ -142,6 +149,7 @@ struct cf_server {
     int maxBatch = 64;
     double bandPx = 0, launchPx = 0;   // row bands of larger frames (0: none); pixels per launch (0: no limit)
     double pendingMax = kMaxPendingPixels;
+    double admitMax = kMaxAdmittedPixels;   // pixels of admitted frames not yet launched (0: no limit)
     // three streams (with the context's own, four: the hardware queues a process gets by default)
     hipStream_t up = nullptr, comp = nullptr, down = nullptr;
     PinnedPool pinned;
@@ -162,9 +170,12 @@ struct cf_server {
     hipEvent_t jobDone[kJobRing] = {};
     int jobNext = 0;
     std::deque<std::pair<int, double>> launchedPx;   // unfinished launches: job entry, pixels
-    // statistics
+    // statistics; a worker pass is one round of admission, launches and the completion scan, without the
+    // waits for work between passes
     uint64_t framesDone = 0, launches = 0;
     double pixelsDone = 0;
+    uint64_t passes = 0, slowPasses = 0, slotAllocs = 0, eventAllocs = 0;
+    double longestPassMs = 0;
     std::thread worker;
 
     hipEvent_t takeEvent() {
@@ -175,6 +186,7 @@ struct cf_server {
         }
         hipEvent_t e;
         CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        eventAllocs++;
         return e;
     }
 
@@ -216,6 +228,7 @@ struct cf_server {
         s->probs.resize(px * fo->C);
         s->labels.resize(px);
         devUsed += bytes;
+        slotAllocs++;
         slots.push_back(std::move(s));
         return slots.back().get();
     }
@@ -229,18 +242,29 @@ void cf_server::run() {
         std::vector<Band> ready;         // bands of frames integrating or integrated, not yet classified
         std::deque<Request*> inflight;   // classified, results on their way to the host, in copy order
         const DevForest df = fo->devForest();
+        double passMs = -1;        // the previous pass
+        double admittedPx = 0;     // pixels of admitted frames whose last band has not been launched
         for (;;) {
             std::vector<Request*> admit;
+            Clock::time_point passStart;
             {
                 std::unique_lock<std::mutex> lk(mu);
+                if (passMs >= 0) {
+                    passes++;
+                    if (passMs > 1.0) slowPasses++;
+                    longestPassMs = std::max(longestPassMs, passMs);
+                }
                 if (queued.empty() && ready.empty() && inflight.empty()) {
                     cvWork.wait(lk, [&] { return stop || !queued.empty(); });
                 } else if (!inflight.empty()) {
                     cvWork.wait_for(lk, std::chrono::microseconds(20));
                 }
+                passStart = Clock::now();
                 if (stop && queued.empty() && ready.empty() && inflight.empty()) break;
                 while (!queued.empty()) {
                     Request* r = queued.front();
+                    const double rpx = static_cast<double>(r->W) * r->H;
+                    if (admitMax > 0 && admittedPx > 0 && admittedPx + rpx > admitMax) break;
                     Slot* s = takeSlot(r->W, r->H);
                     if (!s) break;   // arrival order: later frames wait for this one
                     queued.pop_front();
@@ -248,6 +272,7 @@ void cf_server::run() {
                     r->admitted = Clock::now();
                     r->ready = takeEvent();
                     r->done = takeEvent();
+                    admittedPx += rpx;
                     admit.push_back(r);
                 }
             }
@@ -304,7 +329,6 @@ void cf_server::run() {
                 CF_HIP(hipEventSynchronize(jobDone[e]));   // the entry's previous kernel has run
                 hipStream_t cs = comp;
                 FrameJob* hj = hJobs + static_cast<size_t>(e) * maxBatch;
-                int tiles = 0;
                 for (int i = 0; i < n; i++) {
                     const Band& bd = ready[next + i];
                     Request* r = bd.r;
@@ -312,18 +336,15 @@ void cf_server::run() {
                     hj[i].im = DevImage{s->col.get(), s->dep.get(), r->W, r->H, 0, 0};
                     hj[i].probs = s->probs.get();
                     hj[i].labels = s->labels.get();
-                    hj[i].firstTile = tiles;
                     hj[i].rowBegin = bd.rowBegin;
                     hj[i].rowEnd = bd.rowEnd;
                     hj[i].pad = 0;
-                    const int t = classifyTiles(r->W, bd.rowEnd - bd.rowBegin);
-                    if (tiles > INT32_MAX - t) throw Error(CF_ERR_UNSUPPORTED, "too many pixels in one launch");
-                    tiles += t;
                     if (bd.rowBegin == 0) CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
                 }
+                const ClassifyPlan plan = planClassifyJobs(hj, n, df);
                 FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
                 CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
-                launchClassifyJobs(cs, dj, n, tiles, df);
+                launchClassifyJobs(cs, dj, n, plan, df);
                 CF_HIP(hipEventRecord(jobDone[e], cs));
                 launchedPx.emplace_back(e, px);
                 pendingPx += px;
@@ -333,7 +354,10 @@ void cf_server::run() {
                 // ones finished with it
                 std::vector<Request*> order;
                 for (int i = 0; i < n; i++) {
-                    if (ready[next + i].last) order.push_back(ready[next + i].r);
+                    if (ready[next + i].last) {
+                        order.push_back(ready[next + i].r);
+                        admittedPx -= static_cast<double>(ready[next + i].r->W) * ready[next + i].r->H;
+                    }
                 }
                 if (!order.empty()) CF_HIP(hipStreamWaitEvent(down, jobDone[e], 0));
                 std::stable_sort(order.begin(), order.end(), [](const Request* x, const Request* y) {
@@ -375,6 +399,7 @@ void cf_server::run() {
                 any = true;
             }
             if (any) cvDone.notify_all();
+            passMs = std::chrono::duration<double, std::milli>(Clock::now() - passStart).count();
         }
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> g(mu);
@@ -402,6 +427,7 @@ int cf_server_create(cf_ctx* ctx, cf_forest* fo, double budget_mb, int max_batch
         sv->bandPx = envMpix("CURFIL_SERVE_BAND_MPIX", kBandPixels);
         sv->launchPx = envMpix("CURFIL_SERVE_LAUNCH_MPIX", kLaunchPixels);
         sv->pendingMax = envMpix("CURFIL_SERVE_PENDING_MPIX", kMaxPendingPixels);
+        sv->admitMax = envMpix("CURFIL_SERVE_ADMIT_MPIX", kMaxAdmittedPixels);
         bindForest(fo, ctx);
         uploadForest(fo);
         CF_HIP(hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking));
@@ -497,7 +523,10 @@ int cf_server_stats(cf_server* sv, double* out, int n) {
     if (!sv || !out) return CF_ERR_ARG;
     std::lock_guard<std::mutex> g(sv->mu);
     const double v[] = {static_cast<double>(sv->framesDone), sv->pixelsDone, static_cast<double>(sv->launches),
-            static_cast<double>(sv->devUsed) / (1024.0 * 1024.0), static_cast<double>(sv->queued.size())};
+            static_cast<double>(sv->devUsed) / (1024.0 * 1024.0), static_cast<double>(sv->queued.size()),
+            static_cast<double>(sv->passes), sv->longestPassMs, static_cast<double>(sv->slowPasses),
+            static_cast<double>(sv->slotAllocs), static_cast<double>(sv->pinned.allocs.load()),
+            static_cast<double>(sv->eventAllocs)};
     const int k = static_cast<int>(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < k; i++) out[i] = v[i];
     return k;

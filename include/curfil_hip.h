@@ -322,7 +322,11 @@ int cf_server_submit(cf_server* server, const uint8_t* rgb, const uint16_t* dept
  * - submission, and its parts: waiting for admission, admission to the
  * batch's launch, launch to completion */
 int cf_server_wait(cf_server* server, uint64_t id, float* probs, uint8_t* labels, double* latency_ms);
-/* out[0..4]: frames done, pixels done, classify launches, device MB in use, frames queued */
+/* out[0..10]: frames done, pixels done, classify launches, device MB in use,
+ * frames queued, worker passes (rounds of admission, launches and completion
+ * scan, without the waits for work), the longest pass in ms, passes over
+ * 1 ms, device slots allocated, page-locked host buffers allocated, events
+ * created; returns the number of values the server has (11) */
 int cf_server_stats(cf_server* server, double* out, int n);
 const char* cf_server_error(cf_server* server);
 int cf_server_destroy(cf_server* server);
