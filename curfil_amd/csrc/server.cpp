@@ -455,6 +455,8 @@ int cf_server_create(cf_ctx* ctx, cf_forest* fo, double budget_mb, int max_batch
 
 int cf_server_submit(cf_server* sv, const uint8_t* rgb, const uint16_t* depth, int width, int height,
         uint64_t* id) {
+    // a frame's latency runs from this call, its checks and the copy of its input included
+    const Clock::time_point called = Clock::now();
     if (!sv || !rgb || !depth || !id || width <= 0 || height <= 0 || width > 32767 || height > 32767) {
         return CF_ERR_ARG;
     }
@@ -475,8 +477,11 @@ int cf_server_submit(cf_server* sv, const uint8_t* rgb, const uint16_t* depth, i
         std::memcpy(r->hostIn, rgb, px * 3);
         std::memcpy(static_cast<uint8_t*>(r->hostIn) + depthOffset(width, height), depth, px * 2);
         std::lock_guard<std::mutex> g(sv->mu);
-        if (sv->stop) return CF_ERR_RUNTIME;
-        r->submitted = Clock::now();
+        if (sv->stop) {
+            sv->pinned.give(r->hostIn, inputBytes(width, height));
+            return CF_ERR_RUNTIME;
+        }
+        r->submitted = called;
         r->id = sv->nextId++;
         *id = r->id;
         sv->queued.push_back(r.get());

@@ -319,8 +319,9 @@ int cf_server_submit(cf_server* server, const uint8_t* rgb, const uint16_t* dept
         uint64_t* id);
 /* blocks until frame `id` is done; probs [C][H][W] and labels [H][W] may be
  * NULL; latency_ms[0..3] (may be NULL): completion (results in host memory)
- * - submission, and its parts: waiting for admission, admission to the
- * batch's launch, launch to completion */
+ * - the cf_server_submit call (its input copy included), and its parts:
+ * the call and waiting for admission, admission to the batch's launch,
+ * launch to completion */
 int cf_server_wait(cf_server* server, uint64_t id, float* probs, uint8_t* labels, double* latency_ms);
 /* out[0..10]: frames done, pixels done, classify launches, device MB in use,
  * frames queued, worker passes (rounds of admission, launches and completion
