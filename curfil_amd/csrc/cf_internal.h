@@ -320,8 +320,11 @@ struct ClassifyPlan {
     bool split = false;
     int totalTiles = 0;
 };
-// chooses the shape by the jobs' pixels and sets their firstTile (host copies of the jobs)
-ClassifyPlan planClassifyJobs(FrameJob* jobs, int numJobs, const DevForest& forest);
+// chooses the shape by the jobs' pixels (the tree-split kernel up to splitMaxPixels) and sets their firstTile
+// (host copies of the jobs)
+ClassifyPlan planClassifyJobs(FrameJob* jobs, int numJobs, const DevForest& forest, double splitMaxPixels);
+// the bound below which launches split the trees (CURFIL_CLASSIFY_SPLIT_MPIX or the default)
+double classifySplitPixels();
 void launchClassifyJobs(hipStream_t s, const FrameJob* jobs, int numJobs, const ClassifyPlan& plan,
         const DevForest& forest);
 

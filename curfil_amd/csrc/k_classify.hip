@@ -394,7 +394,8 @@ void dispatchClasses(int C, Launch&& launch) {
     }
 }
 
-// Launches of at most kSplitPixels pixels (CURFIL_CLASSIFY_SPLIT_MPIX overrides it; 0: never) split each
+// Launches of at most kSplitPixels pixels (CURFIL_CLASSIFY_SPLIT_MPIX overrides it, read once per process
+// for single and batched predictions and when a serving loop is created; 0: never) split each
 // tile's trees over the waves of a workgroup, given at least kSplitWaves trees and at most 128 (their
 // leaf numbers, 256 bytes per tree, fill the workgroup's LDS).  Measured with 10 trees of depth 20: one
 // 640x480 frame 0.80 against 0.89 ms to the host; 64 frames (19.7 Mpix) 2.7% slower split.  The bound
@@ -403,11 +404,11 @@ void dispatchClasses(int C, Launch&& launch) {
 // leaves its larger launches to the other kernel, did worse than no split at all.
 constexpr double kSplitPixels = 1.3e6;
 constexpr int kSplitMaxTrees = 128;
-static bool splitLaunch(double pixels, const DevForest& forest) {
-    static const double maxPx = [] {
-        const char* e = std::getenv("CURFIL_CLASSIFY_SPLIT_MPIX");
-        return e ? std::atof(e) * 1e6 : kSplitPixels;
-    }();
+double classifySplitPixels() {
+    const char* e = std::getenv("CURFIL_CLASSIFY_SPLIT_MPIX");
+    return e ? std::atof(e) * 1e6 : kSplitPixels;
+}
+static bool splitLaunch(double pixels, double maxPx, const DevForest& forest) {
     return pixels <= maxPx && forest.numTrees >= kSplitWaves && forest.numTrees <= kSplitMaxTrees;
 }
 static size_t splitLds(const DevForest& forest) {
@@ -416,8 +417,9 @@ static size_t splitLds(const DevForest& forest) {
 
 void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, int numFrames, int W, int H,
         const DevForest& forest, float* probs, uint8_t* labels) {
+    static const double maxPx = classifySplitPixels();   // once per process
     const double pixels = static_cast<double>(numFrames) * W * H;
-    if (splitLaunch(pixels, forest)) {
+    if (splitLaunch(pixels, maxPx, forest)) {
         const dim3 grid = classifyGrid(splitTiles(W, H), numFrames);
         dispatchClasses(forest.C, [&](auto cm) {
             hipLaunchKernelGGL(k_classify_split<decltype(cm)::value>, grid, dim3(64 * kSplitWaves), splitLds(forest),
@@ -432,11 +434,11 @@ void launchClassify(hipStream_t s, const DevImage* imgs, const int* frameIdx, in
     });
 }
 
-ClassifyPlan planClassifyJobs(FrameJob* jobs, int numJobs, const DevForest& forest) {
+ClassifyPlan planClassifyJobs(FrameJob* jobs, int numJobs, const DevForest& forest, double splitMaxPixels) {
     double pixels = 0;
     for (int i = 0; i < numJobs; i++) pixels += static_cast<double>(jobs[i].im.W) * (jobs[i].rowEnd - jobs[i].rowBegin);
     ClassifyPlan plan;
-    plan.split = splitLaunch(pixels, forest);
+    plan.split = splitLaunch(pixels, splitMaxPixels, forest);
     int tiles = 0;
     for (int i = 0; i < numJobs; i++) {
         const int rows = jobs[i].rowEnd - jobs[i].rowBegin;

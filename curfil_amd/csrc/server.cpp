@@ -150,6 +150,7 @@ struct cf_server {
     double bandPx = 0, launchPx = 0;   // row bands of larger frames (0: none); pixels per launch (0: no limit)
     double pendingMax = kMaxPendingPixels;
     double admitMax = kMaxAdmittedPixels;   // pixels of admitted frames not yet launched (0: no limit)
+    double splitPx = 0;                     // launches up to this many pixels split the trees over waves
     // three streams (with the context's own, four: the hardware queues a process gets by default)
     hipStream_t up = nullptr, comp = nullptr, down = nullptr;
     PinnedPool pinned;
@@ -341,7 +342,7 @@ void cf_server::run() {
                     hj[i].pad = 0;
                     if (bd.rowBegin == 0) CF_HIP(hipStreamWaitEvent(cs, r->ready, 0));
                 }
-                const ClassifyPlan plan = planClassifyJobs(hj, n, df);
+                const ClassifyPlan plan = planClassifyJobs(hj, n, df, splitPx);
                 FrameJob* dj = dJobs.get() + static_cast<size_t>(e) * maxBatch;
                 CF_HIP(hipMemcpyAsync(dj, hj, n * sizeof(FrameJob), hipMemcpyHostToDevice, cs));
                 launchClassifyJobs(cs, dj, n, plan, df);
@@ -428,6 +429,7 @@ int cf_server_create(cf_ctx* ctx, cf_forest* fo, double budget_mb, int max_batch
         sv->launchPx = envMpix("CURFIL_SERVE_LAUNCH_MPIX", kLaunchPixels);
         sv->pendingMax = envMpix("CURFIL_SERVE_PENDING_MPIX", kMaxPendingPixels);
         sv->admitMax = envMpix("CURFIL_SERVE_ADMIT_MPIX", kMaxAdmittedPixels);
+        sv->splitPx = classifySplitPixels();
         bindForest(fo, ctx);
         uploadForest(fo);
         CF_HIP(hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking));

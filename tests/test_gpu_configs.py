@@ -201,14 +201,24 @@ def test_dense_forest_prediction_matches_oracle(ctx, full_frames, dense_forest):
 
 
 # ------------------------------------------------------ serving at the workload's frame sizes
-@pytest.mark.parametrize("bands", [None, (0.1, 0.2), (0, 0)])
-def test_serving_at_workload_frame_sizes(ctx, full_frames, dense_forest, bands):
+SERVE_POLICIES = {
+    "default": {},
+    "bands": {"CURFIL_SERVE_BAND_MPIX": 0.1, "CURFIL_SERVE_LAUNCH_MPIX": 0.2},
+    "whole": {"CURFIL_SERVE_BAND_MPIX": 0, "CURFIL_SERVE_LAUNCH_MPIX": 0},
+    "gated_unsplit": {"CURFIL_SERVE_ADMIT_MPIX": 0.1, "CURFIL_CLASSIFY_SPLIT_MPIX": 0},
+}
+
+
+@pytest.mark.parametrize("policy", list(SERVE_POLICIES))
+def test_serving_at_workload_frame_sizes(ctx, full_frames, dense_forest, policy):
     """The serving loop with the dense forest resident and frames of 320x240, 640x480 and 1280x960
     interleaved (with a budget that makes frames wait for memory): each frame's labels and posteriors
     equal its single-frame prediction at full size; the 1280x960 frame (depth sums beyond 2^31 in its
-    integral) also equals the oracle.  The loop's default launch policy (0.35 Mpix row bands, 1.25 Mpix
-    launches); bands above 0.1 Mpix with launches of at most 0.2 Mpix, so bands of one frame run in
-    different launches, between other frames; whole frames in unbounded launches."""
+    integral) also equals the oracle.  Policies: the loop's defaults (0.35 Mpix row bands, 1.25 Mpix
+    launches, every launch by the tree-split kernel); bands above 0.1 Mpix with launches of at most
+    0.2 Mpix, so bands of one frame run in different launches, between other frames; whole frames in
+    unbounded launches; admission gated at 0.1 Mpix (one frame admitted at a time) with every launch
+    by the one-thread-per-pixel kernel."""
     sizes = [(320, 240), (640, 480), (1280, 960)]
     frames, refs = [], []
     big = None
@@ -222,8 +232,7 @@ def test_serving_at_workload_frame_sizes(ctx, full_frames, dense_forest, bands):
             big = (rgb, depth)
     order = [0, 2, 4, 1, 3, 5, 2, 4]
     budget = 1280 * 960 * (5 + 20 + 12 + 8 + 5 * 4 + 1) / (1 << 20) * 1.5
-    env = {} if bands is None else {"CURFIL_SERVE_BAND_MPIX": bands[0], "CURFIL_SERVE_LAUNCH_MPIX": bands[1]}
-    with _Env(**env):
+    with _Env(**SERVE_POLICIES[policy]):
         server = cf.Server(ctx, dense_forest, budget_mb=budget, max_batch=4)
     ids = [server.submit(*frames[j]) for j in order]
     for rid, j in zip(ids, order):
