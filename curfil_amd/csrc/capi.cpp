@@ -216,6 +216,11 @@ int cf_ctx_destroy(cf_ctx* ctx) {
     ctx->scratch0.release();
     ctx->scratch1.release();
     ctx->scratch2.release();
+    if (ctx->copyStream) {
+        (void)hipStreamSynchronize(ctx->copyStream);
+        (void)hipStreamDestroy(ctx->copyStream);
+    }
+    for (hipEvent_t e : ctx->launchDone) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CF_OK;
@@ -879,17 +884,45 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         if (labels) fo->dLabels.resize(static_cast<size_t>(count) * plane);
         CF_HIP(hipMemcpyAsync(fo->dFrames.get(), indices, count * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         const DevForest df = fo->devForest();
-        launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get(), count, im->W, im->H, df,
-                probs ? fo->dProbs.get() : nullptr, labels ? fo->dLabels.get() : nullptr);
-        if (probs) {
-            CF_HIP(hipMemcpyAsync(probs, fo->dProbs.get(), static_cast<size_t>(count) * fo->C * plane * sizeof(float),
-                    hipMemcpyDeviceToHost, ctx->stream));
+        // with posteriors: launches of at most kPredictChunk frames, each one's results copied to the host on
+        // the copy stream while the next launch runs (64 frames of 640x480: 18.8 ms instead of 25.4 ms, the
+        // 8 ms copy no longer after the 16.4 ms launch).  Labels alone (1 byte per pixel) are copied after one
+        // launch: in 8-frame launches they took 17.8 ms instead of 16.9 ms.
+        constexpr int kPredictChunk = 8;
+        const int launches = probs ? (count + kPredictChunk - 1) / kPredictChunk : 1;
+        const int per = (count + launches - 1) / launches;
+        hipStream_t cs = ctx->stream;
+        if (launches > 1) {
+            if (!ctx->copyStream) CF_HIP(hipStreamCreateWithFlags(&ctx->copyStream, hipStreamNonBlocking));
+            while (static_cast<int>(ctx->launchDone.size()) < launches) {
+                hipEvent_t e;
+                CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                ctx->launchDone.push_back(e);
+            }
+            cs = ctx->copyStream;
         }
-        if (labels) {
-            CF_HIP(hipMemcpyAsync(labels, fo->dLabels.get(), static_cast<size_t>(count) * plane, hipMemcpyDeviceToHost,
-                    ctx->stream));
+        for (int k = 0; k < launches; k++) {
+            const int f0 = k * per;
+            const int n = std::min(per, count - f0);
+            if (n <= 0) break;
+            const size_t pOff = static_cast<size_t>(f0) * fo->C * plane, lOff = static_cast<size_t>(f0) * plane;
+            launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get() + f0, n, im->W, im->H, df,
+                    probs ? fo->dProbs.get() + pOff : nullptr, labels ? fo->dLabels.get() + lOff : nullptr);
+            if (cs != ctx->stream) {
+                CF_HIP(hipEventRecord(ctx->launchDone[k], ctx->stream));
+                CF_HIP(hipStreamWaitEvent(cs, ctx->launchDone[k], 0));
+            }
+            if (probs) {
+                CF_HIP(hipMemcpyAsync(probs + pOff, fo->dProbs.get() + pOff,
+                        static_cast<size_t>(n) * fo->C * plane * sizeof(float), hipMemcpyDeviceToHost, cs));
+            }
+            if (labels) {
+                CF_HIP(hipMemcpyAsync(labels + lOff, fo->dLabels.get() + lOff, static_cast<size_t>(n) * plane,
+                        hipMemcpyDeviceToHost, cs));
+            }
         }
-        CF_HIP(hipStreamSynchronize(ctx->stream));
+        CF_HIP(hipStreamSynchronize(cs));
+        if (cs != ctx->stream) CF_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 

@@ -91,6 +91,10 @@ struct ImageSet {
 struct cf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // batched prediction to host memory: results of one launch are copied on this stream while the next
+    // launch runs (created on first use), one event per launch
+    hipStream_t copyStream = nullptr;
+    std::vector<hipEvent_t> launchDone;
     std::string error;
     std::recursive_mutex mu;
     bool profiling = false;
