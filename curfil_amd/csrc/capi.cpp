@@ -184,6 +184,19 @@ void uploadForest(cf_forest* fo) {
 
 cf::ImageSet cf_images::set() const { return cf::ImageSet{col.get(), dep.get(), W, H}; }
 
+namespace {
+// a stream destroyed at scope exit, after its work is done
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+}  // namespace
+
 extern "C" {
 
 int cf_ctx_create(int device, cf_ctx** out) {
@@ -216,10 +229,6 @@ int cf_ctx_destroy(cf_ctx* ctx) {
     ctx->scratch0.release();
     ctx->scratch1.release();
     ctx->scratch2.release();
-    if (ctx->copyStream) {
-        (void)hipStreamSynchronize(ctx->copyStream);
-        (void)hipStreamDestroy(ctx->copyStream);
-    }
     for (hipEvent_t e : ctx->launchDone) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -891,15 +900,20 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         constexpr int kPredictChunk = 8;
         const int launches = probs ? (count + kPredictChunk - 1) / kPredictChunk : 1;
         const int per = (count + launches - 1) / launches;
+        // The copy stream lives for this call only: a stream the process keeps holds one of its hardware queues
+        // (four by default), and a serving loop created afterwards then shared queues between its own streams
+        // (its 640x480 p99 at 2000 frames/s rose from 2.9 to 7.6 ms, and it served 2270 instead of 2990
+        // frames/s at 3000 offered).
+        StreamGuard copy;
         hipStream_t cs = ctx->stream;
         if (launches > 1) {
-            if (!ctx->copyStream) CF_HIP(hipStreamCreateWithFlags(&ctx->copyStream, hipStreamNonBlocking));
+            CF_HIP(hipStreamCreateWithFlags(&copy.s, hipStreamNonBlocking));
             while (static_cast<int>(ctx->launchDone.size()) < launches) {
                 hipEvent_t e;
                 CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 ctx->launchDone.push_back(e);
             }
-            cs = ctx->copyStream;
+            cs = copy.s;
         }
         for (int k = 0; k < launches; k++) {
             const int f0 = k * per;

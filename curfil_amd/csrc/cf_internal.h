@@ -91,9 +91,7 @@ struct ImageSet {
 struct cf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // batched prediction to host memory: results of one launch are copied on this stream while the next
-    // launch runs (created on first use), one event per launch
-    hipStream_t copyStream = nullptr;
+    // batched prediction to host memory: one event per launch, whose results are copied while the next runs
     std::vector<hipEvent_t> launchDone;
     std::string error;
     std::recursive_mutex mu;
