@@ -185,6 +185,38 @@ void uploadForest(cf_forest* fo) {
 cf::ImageSet cf_images::set() const { return cf::ImageSet{col.get(), dep.get(), W, H}; }
 
 namespace {
+// true for page-locked host memory (a copy from the device into it runs as one DMA, asynchronously)
+bool hostPinned(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+// memcpy split over up to 8 host threads (at least 4 MB each)
+void parallelCopy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kMinPart = 4u << 20;
+    const size_t parts = std::max<size_t>(1, std::min<size_t>(8, bytes / kMinPart));
+    const size_t step = (bytes + parts - 1) / parts;
+    std::vector<std::thread> th;
+    for (size_t i = 1; i < parts; i++) {
+        const size_t b = i * step, e = std::min(bytes, b + step);
+        if (b < e) th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b); });
+    }
+    std::memcpy(dst, src, std::min(bytes, step));
+    for (auto& t : th) t.join();
+}
+
+void ensureEvents(std::vector<hipEvent_t>& v, int n) {
+    while (static_cast<int>(v.size()) < n) {
+        hipEvent_t e;
+        CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        v.push_back(e);
+    }
+}
+
 // a stream destroyed at scope exit, after its work is done
 struct StreamGuard {
     hipStream_t s = nullptr;
@@ -230,6 +262,8 @@ int cf_ctx_destroy(cf_ctx* ctx) {
     ctx->scratch1.release();
     ctx->scratch2.release();
     for (hipEvent_t e : ctx->launchDone) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->copyDone) (void)hipEventDestroy(e);
+    if (ctx->hostStage) (void)hipHostFree(ctx->hostStage);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CF_OK;
@@ -908,12 +942,66 @@ int cf_predict_batch(cf_ctx* ctx, cf_forest* fo, cf_images* im, const int* indic
         hipStream_t cs = ctx->stream;
         if (launches > 1) {
             CF_HIP(hipStreamCreateWithFlags(&copy.s, hipStreamNonBlocking));
-            while (static_cast<int>(ctx->launchDone.size()) < launches) {
-                hipEvent_t e;
-                CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-                ctx->launchDone.push_back(e);
-            }
+            ensureEvents(ctx->launchDone, launches);
             cs = copy.s;
+        }
+        // Outputs in pageable memory: the runtime copies into such memory through its own staging, one
+        // thread at 7-8 GB/s, and the call blocks.  Here each launch's results go by DMA into one half of a
+        // page-locked staging buffer, and host threads copy the other half out meanwhile.
+        const bool stageP = launches > 1 && !hostPinned(probs);
+        const bool stageL = stageP && labels && !hostPinned(labels);
+        if (stageP) {
+            const size_t chunkP = static_cast<size_t>(per) * fo->C * plane * sizeof(float);
+            const size_t chunkL = stageL ? static_cast<size_t>(per) * plane : 0;
+            const size_t half = (chunkP + chunkL + 255) & ~static_cast<size_t>(255);
+            if (ctx->hostStageBytes < 2 * half) {
+                if (ctx->hostStage) CF_HIP(hipHostFree(ctx->hostStage));
+                ctx->hostStage = nullptr;
+                ctx->hostStageBytes = 0;
+                CF_HIP(hipHostMalloc(&ctx->hostStage, 2 * half, hipHostMallocDefault));
+                ctx->hostStageBytes = 2 * half;
+            }
+            ensureEvents(ctx->copyDone, launches);
+            for (int k = 0; k < launches; k++) {
+                const int f0 = k * per;
+                const int n = std::min(per, count - f0);
+                const size_t pOff = static_cast<size_t>(f0) * fo->C * plane, lOff = static_cast<size_t>(f0) * plane;
+                launchClassify(ctx->stream, im->desc.get(), fo->dFrames.get() + f0, n, im->W, im->H, df,
+                        fo->dProbs.get() + pOff, labels ? fo->dLabels.get() + lOff : nullptr);
+                CF_HIP(hipEventRecord(ctx->launchDone[k], ctx->stream));
+            }
+            auto stageOf = [&](int k) { return static_cast<char*>(ctx->hostStage) + (k & 1) * half; };
+            auto drain = [&](int k) {   // launch k's results from the staging into the caller's arrays
+                const int f0 = k * per;
+                const int n = std::min(per, count - f0);
+                CF_HIP(hipEventSynchronize(ctx->copyDone[k]));
+                parallelCopy(probs + static_cast<size_t>(f0) * fo->C * plane, stageOf(k),
+                        static_cast<size_t>(n) * fo->C * plane * sizeof(float));
+                if (stageL) parallelCopy(labels + static_cast<size_t>(f0) * plane, stageOf(k) + chunkP, n * plane);
+            };
+            for (int k = 0; k < launches; k++) {
+                const int f0 = k * per;
+                const int n = std::min(per, count - f0);
+                const size_t pOff = static_cast<size_t>(f0) * fo->C * plane, lOff = static_cast<size_t>(f0) * plane;
+                // the half of launch k held launch k - 2's results, drained in the previous round
+                CF_HIP(hipStreamWaitEvent(cs, ctx->launchDone[k], 0));
+                CF_HIP(hipMemcpyAsync(stageOf(k), fo->dProbs.get() + pOff,
+                        static_cast<size_t>(n) * fo->C * plane * sizeof(float), hipMemcpyDeviceToHost, cs));
+                if (stageL) {
+                    CF_HIP(hipMemcpyAsync(stageOf(k) + chunkP, fo->dLabels.get() + lOff, n * plane,
+                            hipMemcpyDeviceToHost, cs));
+                }
+                CF_HIP(hipEventRecord(ctx->copyDone[k], cs));
+                if (labels && !stageL) {
+                    CF_HIP(hipMemcpyAsync(labels + lOff, fo->dLabels.get() + lOff, n * plane, hipMemcpyDeviceToHost,
+                            cs));
+                }
+                if (k > 0) drain(k - 1);
+            }
+            drain(launches - 1);
+            CF_HIP(hipStreamSynchronize(cs));
+            CF_HIP(hipStreamSynchronize(ctx->stream));
+            return;
         }
         for (int k = 0; k < launches; k++) {
             const int f0 = k * per;

@@ -91,8 +91,11 @@ struct ImageSet {
 struct cf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // batched prediction to host memory: one event per launch, whose results are copied while the next runs
-    std::vector<hipEvent_t> launchDone;
+    // batched prediction to host memory: one event per launch, whose results are copied while the next runs,
+    // and one per copy into the page-locked staging (two halves) for outputs in pageable memory
+    std::vector<hipEvent_t> launchDone, copyDone;
+    void* hostStage = nullptr;
+    size_t hostStageBytes = 0;
     std::string error;
     std::recursive_mutex mu;
     bool profiling = false;

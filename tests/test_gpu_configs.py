@@ -173,7 +173,9 @@ def test_dense_forest_prediction_matches_oracle(ctx, full_frames, dense_forest):
     space, Dirichlet(1) leaves over 5 classes) on two 640x480 frames: labels and float posteriors equal the
     oracle's CPU prediction bit for bit, single-frame and batched.  A single frame (0.31 Mpix) goes through
     the kernel that splits each tile's trees over a workgroup's waves; the batch of 5 frames (1.54 Mpix)
-    through the one-thread-per-pixel kernel."""
+    through the one-thread-per-pixel kernel.  A batch of 10 frames runs as two launches whose results are
+    copied while the other runs: into page-locked arrays directly, and into ordinary (pageable) arrays
+    through the page-locked staging halves.  Every copy of a frame equals the oracle."""
     im, ds, _ = full_frames
     trees, posts = [], []
     ones = np.ones(5, np.uint32)
@@ -189,6 +191,11 @@ def test_dense_forest_prediction_matches_oracle(ctx, full_frames, dense_forest):
     assert np.array_equal(posts[3][leaves], post)
     bl, bp = dense_forest.predict_batch(im, [7, 2])
     ql, qp = dense_forest.predict_batch(im, [2, 7, 2, 7, 2])
+    ten = [7, 2] * 5
+    tl, tp = dense_forest.predict_batch(im, ten)
+    pl = cf.pinned_empty(ctx, tl.shape, np.uint8)
+    pp = cf.pinned_empty(ctx, tp.shape, np.float32)
+    dense_forest.predict_batch(im, ten, out=(pl, pp))
     for k, img in enumerate((7, 2)):
         ol, op = ds.predict(img, trees, posts, 5)
         gl, gp = dense_forest.predict(im, img)
@@ -197,6 +204,9 @@ def test_dense_forest_prediction_matches_oracle(ctx, full_frames, dense_forest):
         assert np.array_equal(bl[k], ol) and _same_float_bits(bp[k], op), img
         for q in (1 - k, 3 - k):
             assert np.array_equal(ql[q], ol) and _same_float_bits(qp[q], op), img
+        for q in range(k, 10, 2):
+            assert np.array_equal(tl[q], ol) and _same_float_bits(tp[q], op), (img, q)
+            assert np.array_equal(pl[q], ol) and _same_float_bits(pp[q], op), (img, q)
         print(f"frame {img}: label histogram {np.bincount(ol.ravel(), minlength=5)}")
 
 
