@@ -143,8 +143,11 @@ void bindForest(cf_forest* fo, cf_ctx* ctx) {
     if (fo->ctx != ctx) throw Error(CF_ERR_ARG, "forest belongs to another context");
 }
 
-static_assert(FC_TYPE == P_TYPE && FC_O1X == P_O1X && FC_O2Y == P_O2Y && FC_R1X == P_R1X && FC_R2Y == P_R2Y &&
-                FC_C1 == P_C1 && FC_C2 == P_C2,
+// the packer's feature columns are the feature table's parameter rows
+constexpr bool sameColumn(FeatColumn c, FeatParam p) { return static_cast<int>(c) == static_cast<int>(p); }
+static_assert(sameColumn(FC_TYPE, P_TYPE) && sameColumn(FC_O1X, P_O1X) && sameColumn(FC_O2Y, P_O2Y) &&
+                sameColumn(FC_R1X, P_R1X) && sameColumn(FC_R2Y, P_R2Y) && sameColumn(FC_C1, P_C1) &&
+                sameColumn(FC_C2, P_C2),
         "feature columns");
 static_assert(sizeof(NodeBlock) == kBlockWords * sizeof(uint32_t), "node block size");
 static_assert(postStride(5) == rowStride(5) && postStride(8) == rowStride(8), "posterior row stride");
