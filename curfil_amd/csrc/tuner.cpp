@@ -3,14 +3,19 @@
 // Every shape writes the same bin rows (a shape only changes how a level's
 // (sample, feature) responses are spread over workgroups and lanes), so the
 // tuner measures on real work: while a (threshold width, level size class)
-// key is open, successive pass-1 launches of that key run the candidate
-// shapes in turn and are timed; once every shape has been timed kRounds
-// times on launches of at least kMinResponses responses, the key keeps the
-// shape with the lowest time per response.  Levels below the smallest size
-// class keep the default shape: their launches are too short to time.
+// key is open, successive pass-1 launches of that key alternate between the
+// default shape and each other shape in turn (0, 1, 0, 2, ..., 0, n-1), and
+// each shape's time per response is taken relative to the default launch just
+// before it.  The launches of a key span several levels, whose time per
+// response differs by a few percent; neighbouring launches are alike, so the
+// ratio cancels that.  Once every shape has kRounds ratios (launches of at
+// least kMinResponses responses), the key keeps the shape with the lowest mean
+// ratio (the default's is 1).  Levels below the smallest size class keep the
+// default shape: their launches are too short to time.
 //
 // CURFIL_EVAL_SHAPE=i pins shape i (parity tests run every shape this way;
-// read at every launch), CURFIL_TUNE=0 keeps the default shape, and
+// read at every launch), CURFIL_TUNE=0 keeps the default shape,
+// CURFIL_TUNE_ROUNDS=r sets the ratios per shape, and
 // CURFIL_TUNE_FILE=path loads finished keys from a file and writes them back
 // when a key finishes.
 #include <cstdio>
@@ -27,7 +32,18 @@ namespace cf {
 
 namespace {
 
-constexpr int kRounds = 2;
+// Ratios per shape.  Timing the shapes in turn, without the default between them, chose shape 1 or 8 on
+// the 800-frame workload (17.57 and 17.50 s per step) where shape 7 pinned took 17.41 s: the launch order
+// and the levels' differences decided more than the shapes did.  With ratios, 3 rounds chose shape 3 in
+// three runs out of three (17.48 s per step); 6 rounds, which reach further down the tree, chose shape 7
+// in two runs out of two (17.44 s).
+constexpr int kRounds = 6;
+// CURFIL_TUNE_ROUNDS=r overrides the ratios per shape (tests on small workloads, which have few launches)
+int tuneRounds() {
+    const char* v = std::getenv("CURFIL_TUNE_ROUNDS");
+    const int r = v ? std::atoi(v) : 0;
+    return r > 0 ? r : kRounds;
+}
 constexpr double kMinResponses = 5e7;
 
 // size classes of a level by permutation entries: < 2^20 (not tuned), < 2^23, < 2^25, larger
@@ -39,10 +55,13 @@ int sizeClass(size_t entries) {
 }
 
 struct KeyState {
-    std::vector<double> ms, responses;
-    std::vector<int> timed;
-    int next = 0;        // next shape to time
-    int best = -1;       // >= 0 once finished
+    std::vector<double> ms, responses;   // per shape, for the report
+    std::vector<double> ratioSum;        // per shape: time per response over the preceding default launch's
+    std::vector<int> ratios, timed;
+    int cand = 1;              // the shape the next candidate launch runs
+    bool defaultNext = true;   // the next launch runs the default shape
+    double lastDefault = -1;   // time per response of the last default launch
+    int best = -1;             // >= 0 once finished
 };
 
 class Tuner {
@@ -64,7 +83,7 @@ public:
         KeyState& k = state(TM, cls);
         if (k.best >= 0) return k.best;
         *measure = true;
-        return k.next;
+        return k.defaultNext ? 0 : k.cand;
     }
 
     void record(int TM, size_t entries, int shape, double ms, double responses) {
@@ -74,24 +93,41 @@ public:
         std::lock_guard<std::mutex> g(mu_);
         KeyState& k = state(TM, cls);
         if (k.best >= 0) return;
+        const int n = numEvalShapes();
         k.ms[shape] += ms;
         k.responses[shape] += responses;
         k.timed[shape]++;
-        const int n = numEvalShapes();
-        k.next = (shape + 1) % n;
-        for (int i = 0; i < n; i++) {
-            if (k.timed[i] < kRounds) return;
+        const double per = ms / responses;
+        if (shape == 0) {
+            k.lastDefault = per;
+            if (n > 1) k.defaultNext = false;
+        } else if (shape == k.cand && !k.defaultNext) {
+            if (k.lastDefault > 0) {
+                k.ratioSum[shape] += per / k.lastDefault;
+                k.ratios[shape]++;
+            }
+            k.cand = k.cand + 1 < n ? k.cand + 1 : 1;
+            k.defaultNext = true;
+        }
+        const int rounds = tuneRounds();
+        for (int i = 1; i < n; i++) {
+            if (k.ratios[i] < rounds) return;
         }
         int best = 0;
+        double bestRatio = 1.0;
         for (int i = 1; i < n; i++) {
-            if (k.ms[i] / k.responses[i] < k.ms[best] / k.responses[best]) best = i;
+            const double r = k.ratioSum[i] / k.ratios[i];
+            if (r < bestRatio) {
+                best = i;
+                bestRatio = r;
+            }
         }
         k.best = best;
         save();
     }
 
-    // per key: TM, size class, chosen shape (-1: open), ns per response of the chosen and of the default
-    // shape, launches timed
+    // per key: TM, size class, chosen shape (-1: open), ns per response of the chosen shape (estimated from
+    // its mean ratio) and of the default shape, launches timed
     std::vector<double> table() {
         std::lock_guard<std::mutex> g(mu_);
         std::vector<double> out;
@@ -99,10 +135,13 @@ public:
             const KeyState& k = kv.second;
             int timed = 0;
             for (int v : k.timed) timed += v;
-            const auto nsPer = [&](int i) { return k.responses[i] > 0 ? k.ms[i] * 1e6 / k.responses[i] : 0.0; };
+            const double ns0 = k.responses[0] > 0 ? k.ms[0] * 1e6 / k.responses[0] : 0.0;
+            // the chosen shape on the default's launches: the default's time times the chosen shape's mean ratio
+            double nsBest = 0.0;
+            if (k.best == 0) nsBest = ns0;
+            if (k.best > 0 && k.ratios[k.best] > 0) nsBest = ns0 * k.ratioSum[k.best] / k.ratios[k.best];
             out.insert(out.end(), {static_cast<double>(kv.first.first), static_cast<double>(kv.first.second),
-                    static_cast<double>(k.best), k.best >= 0 ? nsPer(k.best) : 0.0, nsPer(0),
-                    static_cast<double>(timed)});
+                    static_cast<double>(k.best), nsBest, ns0, static_cast<double>(timed)});
         }
         return out;
     }
@@ -145,6 +184,8 @@ private:
             const int n = numEvalShapes();
             k.ms.assign(n, 0.0);
             k.responses.assign(n, 0.0);
+            k.ratioSum.assign(n, 0.0);
+            k.ratios.assign(n, 0);
             k.timed.assign(n, 0);
         }
         return k;

@@ -205,13 +205,17 @@ int cf_forest_profile(const cf_forest* forest, double* out, int n);
  * random_tree_image_gpu.cu:2034-2065, 2095-2108).  A shape: lanes per
  * workgroup, samples per lane, features whose gathers are issued together,
  * workgroups per CU.  Every shape gives the same histograms; the tuner times
- * them on the first launches of each (threshold width, level size class)
- * and keeps the fastest (CURFIL_EVAL_SHAPE=i pins shape i, CURFIL_TUNE=0
- * keeps shape 0, CURFIL_TUNE_FILE=path keeps the choices between runs). */
+ * them on the first launches of each (threshold width, level size class),
+ * each shape's launch right after a launch of shape 0, and keeps the shape
+ * with the lowest mean ratio to it (CURFIL_EVAL_SHAPE=i pins shape i,
+ * CURFIL_TUNE=0 keeps shape 0, CURFIL_TUNE_ROUNDS=r sets the ratios per
+ * shape, CURFIL_TUNE_FILE=path keeps the choices between runs). */
 int cf_eval_shape_info(int shape, int* lanes, int* per_lane, int* features_per_step, int* workgroups_per_cu);
 /* per tuned key, 6 values: threshold width, size class, chosen shape (-1 while
- * open), ns per response of the chosen shape and of shape 0, launches timed;
- * returns the number of keys */
+ * open), ns per response of shape 0 on its timed launches and of the chosen
+ * shape estimated as that times the chosen shape's mean ratio to shape 0
+ * (in that order: chosen, shape 0), launches timed; returns the number of
+ * keys */
 int cf_eval_tuning(double* out, int n);
 int cf_eval_tuning_reset(void);
 /* one tree from given root samples (RandomTreeImage::doTrain,
