@@ -292,11 +292,15 @@ def eval_roofline(profiles, N, W, H, T):
 
 def full_frame_leg(ctx, images, full_kw, N, W, H):
     """The plain full-frame workload on the same frames (3 trees to depth 18, F = 2000 x T = 20), trained
-    batched: one warm-up and one timed step."""
+    batched: warm-up steps until the launch-shape tuner has settled this workload's keys (its 85 pass-1
+    launches per step are fewer than the tuner's 108), then one timed step."""
     import curfil_amd as cf
     cfg = cf.TrainingConfiguration(**full_kw)
     f = cf.RandomForestImage.sample(ctx, images, cfg, 3, num_labels=5)
-    f.grow(cfg, cf.MODE_BATCHED)
+    for _ in range(3):
+        f.grow(cfg, cf.MODE_BATCHED)
+        if all(r["shape"] >= 0 for r in cf.eval_tuning()):
+            break
     ctx.set_profiling(True)
     ctx.synchronize()
     t0 = time.perf_counter()
